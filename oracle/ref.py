@@ -1,0 +1,139 @@
+"""ctypes wrapper over oracle/_ref/libducc_ref.so: the reference ducc0 NUFFT
+compiled from its own sources (oracle/Makefile target `ref`).
+
+TEST INFRASTRUCTURE ONLY.  `available()` is False where the library was not
+built (it needs the ducc0 sources at build time); callers skip then.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_ref", "libducc_ref.so")
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ct.CDLL(LIB)
+        P, I, D, S = ct.c_void_p, ct.c_int, ct.c_double, ct.c_size_t
+        L.ref_last_error.restype = ct.c_char_p
+        L.ref_vlen.restype = I
+        L.ref_vlen.argtypes = [I]
+        L.ref_best_epsilon.restype = D
+        L.ref_best_epsilon.argtypes = [I, I, D, D]
+        L.ref_find_params.restype = I
+        L.ref_find_params.argtypes = [D, D, D, I, P, S, I, I, S, P]
+        L.ref_kernel_coeffs.argtypes = [I, P, P]
+        L.ref_corfac.argtypes = [I, S, D, P]
+        L.ref_sort_index.argtypes = [I, S, P, P, P, D, D, D, I, I, S, P, P, P]
+        L.ref_nu2u.argtypes = [I, S, P, I, P, I, I, D, S, P, P, D, D, P, I]
+        L.ref_u2nu.argtypes = L.ref_nu2u.argtypes
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def _check(rc):
+    if rc:
+        raise RuntimeError(lib().ref_last_error().decode())
+
+
+def vlen(single):
+    return lib().ref_vlen(int(single))
+
+
+def best_epsilon(ndim, single, sigma_min=1.1, sigma_max=2.6):
+    r = lib().ref_best_epsilon(ndim, int(single), sigma_min, sigma_max)
+    if r < 0:
+        raise RuntimeError(lib().ref_last_error().decode())
+    return r
+
+
+def find_params(eps, shape, npoints, gridding, single, nthreads,
+                sigma_min=1.2, sigma_max=2.51):
+    dims = np.asarray(shape, np.uintp)
+    nover = np.zeros(3, np.uintp)
+    k = lib().ref_find_params(eps, sigma_min, sigma_max, len(shape), _p(dims),
+                              npoints, int(gridding), int(single), nthreads,
+                              _p(nover))
+    if k < 0:
+        raise RuntimeError(lib().ref_last_error().decode())
+    return k, tuple(int(x) for x in nover[:len(shape)])
+
+
+def kernel_coeffs(kidx, W):
+    c = np.zeros(64 * 20)
+    D = np.zeros(1, np.int32)
+    _check(lib().ref_kernel_coeffs(kidx, _p(c), _p(D)))
+    d = int(D[0])
+    return c[:(d + 1) * W].reshape(d + 1, W)
+
+
+def corfac(kidx, n, dx):
+    out = np.zeros(n)
+    _check(lib().ref_corfac(kidx, n, dx, _p(out)))
+    return out
+
+
+def sort_index(coords, shape, eps, gridding, single, nthreads=1,
+               period=2 * np.pi, sigma_min=1.2, sigma_max=2.51):
+    """coord_idx of a reference plan -> (idx, nover, supp)"""
+    c = np.ascontiguousarray(coords, np.float64)
+    M, d = c.shape
+    idx = np.zeros(M, np.uint32)
+    nover = np.zeros(3, np.uintp)
+    supp = np.zeros(1, np.int32)
+    per = np.ascontiguousarray(np.broadcast_to(np.asarray(period, float),
+                                               (d,)))
+    _check(lib().ref_sort_index(d, M, _p(c), _p(np.asarray(shape, np.uintp)),
+                                _p(per), eps, sigma_min, sigma_max,
+                                int(gridding), int(single), nthreads, _p(idx),
+                                _p(nover), _p(supp)))
+    return idx, tuple(int(x) for x in nover[:d]), int(supp[0])
+
+
+def _prep(coords, vals, period):
+    coord_single = coords.dtype == np.float32
+    c = np.ascontiguousarray(coords,
+                             np.float32 if coord_single else np.float64)
+    single = vals.dtype == np.complex64
+    v = np.ascontiguousarray(vals, np.complex64 if single else np.complex128)
+    d = c.shape[1]
+    per = np.ascontiguousarray(np.broadcast_to(np.asarray(period, float),
+                                               (d,)))
+    return c, int(coord_single), v, int(single), per
+
+
+def nu2u(points, coord, forward, epsilon, out_shape, nthreads=1,
+         sigma_min=1.2, sigma_max=2.51, periodicity=2 * np.pi,
+         fft_order=False):
+    """ducc0.nufft.nu2u semantics (python/nufft_pymod.cc:96-137)"""
+    c, cs, v, s, per = _prep(coord, points, periodicity)
+    out = np.zeros(out_shape, v.dtype)
+    _check(lib().ref_nu2u(c.shape[1], c.shape[0], _p(c), cs, _p(v), s,
+                          int(forward), epsilon, nthreads,
+                          _p(np.asarray(out_shape, np.uintp)), _p(out),
+                          sigma_min, sigma_max, _p(per), int(fft_order)))
+    return out
+
+
+def u2nu(grid, coord, forward, epsilon, nthreads=1, sigma_min=1.2,
+         sigma_max=2.51, periodicity=2 * np.pi, fft_order=False):
+    """ducc0.nufft.u2nu semantics (python/nufft_pymod.cc:52-94)"""
+    c, cs, v, s, per = _prep(coord, grid, periodicity)
+    out = np.zeros(c.shape[0], v.dtype)
+    _check(lib().ref_u2nu(c.shape[1], c.shape[0], _p(c), cs, _p(v), s,
+                          int(forward), epsilon, nthreads,
+                          _p(np.asarray(v.shape, np.uintp)), _p(out),
+                          sigma_min, sigma_max, _p(per), int(fft_order)))
+    return out

@@ -1,0 +1,212 @@
+// ref_driver.cc -- thin extern "C" shim over the reference ducc0 NUFFT,
+// compiled from the ducc0 sources in place (see oracle/Makefile, target
+// `ref`) into oracle/_ref/libducc_ref.so.
+//
+// TEST INFRASTRUCTURE ONLY: it pins the CPU restatement in oracle.c, writes
+// golden fixtures, and is the timed CPU baseline.  Nothing in the product
+// links it.  Entry points mirror the ducc0 calls they wrap:
+//   nu2u / u2nu free functions      src/ducc0/nufft/nufft.h:1896-1958
+//   findNufftParameters             src/ducc0/nufft/nufft.h:155-208
+//   selectKernel / Coeff / corfunc  src/ducc0/math/gridding_kernel.{h,cc}
+//   bestEpsilon                     src/ducc0/math/gridding_kernel.cc:1562
+//   build_index (coord_idx order)   src/ducc0/nufft/nufft.h:892-907,1364,1862
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <vector>
+
+#include "ducc0/nufft/nufft.h"
+#include "ducc0/fft/fftnd_impl.h"
+
+using namespace ducc0;
+using std::complex;
+using std::size_t;
+using std::vector;
+
+namespace {
+thread_local char g_err[1024];
+template <typename F> int guard(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (std::exception &e) {
+    std::snprintf(g_err, sizeof(g_err), "%s", e.what());
+    return 1;
+  }
+}
+
+// exposes the protected coord_idx / nover / supp of a reference plan
+template <typename T, typename Tc, size_t nd>
+struct Peek : public Nufft<T, T, Tc, nd> {
+  using Base = Nufft<T, T, Tc, nd>;
+  using Anc = detail_nufft::Nufft_ancestor<T, T, nd>;
+  using Base::Base;
+  void get(uint32_t *idx, size_t *nover, int *supp) const {
+    const auto &ci = this->Anc::coord_idx;
+    for (size_t i = 0; i < ci.size(); ++i) idx[i] = ci[i];
+    for (size_t d = 0; d < nd; ++d) nover[d] = this->Anc::nover[d];
+    *supp = int(this->Anc::supp);
+  }
+};
+
+template <typename T, typename Tc, size_t nd>
+void sort_index(size_t M, const Tc *coords, const size_t *nuni,
+                const double *period, double eps, double smin, double smax,
+                int gridding, size_t nthreads, uint32_t *idx, size_t *nover,
+                int *supp) {
+  cmav<Tc, 2> c(coords, {M, nd});
+  std::array<size_t, nd> shp;
+  for (size_t d = 0; d < nd; ++d) shp[d] = nuni[d];
+  vector<double> per(period, period + nd);
+  Peek<T, Tc, nd> p(gridding != 0, c, shp, eps, nthreads, smin, smax, per,
+                    false);
+  p.get(idx, nover, supp);
+}
+
+template <typename T, typename Tc>
+void do_nu2u(int ndim, size_t M, const Tc *coords, const complex<T> *pts,
+             int forward, double eps, size_t nthreads, const size_t *nuni,
+             complex<T> *out, double smin, double smax, const double *period,
+             int fft_order) {
+  cmav<Tc, 2> c(coords, {M, size_t(ndim)});
+  cmav<complex<T>, 1> p(pts, {M});
+  vector<size_t> shp(nuni, nuni + ndim);
+  vfmav<complex<T>> o(out, shp);
+  vector<double> per(period, period + ndim);
+  nu2u<T, T>(c, p, forward != 0, eps, nthreads, o, 0, smin, smax, per,
+             fft_order != 0);
+}
+
+template <typename T, typename Tc>
+void do_u2nu(int ndim, size_t M, const Tc *coords, const complex<T> *grid,
+             int forward, double eps, size_t nthreads, const size_t *nuni,
+             complex<T> *out, double smin, double smax, const double *period,
+             int fft_order) {
+  cmav<Tc, 2> c(coords, {M, size_t(ndim)});
+  vector<size_t> shp(nuni, nuni + ndim);
+  cfmav<complex<T>> g(grid, shp);
+  vmav<complex<T>, 1> o(out, {M});
+  vector<double> per(period, period + ndim);
+  u2nu<T, T>(c, g, forward != 0, eps, nthreads, o, 0, smin, smax, per,
+             fft_order != 0);
+}
+}  // namespace
+
+extern "C" {
+
+const char *ref_last_error() { return g_err; }
+
+int ref_vlen(int single) {
+  return single ? int(detail_nufft::mysimd<float>::size())
+                : int(detail_nufft::mysimd<double>::size());
+}
+
+double ref_best_epsilon(int ndim, int single, double smin, double smax) {
+  double r = -1;
+  if (guard([&] { r = bestEpsilon(size_t(ndim), single != 0, smin, smax); }))
+    return -1;
+  return r;
+}
+
+int ref_find_params(double eps, double smin, double smax, int ndim,
+                    const size_t *dims, size_t npoints, int gridding,
+                    int single, size_t nthreads, size_t *nover) {
+  int res = -1;
+  int rc = guard([&] {
+    vector<size_t> d(dims, dims + ndim);
+    size_t kidx;
+    vector<size_t> bd;
+    if (single) {
+      auto t = detail_nufft::findNufftParameters<float, float>(
+          eps, smin, smax, d, npoints, gridding != 0, nthreads);
+      kidx = std::get<0>(t); bd = std::get<1>(t);
+    } else {
+      auto t = detail_nufft::findNufftParameters<double, double>(
+          eps, smin, smax, d, npoints, gridding != 0, nthreads);
+      kidx = std::get<0>(t); bd = std::get<1>(t);
+    }
+    for (int i = 0; i < ndim; ++i) nover[i] = bd[i];
+    res = int(kidx);
+  });
+  return rc ? -1 : res;
+}
+
+// coefficients of KernelDB row kidx: W*(D+1) doubles, D returned
+int ref_kernel_coeffs(int kidx, double *coeff, int *D) {
+  return guard([&] {
+    auto k = selectKernel(size_t(kidx));
+    const auto &c = k->Coeff();
+    for (size_t i = 0; i < c.size(); ++i) coeff[i] = c[i];
+    *D = int(k->degree());
+  });
+}
+
+int ref_corfac(int kidx, size_t n, double dx, double *out) {
+  return guard([&] {
+    auto k = selectKernel(size_t(kidx));
+    auto v = k->corfunc(n, dx, 1);
+    for (size_t i = 0; i < n; ++i) out[i] = v[i];
+  });
+}
+
+// processing order of a reference plan (coords float64, values per `single`)
+int ref_sort_index(int ndim, size_t M, const double *coords,
+                   const size_t *nuni, const double *period, double eps,
+                   double smin, double smax, int gridding, int single,
+                   size_t nthreads, uint32_t *idx, size_t *nover, int *supp) {
+  return guard([&] {
+#define REF_SORT(T, ND) \
+  sort_index<T, double, ND>(M, coords, nuni, period, eps, smin, smax, \
+                            gridding, nthreads, idx, nover, supp)
+    if (single) {
+      if (ndim == 1) REF_SORT(float, 1);
+      else if (ndim == 2) REF_SORT(float, 2);
+      else REF_SORT(float, 3);
+    } else {
+      if (ndim == 1) REF_SORT(double, 1);
+      else if (ndim == 2) REF_SORT(double, 2);
+      else REF_SORT(double, 3);
+    }
+#undef REF_SORT
+  });
+}
+
+// value dtype: single!=0 -> complex64, else complex128;
+// coord dtype: coord_single!=0 -> float32, else float64.
+int ref_nu2u(int ndim, size_t M, const void *coords, int coord_single,
+             const void *points, int single, int forward, double eps,
+             size_t nthreads, const size_t *nuni, void *out, double smin,
+             double smax, const double *period, int fft_order) {
+  return guard([&] {
+#define REF_CALL(T, TC)                                                       \
+  do_nu2u<T, TC>(ndim, M, (const TC *)coords, (const complex<T> *)points,     \
+                 forward, eps, nthreads, nuni, (complex<T> *)out, smin, smax, \
+                 period, fft_order)
+    if (single) {
+      if (coord_single) REF_CALL(float, float); else REF_CALL(float, double);
+    } else {
+      if (coord_single) REF_CALL(double, float); else REF_CALL(double, double);
+    }
+#undef REF_CALL
+  });
+}
+
+int ref_u2nu(int ndim, size_t M, const void *coords, int coord_single,
+             const void *grid, int single, int forward, double eps,
+             size_t nthreads, const size_t *nuni, void *out, double smin,
+             double smax, const double *period, int fft_order) {
+  return guard([&] {
+#define REF_CALL(T, TC)                                                       \
+  do_u2nu<T, TC>(ndim, M, (const TC *)coords, (const complex<T> *)grid,       \
+                 forward, eps, nthreads, nuni, (complex<T> *)out, smin, smax, \
+                 period, fft_order)
+    if (single) {
+      if (coord_single) REF_CALL(float, float); else REF_CALL(float, double);
+    } else {
+      if (coord_single) REF_CALL(double, float); else REF_CALL(double, double);
+    }
+#undef REF_CALL
+  });
+}
+
+}  // extern "C"
