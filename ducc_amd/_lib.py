@@ -1,0 +1,172 @@
+"""ctypes binding of libducc_amd.so (the C ABI in include/ducc_amd.h).
+
+The library is built in-tree by ducc_amd/csrc/build.py.  There is no CPU
+fallback: a missing library or a missing GPU raises.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libducc_amd.so")
+
+DMX_F32, DMX_F64, DMX_C64, DMX_C128, DMX_U32 = 3, 7, 67, 71, 34
+
+_NP_TC = {np.dtype(np.float32): DMX_F32, np.dtype(np.float64): DMX_F64,
+          np.dtype(np.complex64): DMX_C64, np.dtype(np.complex128): DMX_C128,
+          np.dtype(np.uint32): DMX_U32}
+
+
+class DmxArray(ct.Structure):
+    _fields_ = [("data", ct.c_void_p), ("ndim", ct.c_int64),
+                ("shape", ct.c_int64 * 10), ("stride", ct.c_int64 * 10),
+                ("typecode", ct.c_int32), ("device", ct.c_int32)]
+
+
+class DmxOptions(ct.Structure):
+    _fields_ = [("precision", ct.c_int32), ("chooser", ct.c_int32),
+                ("log2tile", ct.c_int32), ("kernel_idx", ct.c_int32),
+                ("fp64_below", ct.c_double), ("stream", ct.c_void_p),
+                ("sigma", ct.c_double), ("reserved", ct.c_int32 * 8)]
+
+
+class DmxPlanInfo(ct.Structure):
+    _fields_ = [("ndim", ct.c_int32), ("W", ct.c_int32),
+                ("kernel_idx", ct.c_int32), ("fp64", ct.c_int32),
+                ("log2tile", ct.c_int32), ("chooser_used", ct.c_int32),
+                ("nuni", ct.c_int64 * 3), ("nover", ct.c_int64 * 3),
+                ("ofactor", ct.c_double), ("eps_tab", ct.c_double),
+                ("epsilon", ct.c_double), ("npoints", ct.c_int64)]
+
+    def as_dict(self):
+        nd = self.ndim
+        return dict(ndim=nd, W=self.W, kernel_idx=self.kernel_idx,
+                    fp64=bool(self.fp64), log2tile=self.log2tile,
+                    chooser_used=bool(self.chooser_used),
+                    nuni=tuple(self.nuni[:nd]), nover=tuple(self.nover[:nd]),
+                    ofactor=self.ofactor, eps_tab=self.eps_tab,
+                    epsilon=self.epsilon, npoints=self.npoints)
+
+
+# exported names, each with (restype, argtypes)
+_P = ct.c_void_p
+_AP = ct.POINTER(DmxArray)
+_SIG = {
+    "dmx_last_error": (ct.c_char_p, []),
+    "dmx_default_options": (None, [ct.POINTER(DmxOptions)]),
+    "dmx_best_epsilon": (ct.c_double, [ct.c_size_t, ct.c_int, ct.c_double,
+                                       ct.c_double]),
+    "dmx_u2nu": (ct.c_int, [_AP, _AP, ct.c_int, ct.c_double, ct.c_size_t, _AP,
+                            ct.c_size_t, ct.c_double, ct.c_double, _P,
+                            ct.c_int, ct.POINTER(DmxOptions),
+                            ct.POINTER(DmxPlanInfo)]),
+    "dmx_nu2u": (ct.c_int, [_AP, _AP, ct.c_int, ct.c_double, ct.c_size_t, _AP,
+                            ct.c_size_t, ct.c_double, ct.c_double, _P,
+                            ct.c_int, ct.POINTER(DmxOptions),
+                            ct.POINTER(DmxPlanInfo)]),
+    "dmx_plan_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, _AP, _P,
+                                   ct.c_size_t, ct.c_double, ct.c_size_t,
+                                   ct.c_double, ct.c_double, _P, ct.c_int,
+                                   ct.POINTER(DmxOptions)]),
+    "dmx_plan_nu2u": (ct.c_int, [_P, ct.c_int, ct.c_size_t, _AP, _AP]),
+    "dmx_plan_u2nu": (ct.c_int, [_P, ct.c_int, ct.c_size_t, _AP, _AP]),
+    "dmx_plan_get_info": (ct.c_int, [_P, ct.POINTER(DmxPlanInfo)]),
+    "dmx_plan_destroy": (ct.c_int, [_P]),
+    "dmx_u2nu_coord_grad": (ct.c_int, [_AP, _AP, ct.c_int, ct.c_double, _AP,
+                                       ct.c_double, ct.c_double, _P, ct.c_int,
+                                       ct.POINTER(DmxOptions)]),
+    "dmx_nudft": (ct.c_int, [ct.c_int, _AP, _AP, _P, ct.c_size_t, ct.c_int,
+                             _P, ct.c_int, _P, ct.c_size_t, _AP]),
+    "dmx_sort_points": (ct.c_int, [_AP, _P, ct.c_size_t, ct.c_int, ct.c_int,
+                                   _P, _AP, _AP]),
+    "dmx_device_count": (ct.c_int, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(
+                f"{LIBPATH} is missing: build it with "
+                "`python ducc_amd/csrc/build.py` (there is no CPU fallback)")
+        L = ct.CDLL(LIBPATH)
+        for name, (res, args) in _SIG.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_names():
+    return list(_SIG)
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().dmx_last_error().decode())
+
+
+def require_gpu():
+    if lib().dmx_device_count() < 1:
+        raise RuntimeError("ducc_amd needs a HIP device (none visible); "
+                           "there is no CPU fallback")
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def describe(x):
+    """dmx_array of a numpy array (host) or a torch tensor (device or host).
+    Returns (descriptor, keepalive)."""
+    a = DmxArray()
+    if _is_torch(x):
+        import torch
+        tmap = {torch.float32: DMX_F32, torch.float64: DMX_F64,
+                torch.complex64: DMX_C64, torch.complex128: DMX_C128}
+        if x.dtype not in tmap:
+            raise RuntimeError("not yet supported")
+        a.data = x.data_ptr() if x.numel() else None
+        a.ndim = x.dim()
+        for i in range(x.dim()):
+            a.shape[i] = x.shape[i]
+            a.stride[i] = x.stride(i)
+        a.typecode = tmap[x.dtype]
+        a.device = 1 if x.is_cuda else 0
+        return a, x
+    x = np.asarray(x)
+    if x.dtype not in _NP_TC:
+        raise RuntimeError("not yet supported")
+    isz = x.dtype.itemsize
+    for s in x.strides:
+        if s % isz:
+            x = np.ascontiguousarray(x)
+            break
+    a.data = x.ctypes.data if x.size else None
+    a.ndim = x.ndim
+    for i in range(x.ndim):
+        a.shape[i] = x.shape[i]
+        a.stride[i] = x.strides[i] // isz
+    a.typecode = _NP_TC[x.dtype]
+    a.device = 0
+    return a, x
+
+
+def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
+            fp64_below=1e-6, stream=None, sigma=0.0):
+    o = DmxOptions()
+    lib().dmx_default_options(ct.byref(o))
+    o.precision = {"auto": 0, "fp32": 1, "fp64": 2}[precision]
+    o.chooser = {"analytic": 0, "learned": 1}[chooser]
+    o.log2tile = int(log2tile)
+    o.kernel_idx = int(kernel_idx)
+    o.fp64_below = float(fp64_below)
+    o.stream = stream
+    o.sigma = float(sigma)
+    return o

@@ -1,0 +1,541 @@
+// api.hip -- the extern "C" boundary (include/ducc_amd.h).  Validates every
+// argument on the host before any GPU work (messages follow ducc0's
+// MR_assert texts, nufft.h:342-343, 400, 409, 1904-1905,
+// gridding_kernel.h:442, nufft_pymod.cc:93), stages host arrays through the
+// device, runs the plan on one HIP stream and returns synchronously.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ducc_amd.h"
+#include "chooser.h"
+#include "grid.h"
+#include "params.h"
+#include "runtime.h"
+
+namespace dmx {
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F> int guard(F &&f) {
+  try {
+    f();
+    g_err.clear();
+    return 0;
+  } catch (std::exception &e) {
+    g_err = e.what();
+    (void)hipGetLastError();
+    return 1;
+  } catch (...) {
+    g_err = "unknown error";
+    return 1;
+  }
+}
+
+size_t elsize(int tc) {
+  switch (tc) {
+    case DMX_F32: return 4;
+    case DMX_F64: return 8;
+    case DMX_C64: return 8;
+    case DMX_C128: return 16;
+    case DMX_U32: return 4;
+  }
+  throw Error("not yet supported");
+}
+
+int64_t nelem(const dmx_array &a) {
+  int64_t n = 1;
+  for (int i = 0; i < a.ndim; ++i) n *= a.shape[i];
+  return n;
+}
+
+bool contiguous(const dmx_array &a) {
+  int64_t s = 1;
+  for (int i = (int)a.ndim - 1; i >= 0; --i) {
+    if (a.shape[i] != 1 && a.stride[i] != s) return false;
+    s *= a.shape[i];
+  }
+  return true;
+}
+
+// copy a strided host array into a contiguous host buffer (and back)
+void pack(const dmx_array &a, void *dst) {
+  size_t es = elsize(a.typecode);
+  int64_t n = nelem(a);
+  if (contiguous(a)) {
+    std::memcpy(dst, a.data, (size_t)n * es);
+    return;
+  }
+  std::vector<int64_t> idx(a.ndim, 0);
+  const char *src = (const char *)a.data;
+  char *d = (char *)dst;
+  for (int64_t q = 0; q < n; ++q) {
+    int64_t off = 0;
+    for (int i = 0; i < a.ndim; ++i) off += idx[i] * a.stride[i];
+    std::memcpy(d + q * es, src + off * (int64_t)es, es);
+    for (int i = (int)a.ndim - 1; i >= 0; --i) {
+      if (++idx[i] < a.shape[i]) break;
+      idx[i] = 0;
+    }
+  }
+}
+void unpack(const void *srcv, const dmx_array &a) {
+  size_t es = elsize(a.typecode);
+  int64_t n = nelem(a);
+  if (contiguous(a)) {
+    std::memcpy(a.data, srcv, (size_t)n * es);
+    return;
+  }
+  std::vector<int64_t> idx(a.ndim, 0);
+  const char *s = (const char *)srcv;
+  char *dst = (char *)a.data;
+  for (int64_t q = 0; q < n; ++q) {
+    int64_t off = 0;
+    for (int i = 0; i < a.ndim; ++i) off += idx[i] * a.stride[i];
+    std::memcpy(dst + off * (int64_t)es, s + q * es, es);
+    for (int i = (int)a.ndim - 1; i >= 0; --i) {
+      if (++idx[i] < a.shape[i]) break;
+      idx[i] = 0;
+    }
+  }
+}
+
+// An input array on the device.  Host arrays are packed and copied in;
+// device arrays are used in place (contiguity required unless `strided_ok`).
+struct DevIn {
+  const void *p = nullptr;
+  Buf tmp;
+  int64_t st[10] = {0};
+  DevIn(const dmx_array &a, hipStream_t s, bool strided_ok) {
+    size_t es = elsize(a.typecode);
+    int64_t n = nelem(a);
+    if (a.device) {
+      p = a.data;
+      if (!strided_ok)
+        DMX_ASSERT(contiguous(a), "device arrays must be C-contiguous here");
+      for (int i = 0; i < a.ndim; ++i) st[i] = a.stride[i];
+      return;
+    }
+    int64_t s1 = 1;
+    for (int i = (int)a.ndim - 1; i >= 0; --i) {
+      st[i] = s1;
+      s1 *= a.shape[i];
+    }
+    if (n == 0) return;
+    std::vector<char> h((size_t)n * es);
+    pack(a, h.data());
+    tmp.reset((size_t)n * es);
+    DMX_HIP(hipMemcpyAsync(tmp.p, h.data(), h.size(), hipMemcpyHostToDevice, s));
+    DMX_HIP(hipStreamSynchronize(s));
+    p = tmp.p;
+  }
+};
+
+// An output array on the device; host outputs are written back by finish().
+struct DevOut {
+  void *p = nullptr;
+  Buf tmp;
+  const dmx_array *a;
+  int64_t st[10] = {0};
+  DevOut(const dmx_array &arr, bool strided_ok) : a(&arr) {
+    size_t es = elsize(arr.typecode);
+    int64_t n = nelem(arr);
+    if (arr.device) {
+      p = arr.data;
+      if (!strided_ok)
+        DMX_ASSERT(contiguous(arr), "device arrays must be C-contiguous here");
+      for (int i = 0; i < arr.ndim; ++i) st[i] = arr.stride[i];
+      return;
+    }
+    int64_t s1 = 1;
+    for (int i = (int)arr.ndim - 1; i >= 0; --i) {
+      st[i] = s1;
+      s1 *= arr.shape[i];
+    }
+    if (n) {
+      tmp.reset((size_t)n * es);
+      p = tmp.p;
+    }
+  }
+  void finish(hipStream_t s) {
+    DMX_HIP(hipStreamSynchronize(s));
+    if (a->device || !tmp.p) return;
+    std::vector<char> h(tmp.bytes);
+    DMX_HIP(hipMemcpy(h.data(), tmp.p, tmp.bytes, hipMemcpyDeviceToHost));
+    unpack(h.data(), *a);
+  }
+};
+
+hipStream_t library_stream() {
+  thread_local hipStream_t s = nullptr;
+  thread_local int dev = -1;
+  int cur = 0;
+  DMX_HIP(hipGetDevice(&cur));
+  if (!s || dev != cur) {
+    DMX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    dev = cur;
+  }
+  return s;
+}
+
+dmx_options defaults() {
+  dmx_options o;
+  dmx_default_options(&o);
+  return o;
+}
+
+bool is_complex(int tc) { return tc == DMX_C64 || tc == DMX_C128; }
+bool is_real(int tc) { return tc == DMX_F32 || tc == DMX_F64; }
+
+void check_coord(const dmx_array &c) {
+  DMX_ASSERT(c.ndim == 2, "coord must be a 2D array");
+  DMX_ASSERT(is_real(c.typecode), "not yet supported");
+}
+
+// shared request validation of u2nu / nu2u; returns ndim
+int validate(const dmx_array &uni, const dmx_array &coord,
+             const dmx_array &pts) {
+  check_coord(coord);
+  DMX_ASSERT(is_complex(uni.typecode) && is_complex(pts.typecode),
+             "not yet supported");
+  DMX_ASSERT(uni.typecode == pts.typecode, "not yet supported");
+  const int ndim = (int)uni.ndim;
+  DMX_ASSERT(ndim >= 1 && ndim <= 3, "transform must be 1D/2D/3D");
+  DMX_ASSERT(coord.shape[1] == ndim, "dimensionality mismatch");
+  DMX_ASSERT(pts.ndim == 1, "points must be a 1D array");
+  DMX_ASSERT(pts.shape[0] == coord.shape[0], "number of points mismatch");
+  for (int d = 0; d < ndim; ++d)
+    DMX_ASSERT(uni.shape[d] >= 1, "uniform grid dimensions mismatch");
+  return ndim;
+}
+
+hipStream_t pick_stream(const dmx_options &o) {
+  return o.stream ? (hipStream_t)o.stream : library_stream();
+}
+
+int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
+                  const dmx_array *pts, int forward, double epsilon,
+                  double smin, double smax, const double *per, int fft_order,
+                  const dmx_options *opt_in, dmx_plan_info *info,
+                  bool grad = false, const dmx_array *gradout = nullptr) {
+  return guard([&] {
+    DMX_ASSERT(uni && coord && pts && per, "null argument");
+    dmx_options opt = opt_in ? *opt_in : defaults();
+    const int ndim = validate(*uni, *coord, *pts);
+    const size_t M = (size_t)coord->shape[0];
+    size_t nuni[3] = {1, 1, 1};
+    for (int d = 0; d < ndim; ++d) nuni[d] = (size_t)uni->shape[d];
+    const bool vf64 = uni->typecode == DMX_C128;
+    Params p = choose_params(ndim, nuni, M, type1, epsilon, smin, smax, per,
+                             fft_order != 0, vf64, opt);
+    hipStream_t s = pick_stream(opt);
+    Plan plan(p, s);
+    DevIn cin(*coord, s, false);
+    plan.build_index(cin.p, coord->typecode == DMX_F32, M, false);
+    if (type1) {
+      DevIn pin(*pts, s, false);
+      DevOut uo(*uni, true);
+      plan.nu2u(forward != 0, pin.p, vf64, uo.p, vf64, uo.st);
+      uo.finish(s);
+    } else if (!grad) {
+      DevIn ui(*uni, s, true);
+      DevOut po(*pts, false);
+      plan.u2nu(forward != 0, ui.p, vf64, ui.st, po.p, vf64);
+      po.finish(s);
+    } else {
+      DevIn ui(*uni, s, true);
+      DevOut go(*gradout, false);
+      plan.u2nu_grad(forward != 0, ui.p, vf64, ui.st, go.p, vf64);
+      go.finish(s);
+    }
+    if (info) plan.info(info);
+  });
+}
+}  // namespace
+
+void set_error(const char *msg) { g_err = msg; }
+
+}  // namespace dmx
+
+struct dmx_plan {
+  std::unique_ptr<dmx::Plan> plan;
+  bool nu2u;
+  int ndim;
+  int64_t shape[3];
+  hipStream_t stream;
+  dmx::Buf coords;  // device copy when the caller passed host coordinates
+};
+
+using namespace dmx;
+
+extern "C" {
+
+const char *dmx_last_error(void) { return g_err.c_str(); }
+
+void dmx_default_options(dmx_options *o) {
+  std::memset(o, 0, sizeof(*o));
+  o->precision = 0;
+  o->chooser = 0;
+  o->log2tile = 0;
+  o->kernel_idx = -1;
+  o->fp64_below = 1e-6;
+  o->stream = nullptr;
+  o->sigma = 0;
+}
+
+double dmx_best_epsilon(size_t ndim, int singleprec, double sigma_min,
+                        double sigma_max) {
+  double r = -1;
+  int rc = guard([&] {
+    DMX_ASSERT(ndim >= 1 && ndim <= 3, "bad dimensionality");
+    r = best_epsilon((int)ndim, singleprec != 0, sigma_min, sigma_max);
+    DMX_ASSERT(r > 0, "no appropriate kernel found");
+  });
+  return rc ? -1. : r;
+}
+
+int dmx_u2nu(const dmx_array *grid, const dmx_array *coord, int forward,
+             double epsilon, size_t nthreads, dmx_array *out,
+             size_t verbosity, double sigma_min, double sigma_max,
+             const double *periodicity, int fft_order,
+             const dmx_options *opt, dmx_plan_info *info) {
+  (void)nthreads;
+  (void)verbosity;
+  return run_unplanned(false, grid, coord, out, forward, epsilon, sigma_min,
+                       sigma_max, periodicity, fft_order, opt, info);
+}
+
+int dmx_nu2u(const dmx_array *points, const dmx_array *coord, int forward,
+             double epsilon, size_t nthreads, dmx_array *out,
+             size_t verbosity, double sigma_min, double sigma_max,
+             const double *periodicity, int fft_order,
+             const dmx_options *opt, dmx_plan_info *info) {
+  (void)nthreads;
+  (void)verbosity;
+  return run_unplanned(true, out, coord, points, forward, epsilon, sigma_min,
+                       sigma_max, periodicity, fft_order, opt, info);
+}
+
+int dmx_u2nu_coord_grad(const dmx_array *grid, const dmx_array *coord,
+                        int forward, double epsilon, dmx_array *out,
+                        double sigma_min, double sigma_max,
+                        const double *periodicity, int fft_order,
+                        const dmx_options *opt) {
+  return guard([&] {
+    DMX_ASSERT(grid && coord && out && periodicity, "null argument");
+    check_coord(*coord);
+    DMX_ASSERT(out->ndim == 2 && out->shape[0] == coord->shape[0] &&
+                   out->shape[1] == coord->shape[1],
+               "gradient output must have the shape of coord");
+    DMX_ASSERT(out->typecode == grid->typecode, "not yet supported");
+    // view the (M, ndim) output as the points array for validation
+    dmx_array pts = *out;
+    pts.ndim = 1;
+    pts.shape[0] = coord->shape[0];
+    pts.stride[0] = 1;
+    int rc = run_unplanned(false, grid, coord, &pts, forward, epsilon,
+                           sigma_min, sigma_max, periodicity, fft_order, opt,
+                           nullptr, true, out);
+    if (rc) throw Error(g_err);
+  });
+}
+
+int dmx_plan_create(dmx_plan **plan, int nu2u, const dmx_array *coord,
+                    const int64_t *grid_shape, size_t ndim, double epsilon,
+                    size_t nthreads, double sigma_min, double sigma_max,
+                    const double *periodicity, int fft_order,
+                    const dmx_options *opt_in) {
+  (void)nthreads;
+  return guard([&] {
+    DMX_ASSERT(plan && coord && grid_shape && periodicity, "null argument");
+    *plan = nullptr;
+    check_coord(*coord);
+    DMX_ASSERT(ndim >= 1 && ndim <= 3, "transform must be 1D/2D/3D");
+    DMX_ASSERT(coord->shape[1] == (int64_t)ndim, "dimensionality mismatch");
+    dmx_options opt = opt_in ? *opt_in : defaults();
+    size_t nuni[3] = {1, 1, 1};
+    for (size_t d = 0; d < ndim; ++d) {
+      DMX_ASSERT(grid_shape[d] >= 1, "uniform grid dimensions mismatch");
+      nuni[d] = (size_t)grid_shape[d];
+    }
+    const size_t M = (size_t)coord->shape[0];
+    // a plan computes in the precision of its coordinates unless told
+    // otherwise (nufft_pymod.cc:145-150)
+    if (opt.precision == 0 && coord->typecode == DMX_F64) opt.precision = 0;
+    Params p = choose_params((int)ndim, nuni, M, nu2u != 0, epsilon,
+                             sigma_min, sigma_max, periodicity,
+                             fft_order != 0, coord->typecode == DMX_F64, opt);
+    auto h = std::make_unique<dmx_plan>();
+    h->stream = pick_stream(opt);
+    h->nu2u = nu2u != 0;
+    h->ndim = (int)ndim;
+    for (size_t d = 0; d < 3; ++d) h->shape[d] = d < ndim ? grid_shape[d] : 1;
+    h->plan = std::make_unique<Plan>(p, h->stream);
+    DevIn cin(*coord, h->stream, false);
+    h->plan->build_index(cin.p, coord->typecode == DMX_F32, M, true);
+    *plan = h.release();
+  });
+}
+
+int dmx_plan_nu2u(dmx_plan *plan, int forward, size_t verbosity,
+                  const dmx_array *points, dmx_array *out) {
+  (void)verbosity;
+  return guard([&] {
+    DMX_ASSERT(plan && points && out, "null argument");
+    DMX_ASSERT(is_complex(points->typecode) && points->typecode == out->typecode,
+               "not yet supported");
+    DMX_ASSERT(points->ndim == 1 &&
+                   (size_t)points->shape[0] == plan->plan->npoints(),
+               "number of points mismatch");
+    DMX_ASSERT(out->ndim == plan->ndim, "uniform grid dimensions mismatch");
+    for (int d = 0; d < plan->ndim; ++d)
+      DMX_ASSERT(out->shape[d] == plan->shape[d],
+                 "uniform grid dimensions mismatch");
+    hipStream_t s = plan->stream;
+    const bool f64 = points->typecode == DMX_C128;
+    DevIn pin(*points, s, false);
+    DevOut uo(*out, true);
+    plan->plan->nu2u(forward != 0, pin.p, f64, uo.p, f64, uo.st);
+    uo.finish(s);
+  });
+}
+
+int dmx_plan_u2nu(dmx_plan *plan, int forward, size_t verbosity,
+                  const dmx_array *grid, dmx_array *out) {
+  (void)verbosity;
+  return guard([&] {
+    DMX_ASSERT(plan && grid && out, "null argument");
+    DMX_ASSERT(is_complex(grid->typecode) && grid->typecode == out->typecode,
+               "not yet supported");
+    DMX_ASSERT(out->ndim == 1 && (size_t)out->shape[0] == plan->plan->npoints(),
+               "number of points mismatch");
+    DMX_ASSERT(grid->ndim == plan->ndim, "uniform grid dimensions mismatch");
+    for (int d = 0; d < plan->ndim; ++d)
+      DMX_ASSERT(grid->shape[d] == plan->shape[d],
+                 "uniform grid dimensions mismatch");
+    hipStream_t s = plan->stream;
+    const bool f64 = grid->typecode == DMX_C128;
+    DevIn gin(*grid, s, true);
+    DevOut po(*out, false);
+    plan->plan->u2nu(forward != 0, gin.p, f64, gin.st, po.p, f64);
+    po.finish(s);
+  });
+}
+
+int dmx_plan_get_info(const dmx_plan *plan, dmx_plan_info *info) {
+  return guard([&] {
+    DMX_ASSERT(plan && info, "null argument");
+    plan->plan->info(info);
+  });
+}
+
+int dmx_plan_destroy(dmx_plan *plan) {
+  return guard([&] { delete plan; });
+}
+
+int dmx_nudft(int type, const dmx_array *coord, const dmx_array *values,
+              const int64_t *shape, size_t ndim, int forward,
+              const double *periodicity, int fft_order, const int64_t *sel,
+              size_t nsel, dmx_array *out) {
+  return guard([&] {
+    DMX_ASSERT(coord && values && shape && periodicity && out, "null argument");
+    DMX_ASSERT(type == 1 || type == 2, "type must be 1 or 2");
+    DMX_ASSERT(coord->typecode == DMX_F64 && values->typecode == DMX_C128 &&
+                   out->typecode == DMX_C128,
+               "direct NUDFT works on float64 / complex128");
+    DMX_ASSERT(ndim >= 1 && ndim <= 3 && coord->ndim == 2 &&
+                   coord->shape[1] == (int64_t)ndim,
+               "dimensionality mismatch");
+    DMX_ASSERT(nelem(*out) == (int64_t)nsel, "output size mismatch");
+    hipStream_t s = library_stream();
+    DevIn c(*coord, s, false), v(*values, s, false);
+    DevOut o(*out, false);
+    Buf selb(nsel * 8);
+    if (nsel) {
+      DMX_HIP(hipMemcpyAsync(selb.p, sel, nsel * 8, hipMemcpyHostToDevice, s));
+      DMX_HIP(hipStreamSynchronize(s));
+    }
+    NudftArgs a{};
+    a.type = type;
+    a.ndim = (int)ndim;
+    a.forward = forward;
+    a.fft_order = fft_order;
+    a.M = coord->shape[0];
+    a.nsel = (int64_t)nsel;
+    for (int d = 0; d < 3; ++d) {
+      a.n[d] = d < (int)ndim ? shape[d] : 1;
+      a.period[d] = d < (int)ndim ? periodicity[d] : 1.;
+    }
+    a.coord = (const double *)c.p;
+    a.vals = (const double *)v.p;
+    a.sel = selb.as<int64_t>();
+    a.out = (double *)o.p;
+    launch_nudft(a, s);
+    o.finish(s);
+  });
+}
+
+int dmx_sort_points(const dmx_array *coord, const int64_t *nover,
+                    size_t ndim, int W, int log2tile,
+                    const double *periodicity, dmx_array *keys,
+                    dmx_array *idx) {
+  return guard([&] {
+    DMX_ASSERT(coord && nover && periodicity && keys && idx, "null argument");
+    check_coord(*coord);
+    DMX_ASSERT(ndim >= 1 && ndim <= 3 && coord->shape[1] == (int64_t)ndim,
+               "dimensionality mismatch");
+    DMX_ASSERT(keys->typecode == DMX_U32 && idx->typecode == DMX_U32,
+               "keys and idx must be uint32");
+    const size_t M = (size_t)coord->shape[0];
+    DMX_ASSERT(nelem(*keys) == (int64_t)M && nelem(*idx) == (int64_t)M,
+               "number of points mismatch");
+    DMX_ASSERT(W >= 1 && W <= 16 && log2tile >= 0 && log2tile <= 12,
+               "bad sort geometry");
+    Params p;
+    p.ndim = (int)ndim;
+    p.W = W;
+    p.log2tile = log2tile;
+    for (size_t d = 0; d < ndim; ++d) {
+      p.nover[d] = (size_t)nover[d];
+      p.nuni[d] = (size_t)nover[d] / 2;
+      p.period[d] = periodicity[d];
+      DMX_ASSERT(nover[d] >= 2 * ((W + 1) / 2), "oversampled length too small");
+    }
+    // the sort never evaluates the kernel; any row of matching W serves
+    const auto &db = kernel_db();
+    for (size_t i = 0; i < db.size(); ++i)
+      if (db[i].W == W) {
+        p.beta = db[i].beta;
+        p.e0 = db[i].e0;
+        break;
+      }
+    DMX_ASSERT(p.beta > 0, "no kernel of this support");
+    hipStream_t s = library_stream();
+    Plan plan(p, s);
+    DevIn cin(*coord, s, false);
+    plan.build_index(cin.p, coord->typecode == DMX_F32, M, false);
+    DevOut ko(*keys, false), io(*idx, false);
+    if (M) {
+      plan.keys_copy_to((uint32_t *)ko.p);
+      DMX_HIP(hipMemcpyAsync(io.p, plan.idx(), M * 4, hipMemcpyDeviceToDevice,
+                             s));
+    }
+    ko.finish(s);
+    io.finish(s);
+  });
+}
+
+int dmx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+// dmx.h -- internal declarations shared by the host runtime and the gfx950
+// kernels of the NUFFT.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace dmx {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+#define DMX_ASSERT(cond, msg) \
+  do {                        \
+    if (!(cond)) throw ::dmx::Error(msg); \
+  } while (0)
+#define DMX_HIP(call)                                                   \
+  do {                                                                  \
+    hipError_t e_ = (call);                                             \
+    if (e_ != hipSuccess)                                               \
+      throw ::dmx::Error(std::string("HIP error: ") + hipGetErrorString(e_) + \
+                         " at " #call);                                 \
+  } while (0)
+
+// ------------------------------------------------------------- geometry
+// Fold / tile geometry of one transform, passed by value to kernels.
+// nover: oversampled grid; shift = nover - W/2 + 1; maxi0 = nover+nsafe-W
+// (nufft.h:415-421); ntiles = (nover >> log2tile) + 3 (nufft.h:1366-1367).
+struct Geo {
+  int ndim, W, nsafe, log2tile;
+  int nover[3], ntiles[3], maxi0[3];
+  double coordfct[3], shift[3];
+  int coord_f32;  // 1: coordinates stored as float32
+  int pad_;
+};
+
+// One subproblem: a run of at most MAXSUB sorted points of one tile.
+struct Sub {
+  uint32_t tile, start, end, pad_;
+};
+
+constexpr int MAXSUB = 1024;
+
+// ----------------------------------------------------------- device helpers
+#ifdef __HIPCC__
+
+// Fold one coordinate into (first grid index i0, local kernel coordinate).
+// For float64 coordinates the product (t - floor t) * nover is formed exactly
+// as a double-double, so i0 agrees with an 80-bit range reduction
+// (nufft.h:285-298) except when that product lies within 2^-64 (relative) of
+// an integer.  For float32 coordinates the reference reduces in double, which
+// is reproduced operation for operation (no contraction).
+__device__ __forceinline__ int fold_axis(const Geo &g, int d, double x,
+                                         double &frac) {
+#pragma clang fp contract(off)
+  const double n = (double)g.nover[d];
+  double tmp = x * g.coordfct[d];
+  double a = tmp - floor(tmp);
+  int i;
+  if (!g.coord_f32) {
+    double hi = a * n;
+    double lo = fma(a, n, -hi);
+    double s = hi + g.shift[d];
+    double bv = s - hi;
+    double err = (hi - (s - bv)) + (g.shift[d] - bv);
+    double fs = floor(s);
+    double r = (s - fs) + (err + lo);
+    i = (int)fs;
+    if (r < 0.) i -= 1;
+    else if (r >= 1.) i += 1;
+    i -= g.nover[d];
+    if (i > g.maxi0[d]) i = g.maxi0[d];
+    frac = (hi - (double)i) + lo;
+  } else {
+    double t2 = a * n;
+    i = (int)(t2 + g.shift[d]) - g.nover[d];
+    if (i > g.maxi0[d]) i = g.maxi0[d];
+    frac = t2 - (double)i;
+  }
+  // finite coordinates always give 1-nsafe <= i <= maxi0; the clamp keeps
+  // non-finite ones (whose results are meaningless) inside the tile range
+  if (!(i >= 1 - g.nsafe)) i = 1 - g.nsafe;
+  return i;
+}
+
+__device__ __forceinline__ double load_coord(const void *c, size_t j, int ndim,
+                                             int d, int f32) {
+  return f32 ? (double)((const float *)c)[j * ndim + d]
+             : ((const double *)c)[j * ndim + d];
+}
+
+// Values of the W kernel pieces at local coordinate x (x = W-1-2*frac),
+// i.e. the weights of grid points i0..i0+W-1.  Horner in x^2 on the odd and
+// even coefficient halves; piece W-1-i is piece i mirrored
+// (gridding_kernel.h:300-318).  cf layout: for s = 0..NE-1,
+// cf[(2s)*NH+i] = coefficient of x^(DP-2s), cf[(2s+1)*NH+i] = of x^(DP-1-2s).
+template <typename T, int W>
+__device__ __forceinline__ void kernel_weights(T x, const T *__restrict__ cf,
+                                               T *w) {
+  constexpr int DP = W + 3 + (W & 1);
+  constexpr int NE = (DP + 1) / 2;
+  constexpr int NH = (W + 1) / 2;
+  const T x2 = x * x;
+#pragma unroll
+  for (int i = 0; i < NH; ++i) {
+    T o = cf[i], e = cf[NH + i];
+#pragma unroll
+    for (int s = 1; s < NE; ++s) {
+      o = o * x2 + cf[(2 * s) * NH + i];
+      e = e * x2 + cf[(2 * s + 1) * NH + i];
+    }
+    w[i] = e + x * o;
+    if (W - 1 - i != i) w[W - 1 - i] = e - x * o;
+  }
+}
+
+// Weights and their derivatives d/dx of the W pieces at x.
+template <typename T, int W>
+__device__ __forceinline__ void kernel_weights_deriv(T x,
+                                                     const T *__restrict__ cf,
+                                                     T *w, T *dw) {
+  constexpr int DP = W + 3 + (W & 1);
+  constexpr int NE = (DP + 1) / 2;
+  constexpr int NH = (W + 1) / 2;
+  const T x2 = x * x;
+#pragma unroll
+  for (int i = 0; i < NH; ++i) {
+    // p(x) = E(x^2) + x O(x^2); p'(x) = 2x E'(x^2) + O(x^2) + 2x^2 O'(x^2)
+    T o = cf[i], e = cf[NH + i], od = T(0), ed = T(0);
+#pragma unroll
+    for (int s = 1; s < NE; ++s) {
+      od = od * x2 + o;
+      ed = ed * x2 + e;
+      o = o * x2 + cf[(2 * s) * NH + i];
+      e = e * x2 + cf[(2 * s + 1) * NH + i];
+    }
+    T dE = T(2) * x * ed;           // d/dx E(x^2)
+    T dO = o + T(2) * x2 * od;      // d/dx (x O(x^2))
+    w[i] = e + x * o;
+    dw[i] = dE + dO;
+    if (W - 1 - i != i) {
+      w[W - 1 - i] = e - x * o;
+      dw[W - 1 - i] = dE - dO;  // d/dx p(-x) = -p'(-x); E part even, O part
+    }
+  }
+}
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+#endif  // __HIPCC__
+
+// --------------------------------------------------------- host plumbing
+struct FftAxisPlan;  // defined in fft.hip
+
+// Launch wrappers (defined in the .hip files).
+void launch_fold_keys(const Geo &g, const void *coords, size_t M,
+                      uint32_t *keys, uint32_t *counts, hipStream_t s);
+void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
+                      uint32_t *idx_out, uint32_t *keys_out, void *work,
+                      size_t work_bytes, hipStream_t s);
+size_t radix_sort_workspace(size_t M);
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n,
+                        uint32_t *total, void *work, hipStream_t s);
+size_t scan_workspace(size_t n);
+void launch_build_subs(const uint32_t *counts, const uint32_t *tile_start,
+                       size_t ntiles, uint32_t *nsub, uint32_t *sub_start,
+                       Sub *subs, uint32_t *nsub_total, void *scanwork,
+                       hipStream_t s);
+void launch_gather_coords(const void *coords, int coord_f32, int ndim,
+                          const uint32_t *idx, size_t M, void *out,
+                          hipStream_t s);
+void launch_zero(void *p, size_t bytes, hipStream_t s);
+
+// spreading / interpolation (spread.hip, interp.hip)
+struct SpreadArgs {
+  Geo g;
+  const Sub *subs;
+  const uint32_t *nsub_total;
+  const uint32_t *idx;      // sorted order; NULL: identity (coords sorted)
+  const void *coords;       // (M, ndim) float32/float64
+  const void *values;       // (M,) c64 or c128 in original order
+  int val_f64;
+  void *grid;               // oversampled grid, complex of compute type
+  const void *coef;         // kernel coefficients of compute type
+  size_t nsub_max;
+  int lds_side;             // tile + W - 1
+  int use_lds;
+  int coords_sorted;        // 1: coords are stored in sorted order (plan)
+};
+void launch_spread(bool fp64, const SpreadArgs &a, hipStream_t s);
+void launch_interp(bool fp64, const SpreadArgs &a, void *out, int out_f64,
+                   hipStream_t s);
+void launch_interp_grad(bool fp64, const SpreadArgs &a, void *out,
+                        int out_f64, hipStream_t s);
+
+}  // namespace dmx

@@ -1,0 +1,384 @@
+// fft.hip -- hand-written mixed-radix Stockham c2c FFT over the oversampled
+// grid (role of c2c / general_nd / cfftp passes, fft.h:848-850,
+// fftnd_impl.h:593-, fft1d_impl.h:194-825).  Unnormalised; forward means
+// exponent -1.  Grid sizes are 2*(2,3,5,7,11-smooth), so the radices are
+// 4, 2, 3, 5, 7, 11 (4 first, as the reference factorises).
+//
+// A "line set" is every line along one axis of a row-major 3-D view
+// (1-D and 2-D grids get leading unit axes); the two other axes index the
+// lines and each may be restricted to the kept bands [0,(n+1)/2) and
+// [N-n/2, N), which is the reference's pruned FFT (nufft.h:1287-1298,
+// 1345-1356, 1749-1781, 1815-1851).
+//
+// LDS kernel: a workgroup stages Bl whole lines in LDS (coalesced: along
+// rows for the last axis, across Bl adjacent columns otherwise), runs all
+// Stockham passes in LDS with register staging (one LDS buffer, read all
+// inputs -> barrier -> write all outputs) and writes the lines back.
+// Lines too long for LDS use global-memory Stockham passes (out of place).
+#include <cmath>
+#include <vector>
+
+#include "dmx.h"
+#include "fft.h"
+
+namespace dmx {
+
+namespace {
+
+template <typename T> struct V2;
+template <> struct V2<float> { using t = float2; };
+template <> struct V2<double> { using t = double2; };
+
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) {
+  C r; r.x = a.x + b.x; r.y = a.y + b.y; return r;
+}
+template <typename C> __device__ __forceinline__ C csub(C a, C b) {
+  C r; r.x = a.x - b.x; r.y = a.y - b.y; return r;
+}
+// multiply by sign*i
+template <typename C> __device__ __forceinline__ C cmuli(C a, int sign) {
+  C r;
+  if (sign < 0) { r.x = a.y; r.y = -a.x; }
+  else { r.x = -a.y; r.y = a.x; }
+  return r;
+}
+
+// cos/sin(2 pi k / R) for the odd radices
+template <int R> struct Tw;
+template <> struct Tw<3> {
+  static constexpr double c[2] = {-0.5, -0.5};
+  static constexpr double s[2] = {0.86602540378443864676, -0.86602540378443864676};
+};
+template <> struct Tw<5> {
+  static constexpr double c[4] = {0.30901699437494742410, -0.80901699437494742410,
+                                  -0.80901699437494742410, 0.30901699437494742410};
+  static constexpr double s[4] = {0.95105651629515357212, 0.58778525229247312917,
+                                  -0.58778525229247312917, -0.95105651629515357212};
+};
+template <> struct Tw<7> {
+  static constexpr double c[6] = {0.62348980185873353053, -0.22252093395631440429,
+                                  -0.90096886790241912624, -0.90096886790241912624,
+                                  -0.22252093395631440429, 0.62348980185873353053};
+  static constexpr double s[6] = {0.78183148246802980871, 0.97492791218182360702,
+                                  0.43388373911755812048, -0.43388373911755812048,
+                                  -0.97492791218182360702, -0.78183148246802980871};
+};
+template <> struct Tw<11> {
+  static constexpr double c[10] = {
+      0.84125353283118116886, 0.41541501300188642553, -0.14231483827328514044,
+      -0.65486073394528506406, -0.95949297361449738989, -0.95949297361449738989,
+      -0.65486073394528506406, -0.14231483827328514044, 0.41541501300188642553,
+      0.84125353283118116886};
+  static constexpr double s[10] = {
+      0.54064081745559758211, 0.90963199535451837141, 0.98982144188093273238,
+      0.75574957435425828377, 0.28173255684142969771, -0.28173255684142969771,
+      -0.75574957435425828377, -0.98982144188093273238, -0.90963199535451837141,
+      -0.54064081745559758211};
+};
+
+// in-register DFT of length R, exponent sign*2*pi*i*r*k/R
+template <typename T, int R, typename C>
+__device__ __forceinline__ void dft(C *v, int sign) {
+  if constexpr (R == 1) {
+    (void)v; (void)sign;
+  } else if constexpr (R == 2) {
+    C a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    C a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    C a2 = cadd(v[1], v[3]), a3 = cmuli(csub(v[1], v[3]), sign);
+    v[0] = cadd(a0, a2);
+    v[2] = csub(a0, a2);
+    v[1] = cadd(a1, a3);
+    v[3] = csub(a1, a3);
+  } else {
+    // odd R: pair r with R-r
+    constexpr int H = (R - 1) / 2;
+    C sp[H], sm[H];
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      sp[r - 1] = cadd(v[r], v[R - r]);
+      sm[r - 1] = csub(v[r], v[R - r]);
+    }
+    C y0 = v[0];
+#pragma unroll
+    for (int r = 0; r < H; ++r) y0 = cadd(y0, sp[r]);
+    C out[R];
+    out[0] = y0;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+      C a = v[0], b;
+      b.x = T(0);
+      b.y = T(0);
+#pragma unroll
+      for (int r = 1; r <= H; ++r) {
+        const int m = (r * k) % R;  // m in 1..R-1
+        const T cc = (T)Tw<R>::c[m - 1], ss = (T)Tw<R>::s[m - 1];
+        a.x += sp[r - 1].x * cc;
+        a.y += sp[r - 1].y * cc;
+        b.x += sm[r - 1].x * ss;
+        b.y += sm[r - 1].y * ss;
+      }
+      C ib = cmuli(b, sign);
+      out[k] = cadd(a, ib);
+      out[R - k] = csub(a, ib);
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = out[k];
+  }
+}
+
+// One radix-R Stockham pass over Bl LDS-resident lines (stride LP).
+template <typename T, int R, int KM>
+__device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
+                                         int Bl, int Ns,
+                                         const typename V2<T>::t *tw,
+                                         int sign) {
+  using C = typename V2<T>::t;
+  const int nbf = L / R;
+  const int total = nbf * Bl;
+  const int twstep = L / (Ns * R);
+  C v[KM][R];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    const int q = threadIdx.x + k * blockDim.x;
+    if (q < total) {
+      const int b = q / nbf, jj = q - b * nbf;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[k][r] = buf[b * LP + jj + r * nbf];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    const int q = threadIdx.x + k * blockDim.x;
+    if (q < total) {
+      const int b = q / nbf, jj = q - b * nbf;
+      const int kk = jj % Ns;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          C w = tw[r * kk * twstep];
+          if (sign > 0) w.y = -w.y;
+          v[k][r] = cmul(v[k][r], w);
+        }
+      }
+      dft<T, R>(v[k], sign);
+      const int base = (jj / Ns) * Ns * R + kk;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[b * LP + base + r * Ns] = v[k][r];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int64_t map_range(int v, int n1, int a1, int a2) {
+  return v < n1 ? (int64_t)(a1 + v) : (int64_t)(a2 + (v - n1));
+}
+
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_lds(FftLaunch p) {
+  using C = typename V2<T>::t;
+  extern __shared__ unsigned char smem_raw[];
+  C *buf = (C *)smem_raw;
+  C *data = (C *)p.data;
+  const C *tw = (const C *)p.tw;
+  const int L = p.L, LP = p.LP, Bl = p.Bl;
+  const int u = blockIdx.y;
+  const int v0 = blockIdx.x * Bl;
+  const int64_t ubase = map_range(u, p.un1, p.ua1, p.ua2) * p.st_u;
+  const int nlines = min(Bl, p.nv - v0);
+  const int nel = nlines * L;
+  // load
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+    int b, k;
+    if (p.st_ax == 1) { b = e / L; k = e - b * L; }
+    else { k = e / nlines; b = e - k * nlines; }
+    const int64_t addr =
+        ubase + map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v + k * p.st_ax;
+    buf[b * LP + k] = data[addr];
+  }
+  __syncthreads();
+  int Ns = 1;
+  for (int s = 0; s < p.npass; ++s) {
+    const int R = p.radix[s];
+    switch (R) {
+      case 4: lds_pass<T, 4, FFT_KM(T, 4)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 2: lds_pass<T, 2, FFT_KM(T, 2)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 3: lds_pass<T, 3, FFT_KM(T, 3)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 5: lds_pass<T, 5, FFT_KM(T, 5)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 7: lds_pass<T, 7, FFT_KM(T, 7)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 11: lds_pass<T, 11, FFT_KM(T, 11)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      default: break;
+    }
+    Ns *= R;
+  }
+  // store
+  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+    int b, k;
+    if (p.st_ax == 1) { b = e / L; k = e - b * L; }
+    else { k = e / nlines; b = e - k * nlines; }
+    const int64_t addr =
+        ubase + map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v + k * p.st_ax;
+    data[addr] = buf[b * LP + k];
+  }
+}
+
+// One out-of-place radix-R pass in global memory over all lines of the set.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_fft_global_pass(FftLaunch p,
+                                                         const void *src_,
+                                                         void *dst_, int Ns) {
+  using C = typename V2<T>::t;
+  const C *src = (const C *)src_;
+  C *dst = (C *)dst_;
+  const C *tw = (const C *)p.tw;
+  const int L = p.L;
+  const int nbf = L / R;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nl = (int64_t)p.nu * p.nv;
+  if (q >= nl * nbf) return;
+  const int64_t line = q / nbf;
+  const int jj = (int)(q - line * nbf);
+  const int u = (int)(line / p.nv), v = (int)(line - (int64_t)u * p.nv);
+  const int64_t base = map_range(u, p.un1, p.ua1, p.ua2) * p.st_u +
+                       map_range(v, p.vn1, p.va1, p.va2) * p.st_v;
+  C x[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) x[r] = src[base + (int64_t)(jj + r * nbf) * p.st_ax];
+  if (R > 1) {
+    const int kk = jj % Ns;
+    const int twstep = L / (Ns * R);
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C w = tw[(int64_t)r * kk * twstep];
+        if (p.sign > 0) w.y = -w.y;
+        x[r] = cmul(x[r], w);
+      }
+    }
+    dft<T, R>(x, p.sign);
+    const int ob = (jj / Ns) * Ns * R + kk;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      dst[base + (int64_t)(ob + r * Ns) * p.st_ax] = x[r];
+  } else {
+    dst[base + (int64_t)jj * p.st_ax] = x[0];
+  }
+}
+
+std::vector<int> factorize(int L) {
+  std::vector<int> f;
+  for (int r : {4, 2, 3, 5, 7, 11})
+    while (L % r == 0 && L > 1) {
+      f.push_back(r);
+      L /= r;
+    }
+  DMX_ASSERT(L == 1, "FFT length is not 2,3,5,7,11-smooth");
+  return f;
+}
+
+}  // namespace
+
+void fft_twiddles(int L, bool fp64, std::vector<unsigned char> &out) {
+  // exp(-2 pi i m / L), m = 0..L-1, computed in double
+  const double pi = 3.141592653589793238462643383279502884197;
+  size_t esz = fp64 ? 16 : 8;
+  out.resize((size_t)L * esz);
+  for (int m = 0; m < L; ++m) {
+    // reduce the angle to the first octant-symmetric form for accuracy
+    double c, s;
+    long mm = m;
+    // use sincospi-like evaluation on 2m/L in [0,2)
+    double x = 2.0 * (double)mm / (double)L;
+    if (x > 1.0) x -= 2.0;  // (-1, 1]
+    c = std::cos(pi * x);
+    s = -std::sin(pi * x);
+    if (4 * (long)m == (long)L) { c = 0; s = -1; }
+    if (2 * (long)m == (long)L) { c = -1; s = 0; }
+    if (4 * (long)m == 3 * (long)L) { c = 0; s = 1; }
+    if (fp64) {
+      double *p = (double *)out.data() + 2 * (size_t)m;
+      p[0] = c; p[1] = s;
+    } else {
+      float *p = (float *)out.data() + 2 * (size_t)m;
+      p[0] = (float)c; p[1] = (float)s;
+    }
+  }
+}
+
+void fft_lineset(const FftLaunch &base, bool fp64, void *scratch,
+                 hipStream_t s) {
+  FftLaunch p = base;
+  if (p.nu <= 0 || p.nv <= 0 || p.L <= 1) return;
+  auto f = factorize(p.L);
+  DMX_ASSERT(f.size() <= 24, "too many FFT passes");
+  p.npass = (int)f.size();
+  for (size_t i = 0; i < f.size(); ++i) p.radix[i] = f[i];
+  const size_t esz = fp64 ? 16 : 8;
+  const int maxel = fp64 ? FFT_MAXEL64 : FFT_MAXEL32;
+  if (p.L <= maxel) {
+    int Bl = maxel / p.L;
+    if (p.st_ax == 1) Bl = std::min(Bl, 8);
+    Bl = std::max(1, std::min(Bl, p.nv));
+    // cap lines per block by the register staging capacity: the staged
+    // butterflies per thread never exceed FFT_KM for any radix
+    p.Bl = Bl;
+    p.LP = p.L;
+    size_t sh = (size_t)Bl * p.LP * esz;
+    dim3 grid((unsigned)((p.nv + Bl - 1) / Bl), (unsigned)p.nu);
+    DMX_ASSERT(p.nu <= 65535, "FFT line set too large");
+    if (fp64)
+      hipLaunchKernelGGL(k_fft_lds<double>, grid, dim3(FFT_THREADS), sh, s, p);
+    else
+      hipLaunchKernelGGL(k_fft_lds<float>, grid, dim3(FFT_THREADS), sh, s, p);
+    DMX_HIP(hipGetLastError());
+    return;
+  }
+  // long lines: global Stockham passes, ping-pong between data and scratch
+  DMX_ASSERT(scratch != nullptr, "internal: FFT scratch missing");
+  void *src = p.data, *dst = scratch;
+  int Ns = 1;
+  auto run = [&](int R) {
+    int64_t nthr = (int64_t)p.nu * p.nv * (p.L / R);
+    unsigned nb = (unsigned)((nthr + 255) / 256);
+#define DMX_GP(TT, RR) \
+  hipLaunchKernelGGL((k_fft_global_pass<TT, RR>), dim3(nb), dim3(256), 0, s, p, src, dst, Ns)
+    if (fp64) {
+      switch (R) {
+        case 1: DMX_GP(double, 1); break; case 2: DMX_GP(double, 2); break;
+        case 3: DMX_GP(double, 3); break; case 4: DMX_GP(double, 4); break;
+        case 5: DMX_GP(double, 5); break; case 7: DMX_GP(double, 7); break;
+        case 11: DMX_GP(double, 11); break;
+      }
+    } else {
+      switch (R) {
+        case 1: DMX_GP(float, 1); break; case 2: DMX_GP(float, 2); break;
+        case 3: DMX_GP(float, 3); break; case 4: DMX_GP(float, 4); break;
+        case 5: DMX_GP(float, 5); break; case 7: DMX_GP(float, 7); break;
+        case 11: DMX_GP(float, 11); break;
+      }
+    }
+#undef DMX_GP
+    DMX_HIP(hipGetLastError());
+    std::swap(src, dst);
+  };
+  for (int R : f) {
+    run(R);
+    Ns *= R;
+  }
+  if (src != p.data) {
+    // odd number of passes: copy back with an identity pass
+    Ns = 1;
+    run(1);
+  }
+}
+
+}  // namespace dmx

@@ -1,0 +1,43 @@
+// grid.h -- argument blocks of the grid-side kernels in grid.hip.
+#pragma once
+#include <cstdint>
+
+#include "dmx.h"
+
+namespace dmx {
+
+struct CorfacArgs {
+  int n, nq, W, pad_;
+  double dx;
+  double x[32], wpsi[32];
+};
+
+// uniform <-> oversampled grid copy with correction.  Axes padded to 3 with
+// n = N = 1; cf* point at n/2+1 correction factors per axis (a single 1.0
+// for padded axes); ust: strides of the uniform array in elements.
+struct GridCopyArgs {
+  void *grid;
+  void *uni;
+  int u_f64, fft_order;
+  int n[3], N[3];
+  int64_t ust[3];
+  const double *cf0, *cf1, *cf2;
+};
+
+struct NudftArgs {
+  int type, ndim, forward, fft_order;
+  int64_t M, nsel;
+  int64_t n[3];
+  double period[3];
+  const double *coord;  // (M, ndim)
+  const double *vals;   // complex128: M points (type 1) or prod(n) modes
+  const int64_t *sel;
+  double *out;          // complex128 nsel
+};
+
+void launch_corfac(const CorfacArgs &a, double *out, hipStream_t s);
+void launch_grid_copy(bool fp64, bool pad, const GridCopyArgs &a,
+                      hipStream_t s);
+void launch_nudft(const NudftArgs &a, hipStream_t s);
+
+}  // namespace dmx

@@ -1,0 +1,128 @@
+// runtime.h -- device memory pool, per-call staging of host/device arrays,
+// and the NUFFT plan (sorted index, kernel, correction factors, grid).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/ducc_amd.h"
+#include "dmx.h"
+#include "fft.h"
+
+namespace dmx {
+
+// Caching allocator per device: freed blocks are kept and reused for
+// requests of at most their size (and at least half of it).
+class Pool {
+ public:
+  static Pool &get();
+  void *alloc(size_t bytes);
+  void release(void *p);
+  void trim();
+
+ private:
+  std::mutex mu_;
+  std::map<void *, std::pair<int, size_t>> live_;
+  std::multimap<std::pair<int, size_t>, void *> free_;
+};
+
+struct Buf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  Buf() = default;
+  explicit Buf(size_t n) { reset(n); }
+  ~Buf() { free(); }
+  Buf(const Buf &) = delete;
+  Buf &operator=(const Buf &) = delete;
+  Buf(Buf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  void reset(size_t n) {
+    free();
+    if (n) p = Pool::get().alloc(n);
+    bytes = n;
+  }
+  void free() {
+    if (p) Pool::get().release(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T *as() const { return (T *)p; }
+};
+
+struct Params {
+  int ndim = 0;
+  size_t nuni[3] = {1, 1, 1};
+  size_t nover[3] = {1, 1, 1};
+  double period[3] = {1, 1, 1};
+  bool fft_order = false;
+  double eps = 0;
+  int kidx = -1, W = 0;
+  double beta = 0, e0 = 0, ofactor = 0, eps_tab = 0;
+  bool fp64 = true;
+  int log2tile = 4;
+  bool use_lds = true;
+  int chooser_used = 0;
+};
+
+// choose precision, kernel, oversampled grid and tile for a request
+Params choose_params(int ndim, const size_t *nuni, size_t M, bool gridding,
+                     double eps, double smin, double smax,
+                     const double *period, bool fft_order, bool values_f64,
+                     const dmx_options &opt);
+
+class Plan {
+ public:
+  Plan(const Params &p, hipStream_t s);
+  ~Plan();
+  // sorts the points; coords: device (M, ndim), f32 or f64, contiguous.
+  // keep_sorted_coords: store a sorted copy (planned use).
+  void build_index(const void *coords, bool coord_f32, size_t M,
+                   bool keep_sorted_coords);
+  // type 1: points (device, M complex, c64/c128) -> uniform (device, strided)
+  void nu2u(bool forward, const void *points, bool pts_f64, void *uni,
+            bool uni_f64, const int64_t *ust);
+  // type 2: uniform (device, strided) -> points (device, M complex)
+  void u2nu(bool forward, const void *uni, bool uni_f64, const int64_t *ust,
+            void *out, bool out_f64);
+  // d u2nu / d coord -> out (M, ndim) complex
+  void u2nu_grad(bool forward, const void *uni, bool uni_f64,
+                 const int64_t *ust, void *out, bool out_f64);
+  // the sort results of build_index (device)
+  const uint32_t *idx() const { return idx_.as<uint32_t>(); }
+  uint32_t *keys_copy_to(uint32_t *dev_keys_out);  // re-fold keys
+  void info(dmx_plan_info *o) const;
+  const Params &params() const { return p_; }
+  hipStream_t stream() const { return s_; }
+  size_t npoints() const { return M_; }
+  Geo geo() const { return g_; }
+
+ private:
+  void prepare_grid_side();
+  void run_fft(bool forward, bool type1);
+  void fill_spread_args(SpreadArgs &a) const;
+
+  Params p_;
+  hipStream_t s_;
+  Geo g_{};
+  size_t M_ = 0;
+  size_t ntiles_ = 0;
+  size_t nsub_max_ = 0;
+  int lds_side_ = 0;
+  const void *coords_ = nullptr;  // points to user coords or sorted copy
+  bool coords_sorted_ = false;
+  bool coord_f32_ = false;
+  Buf coef_, corfac_[3], one_;
+  std::vector<Buf> tw_;
+  std::vector<int> tw_len_;
+  Buf idx_, subs_, nsub_total_, sorted_coords_;
+  Buf grid_, scratch_;
+  const void *twiddle_for(int L);
+};
+
+// error channel of the C ABI
+void set_error(const char *msg);
+
+}  // namespace dmx

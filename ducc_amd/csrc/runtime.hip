@@ -1,0 +1,473 @@
+// runtime.hip -- host runtime of the NUFFT: memory pool, parameter choice,
+// and the plan executor that sequences the gfx950 kernels.
+//
+// Type 1 (nu2u):  zero grid -> spread (LDS tiles, atomic flush) -> pruned
+//                 FFT (last axis over all lines, then each earlier axis over
+//                 the kept bands only) -> crop + deconvolve.
+// Type 2 (u2nu):  zero grid -> deconvolve + pad -> pruned FFT in reverse
+//                 order -> interpolate (LDS tiles).
+// (nufft.h:1274-1362, 836-890, 1731-1860.)
+#include "runtime.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "chooser.h"
+#include "grid.h"
+#include "params.h"
+
+namespace dmx {
+
+// ------------------------------------------------------------------ pool
+Pool &Pool::get() {
+  static Pool *p = new Pool();  // never destroyed: outlives static teardown
+  return *p;
+}
+
+void *Pool::alloc(size_t bytes) {
+  int dev = 0;
+  DMX_HIP(hipGetDevice(&dev));
+  bytes = (bytes + 255) & ~(size_t)255;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = free_.lower_bound({dev, bytes});
+    if (it != free_.end() && it->first.first == dev &&
+        it->first.second <= 2 * bytes) {
+      void *p = it->second;
+      live_[p] = it->first;
+      free_.erase(it);
+      return p;
+    }
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    // release cached blocks and retry once
+    trim();
+    (void)hipGetLastError();
+    e = hipMalloc(&p, bytes);
+  }
+  if (e != hipSuccess)
+    throw Error(std::string("device memory allocation of ") +
+                std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+  std::lock_guard<std::mutex> lk(mu_);
+  live_[p] = {dev, bytes};
+  return p;
+}
+
+void Pool::release(void *p) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  free_.insert({it->second, p});
+  live_.erase(it);
+}
+
+void Pool::trim() {
+  std::lock_guard<std::mutex> lk(mu_);
+  (void)hipDeviceSynchronize();
+  for (auto &kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+}
+
+// -------------------------------------------------------- parameter choice
+namespace {
+int default_log2tile(int ndim, bool fp64) {
+  if (ndim == 1) return 9;
+  if (ndim == 2) return fp64 ? 4 : 5;
+  return 3;
+}
+size_t lds_bytes(int ndim, int tile, int W, bool fp64) {
+  size_t S = (size_t)tile + W - 1, c = S;
+  for (int d = 1; d < ndim; ++d) c *= S;
+  return c * (fp64 ? 16 : 8);  // re and im planes
+}
+constexpr size_t LDS_LIMIT = 65536;
+
+// reference limits: nufft.h:225-228, 400, 409 with its own tile sizes
+void reference_limits(const Params &p) {
+  int l2 = p.ndim == 1 ? 9 : (p.ndim == 2 ? (p.fp64 ? 4 : 5) : 4);
+  double mx = p.ndim == 1 ? 4294967285.0 : (p.ndim == 2 ? 65526.0 : 1014.0);
+  for (int d = 0; d < p.ndim; ++d)
+    DMX_ASSERT((double)(p.nover[d] >> l2) <= mx, "oversampled grid too large");
+}
+}  // namespace
+
+Params choose_params(int ndim, const size_t *nuni, size_t M, bool gridding,
+                     double eps, double smin, double smax,
+                     const double *period, bool fft_order, bool values_f64,
+                     const dmx_options &opt) {
+  DMX_ASSERT(ndim >= 1 && ndim <= 3, "transform must be 1D/2D/3D");
+  DMX_ASSERT(M <= 0xFFFFFFFFull, "too many nonuniform points");
+  DMX_ASSERT(eps > 0, "epsilon must be positive");
+  Params p;
+  p.ndim = ndim;
+  for (int d = 0; d < ndim; ++d) {
+    p.nuni[d] = nuni[d];
+    p.period[d] = period[d];
+    DMX_ASSERT(period[d] > 0, "periodicity must be positive");
+  }
+  p.fft_order = fft_order;
+  p.eps = eps;
+  if (opt.precision == 2)
+    p.fp64 = true;
+  else if (opt.precision == 1)
+    p.fp64 = false;
+  else  // the value precision, raised to FP64 for tight tolerances
+    p.fp64 = values_f64 || eps < opt.fp64_below ||
+             available_kernels(eps, ndim, true, smin, smax).empty();
+  const auto &db = kernel_db();
+  int kidx = -1;
+  int l2t = 0;
+  if (opt.kernel_idx >= 0) {
+    DMX_ASSERT(opt.kernel_idx < (int)db.size(), "no appropriate kernel found");
+    kidx = opt.kernel_idx;
+    const KRow &k = db[kidx];
+    DMX_ASSERT(k.ndim == ndim, "kernel table row has the wrong dimensionality");
+    DMX_ASSERT(p.fp64 || k.W <= 8, "FP32 path supports W <= 8");
+    double of = opt.sigma > 0 ? opt.sigma : k.ofactor;
+    for (int d = 0; d < ndim; ++d) p.nover[d] = oversampled_length(nuni[d], of);
+  } else if (opt.chooser == 1) {
+    LearnedChoice lc;
+    if (learned_choose(ndim, nuni, M, eps, smin, smax, p.fp64, gridding, lc)) {
+      kidx = lc.kidx;
+      l2t = lc.log2tile;
+      for (int d = 0; d < ndim; ++d) p.nover[d] = lc.nover[d];
+      p.chooser_used = 1;
+    }
+  }
+  if (kidx < 0) {
+    kidx = find_params(eps, smin, smax, ndim, nuni, M, gridding, !p.fp64, 8, 8,
+                       p.nover);
+    DMX_ASSERT(kidx >= 0, "no appropriate kernel found");
+  }
+  const KRow &k = db[kidx];
+  p.kidx = kidx;
+  p.W = k.W;
+  p.beta = k.beta;
+  p.e0 = k.e0;
+  p.ofactor = k.ofactor;
+  p.eps_tab = k.epsilon;
+  reference_limits(p);
+  for (int d = 0; d < ndim; ++d)
+    DMX_ASSERT(p.nover[d] >= 2 * (size_t)((p.W + 1) / 2),
+               "oversampled length too small");
+  // tile: requested, else the default shrunk until the LDS region fits
+  if (opt.log2tile > 0) l2t = opt.log2tile;
+  int def = default_log2tile(ndim, p.fp64);
+  if (l2t <= 0) l2t = def;
+  DMX_ASSERT(l2t >= 2 && l2t <= 10, "log2tile out of range");
+  p.use_lds = true;
+  while (lds_bytes(ndim, 1 << l2t, p.W, p.fp64) > LDS_LIMIT && l2t > 2) --l2t;
+  if (lds_bytes(ndim, 1 << l2t, p.W, p.fp64) > LDS_LIMIT) {
+    DMX_ASSERT(ndim == 3, "internal: LDS region too large");
+    p.use_lds = false;
+  }
+  p.log2tile = l2t;
+  size_t ntot = 1;
+  for (int d = 0; d < ndim; ++d) ntot *= ((p.nover[d] >> l2t) + 3);
+  DMX_ASSERT(ntot < 0xFFFFFFFFull, "oversampled grid too large");
+  return p;
+}
+
+// ------------------------------------------------------------------ plan
+Plan::Plan(const Params &p, hipStream_t s) : p_(p), s_(s) {
+  g_.ndim = p.ndim;
+  g_.W = p.W;
+  g_.nsafe = (p.W + 1) / 2;
+  g_.log2tile = p.log2tile;
+  for (int d = 0; d < 3; ++d) {
+    if (d < p.ndim) {
+      g_.nover[d] = (int)p.nover[d];
+      g_.ntiles[d] = (int)((p.nover[d] >> p.log2tile) + 3);
+      g_.maxi0[d] = (int)(p.nover[d] + g_.nsafe) - p.W;
+      g_.coordfct[d] = 1. / p.period[d];
+      g_.shift[d] = p.W * (-0.5) + 1 + (double)p.nover[d];
+    } else {
+      g_.nover[d] = 1;
+      g_.ntiles[d] = 1;
+      g_.maxi0[d] = 0;
+      g_.coordfct[d] = 1.;
+      g_.shift[d] = 0.;
+    }
+  }
+  lds_side_ = (1 << p.log2tile) + p.W - 1;
+  ntiles_ = (size_t)g_.ntiles[0] * g_.ntiles[1] * g_.ntiles[2];
+  // kernel coefficients in the compute type
+  auto c = device_coeffs(p.W, p.beta, p.e0);
+  if (p.fp64) {
+    coef_.reset(c.size() * 8);
+    DMX_HIP(hipMemcpyAsync(coef_.p, c.data(), c.size() * 8,
+                           hipMemcpyHostToDevice, s_));
+  } else {
+    std::vector<float> cf(c.begin(), c.end());
+    coef_.reset(cf.size() * 4);
+    DMX_HIP(hipMemcpyAsync(coef_.p, cf.data(), cf.size() * 4,
+                           hipMemcpyHostToDevice, s_));
+  }
+  // correction factors, FP64 on the device
+  CorfacArgs ca{};
+  ca.nq = corr_quadrature(p.W, p.beta * p.W, p.e0, ca.x, ca.wpsi);
+  ca.W = p.W;
+  for (int d = 0; d < p.ndim; ++d) {
+    bool reuse = d > 0 && p.nuni[d] == p.nuni[d - 1] && p.nover[d] == p.nover[d - 1];
+    if (reuse) continue;
+    ca.n = (int)(p.nuni[d] / 2 + 1);
+    ca.dx = 1. / (double)p.nover[d];
+    corfac_[d].reset((size_t)ca.n * 8);
+    launch_corfac(ca, corfac_[d].as<double>(), s_);
+  }
+  static const double one = 1.0;
+  one_.reset(8);
+  DMX_HIP(hipMemcpyAsync(one_.p, &one, 8, hipMemcpyHostToDevice, s_));
+  // the host staging above is pageable; make sure it is consumed
+  DMX_HIP(hipStreamSynchronize(s_));
+}
+
+Plan::~Plan() { (void)hipStreamSynchronize(s_); }
+
+const void *Plan::twiddle_for(int L) {
+  for (size_t i = 0; i < tw_len_.size(); ++i)
+    if (tw_len_[i] == L) return tw_[i].p;
+  std::vector<unsigned char> h;
+  fft_twiddles(L, p_.fp64, h);
+  tw_.emplace_back(h.size());
+  DMX_HIP(hipMemcpyAsync(tw_.back().p, h.data(), h.size(),
+                         hipMemcpyHostToDevice, s_));
+  DMX_HIP(hipStreamSynchronize(s_));
+  tw_len_.push_back(L);
+  return tw_.back().p;
+}
+
+void Plan::build_index(const void *coords, bool coord_f32, size_t M,
+                       bool keep_sorted_coords) {
+  M_ = M;
+  coord_f32_ = coord_f32;
+  g_.coord_f32 = coord_f32 ? 1 : 0;
+  coords_ = coords;
+  coords_sorted_ = false;
+  if (M == 0) return;
+  int keybits = 1;
+  while (((size_t)1 << keybits) < ntiles_) ++keybits;
+  Buf keys(M * 4), counts(ntiles_ * 4), tstart(ntiles_ * 4),
+      nsub(ntiles_ * 4), sstart(ntiles_ * 4);
+  size_t ws = std::max(radix_sort_workspace(M), scan_workspace(ntiles_));
+  Buf work(ws);
+  launch_zero(counts.p, ntiles_ * 4, s_);
+  launch_fold_keys(g_, coords, M, keys.as<uint32_t>(), counts.as<uint32_t>(),
+                   s_);
+  idx_.reset(M * 4);
+  radix_sort_pairs(keys.as<uint32_t>(), M, keybits, idx_.as<uint32_t>(),
+                   nullptr, work.p, ws, s_);
+  exclusive_scan_u32(counts.as<uint32_t>(), tstart.as<uint32_t>(), ntiles_,
+                     nullptr, work.p, s_);
+  nsub_max_ = std::min(ntiles_, M) + (M + MAXSUB - 1) / MAXSUB;
+  subs_.reset(nsub_max_ * sizeof(Sub));
+  nsub_total_.reset(4);
+  launch_build_subs(counts.as<uint32_t>(), tstart.as<uint32_t>(), ntiles_,
+                    nsub.as<uint32_t>(), sstart.as<uint32_t>(),
+                    subs_.as<Sub>(), nsub_total_.as<uint32_t>(), work.p, s_);
+  if (keep_sorted_coords) {
+    sorted_coords_.reset(M * p_.ndim * (coord_f32 ? 4 : 8));
+    launch_gather_coords(coords, coord_f32, p_.ndim, idx_.as<uint32_t>(), M,
+                         sorted_coords_.p, s_);
+    coords_ = sorted_coords_.p;
+    coords_sorted_ = true;
+  }
+  // temporaries go back to the pool; the stream must finish with them first
+  DMX_HIP(hipStreamSynchronize(s_));
+}
+
+uint32_t *Plan::keys_copy_to(uint32_t *dev_keys_out) {
+  launch_fold_keys(g_, coords_, M_, dev_keys_out, nullptr, s_);
+  return dev_keys_out;
+}
+
+void Plan::fill_spread_args(SpreadArgs &a) const {
+  a.g = g_;
+  a.subs = subs_.as<Sub>();
+  a.nsub_total = nsub_total_.as<uint32_t>();
+  a.idx = idx_.as<uint32_t>();
+  a.coords = coords_;
+  a.coef = coef_.p;
+  a.nsub_max = nsub_max_;
+  a.lds_side = lds_side_;
+  a.use_lds = p_.use_lds ? 1 : 0;
+  a.coords_sorted = coords_sorted_ ? 1 : 0;
+}
+
+void Plan::prepare_grid_side() {
+  size_t n = 1;
+  for (int d = 0; d < p_.ndim; ++d) n *= p_.nover[d];
+  size_t bytes = n * (p_.fp64 ? 16 : 8);
+  if (grid_.bytes != bytes) grid_.reset(bytes);
+  bool need_scratch = false;
+  for (int d = 0; d < p_.ndim; ++d)
+    if ((int)p_.nover[d] > (p_.fp64 ? FFT_MAXEL64 : FFT_MAXEL32))
+      need_scratch = true;
+  if (need_scratch && scratch_.bytes != bytes) scratch_.reset(bytes);
+}
+
+void Plan::run_fft(bool forward, bool type1) {
+  const int nd = p_.ndim, f = 3 - nd;
+  int64_t E[3], st[3];
+  int n_act[3], N_act[3];
+  for (int a = 0; a < 3; ++a) {
+    if (a < f) {
+      E[a] = 1; n_act[a] = 1; N_act[a] = 1;
+    } else {
+      E[a] = (int64_t)p_.nover[a - f];
+      n_act[a] = (int)p_.nuni[a - f];
+      N_act[a] = (int)p_.nover[a - f];
+    }
+  }
+  st[2] = 1;
+  st[1] = E[2];
+  st[0] = E[1] * E[2];
+  auto range = [&](int ax, bool restrict_, int &cnt, int &n1, int &a1,
+                   int &a2) {
+    if (!restrict_ || ax < f) {
+      cnt = (int)E[ax]; n1 = (int)E[ax]; a1 = 0; a2 = 0;
+    } else {
+      int n = n_act[ax], N = N_act[ax];
+      n1 = (n + 1) / 2; a1 = 0; a2 = N - n / 2; cnt = n1 + n / 2;
+    }
+  };
+  std::vector<int> order;
+  if (type1) for (int a = 2; a >= f; --a) order.push_back(a);
+  else for (int a = f; a <= 2; ++a) order.push_back(a);
+  for (int a : order) {
+    int pax = a == 0 ? 1 : 0, qax = a == 2 ? 1 : 2;
+    FftLaunch L{};
+    L.data = grid_.p;
+    L.L = (int)E[a];
+    L.st_ax = st[a];
+    L.st_u = st[pax];
+    L.st_v = st[qax];
+    L.sign = forward ? -1 : 1;
+    range(pax, pax > a, L.nu, L.un1, L.ua1, L.ua2);
+    range(qax, qax > a, L.nv, L.vn1, L.va1, L.va2);
+    if (L.L <= 1) continue;
+    L.tw = twiddle_for(L.L);
+    fft_lineset(L, p_.fp64, scratch_.p, s_);
+  }
+}
+
+void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
+                bool uni_f64, const int64_t *ust) {
+  prepare_grid_side();
+  launch_zero(grid_.p, grid_.bytes, s_);
+  if (M_ > 0) {
+    SpreadArgs a{};
+    fill_spread_args(a);
+    a.values = points;
+    a.val_f64 = pts_f64 ? 1 : 0;
+    a.grid = grid_.p;
+    launch_spread(p_.fp64, a, s_);
+  }
+  run_fft(forward, true);
+  GridCopyArgs c{};
+  c.grid = grid_.p;
+  c.uni = uni;
+  c.u_f64 = uni_f64 ? 1 : 0;
+  c.fft_order = p_.fft_order ? 1 : 0;
+  for (int d = 0; d < 3; ++d) {
+    c.n[d] = d < p_.ndim ? (int)p_.nuni[d] : 1;
+    c.N[d] = d < p_.ndim ? (int)p_.nover[d] : 1;
+    c.ust[d] = d < p_.ndim ? ust[d] : 0;
+  }
+  const double *cf[3];
+  for (int d = 0; d < 3; ++d) {
+    if (d >= p_.ndim) { cf[d] = one_.as<double>(); continue; }
+    int e = d;
+    while (!corfac_[e].p) --e;  // reused from the previous axis
+    cf[d] = corfac_[e].as<double>();
+  }
+  c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
+  launch_grid_copy(p_.fp64, false, c, s_);
+}
+
+void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
+                const int64_t *ust, void *out, bool out_f64) {
+  prepare_grid_side();
+  launch_zero(grid_.p, grid_.bytes, s_);
+  GridCopyArgs c{};
+  c.grid = grid_.p;
+  c.uni = const_cast<void *>(uni);
+  c.u_f64 = uni_f64 ? 1 : 0;
+  c.fft_order = p_.fft_order ? 1 : 0;
+  for (int d = 0; d < 3; ++d) {
+    c.n[d] = d < p_.ndim ? (int)p_.nuni[d] : 1;
+    c.N[d] = d < p_.ndim ? (int)p_.nover[d] : 1;
+    c.ust[d] = d < p_.ndim ? ust[d] : 0;
+  }
+  const double *cf[3];
+  for (int d = 0; d < 3; ++d) {
+    if (d >= p_.ndim) { cf[d] = one_.as<double>(); continue; }
+    int e = d;
+    while (!corfac_[e].p) --e;
+    cf[d] = corfac_[e].as<double>();
+  }
+  c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
+  launch_grid_copy(p_.fp64, true, c, s_);
+  run_fft(forward, false);
+  if (M_ > 0) {
+    SpreadArgs a{};
+    fill_spread_args(a);
+    a.grid = grid_.p;
+    launch_interp(p_.fp64, a, out, out_f64 ? 1 : 0, s_);
+  }
+}
+
+void Plan::u2nu_grad(bool forward, const void *uni, bool uni_f64,
+                     const int64_t *ust, void *out, bool out_f64) {
+  prepare_grid_side();
+  launch_zero(grid_.p, grid_.bytes, s_);
+  GridCopyArgs c{};
+  c.grid = grid_.p;
+  c.uni = const_cast<void *>(uni);
+  c.u_f64 = uni_f64 ? 1 : 0;
+  c.fft_order = p_.fft_order ? 1 : 0;
+  for (int d = 0; d < 3; ++d) {
+    c.n[d] = d < p_.ndim ? (int)p_.nuni[d] : 1;
+    c.N[d] = d < p_.ndim ? (int)p_.nover[d] : 1;
+    c.ust[d] = d < p_.ndim ? ust[d] : 0;
+  }
+  const double *cf[3];
+  for (int d = 0; d < 3; ++d) {
+    if (d >= p_.ndim) { cf[d] = one_.as<double>(); continue; }
+    int e = d;
+    while (!corfac_[e].p) --e;
+    cf[d] = corfac_[e].as<double>();
+  }
+  c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
+  launch_grid_copy(p_.fp64, true, c, s_);
+  run_fft(forward, false);
+  if (M_ > 0) {
+    SpreadArgs a{};
+    fill_spread_args(a);
+    a.grid = grid_.p;
+    launch_interp_grad(p_.fp64, a, out, out_f64 ? 1 : 0, s_);
+  }
+}
+
+void Plan::info(dmx_plan_info *o) const {
+  std::memset(o, 0, sizeof(*o));
+  o->ndim = p_.ndim;
+  o->W = p_.W;
+  o->kernel_idx = p_.kidx;
+  o->fp64 = p_.fp64;
+  o->log2tile = p_.log2tile;
+  o->chooser_used = p_.chooser_used;
+  for (int d = 0; d < 3; ++d) {
+    o->nuni[d] = d < p_.ndim ? (int64_t)p_.nuni[d] : 1;
+    o->nover[d] = d < p_.ndim ? (int64_t)p_.nover[d] : 1;
+  }
+  o->ofactor = p_.ofactor;
+  o->eps_tab = p_.eps_tab;
+  o->epsilon = p_.eps;
+  o->npoints = (int64_t)M_;
+}
+
+}  // namespace dmx

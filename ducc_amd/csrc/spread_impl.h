@@ -1,0 +1,391 @@
+// spread_impl.h -- type-1 spreading and type-2 interpolation kernels.
+//
+// Role of spreading_helper / interpolation_helper (nufft.h:639-834,
+// 1141-1272, 1589-1729), designed for the GPU: one workgroup per
+// subproblem (at most MAXSUB sorted points of one tile).  The spreader
+// accumulates the tile plus its kernel halo, (2^log2tile + W - 1)^d cells,
+// in LDS with LDS float atomics and flushes the non-zero cells to the
+// oversampled grid in HBM with hardware float atomics (periodic wrap).  The
+// interpolator stages the same region from HBM into LDS once and reduces the
+// W^d taps per point from LDS.  A no-LDS variant (direct HBM atomics /
+// loads) serves 3-D FP64 kernels whose region would not fit in 64 KiB.
+//
+// Per point: FP64 fold (fold_axis), kernel argument x = W-1-2*frac cast to
+// the compute type, W weights per axis by even/odd Horner (kernel_weights).
+#pragma once
+#include "dmx.h"
+
+namespace dmx {
+
+template <typename T> struct cplx2;
+template <> struct cplx2<float> { using type = float2; };
+template <> struct cplx2<double> { using type = double2; };
+
+template <typename T>
+__device__ __forceinline__ void load_value(const void *v, size_t j, int f64,
+                                           T &re, T &im) {
+  if (f64) {
+    double2 x = ((const double2 *)v)[j];
+    re = (T)x.x;
+    im = (T)x.y;
+  } else {
+    float2 x = ((const float2 *)v)[j];
+    re = (T)x.x;
+    im = (T)x.y;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_value(void *v, size_t j, int f64, T re,
+                                            T im) {
+  if (f64)
+    ((double2 *)v)[j] = make_double2((double)re, (double)im);
+  else
+    ((float2 *)v)[j] = make_float2((float)re, (float)im);
+}
+
+// tile index -> per-axis tile coordinates -> region origin b0
+__device__ __forceinline__ void tile_origin(const Geo &g, uint32_t tile,
+                                            int *b0) {
+  uint32_t t[3] = {0, 0, 0};
+  if (g.ndim == 1) {
+    t[0] = tile;
+  } else if (g.ndim == 2) {
+    t[0] = tile / (uint32_t)g.ntiles[1];
+    t[1] = tile % (uint32_t)g.ntiles[1];
+  } else {
+    uint32_t n12 = (uint32_t)g.ntiles[1] * (uint32_t)g.ntiles[2];
+    t[0] = tile / n12;
+    t[1] = (tile / (uint32_t)g.ntiles[2]) % (uint32_t)g.ntiles[1];
+    t[2] = tile % (uint32_t)g.ntiles[2];
+  }
+  for (int d = 0; d < 3; ++d) b0[d] = ((int)t[d] << g.log2tile) - g.nsafe;
+}
+
+template <typename T, int NDIM, int W>
+__device__ __forceinline__ void point_setup(const SpreadArgs &a, uint32_t p,
+                                            uint32_t &j, int *i0, T (*w)[W]) {
+  j = a.idx[p];
+  const size_t cj = a.coords_sorted ? (size_t)p : (size_t)j;
+  const T *cf = (const T *)a.coef;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    double frac;
+    i0[d] = fold_axis(a.g, d, load_coord(a.coords, cj, NDIM, d, a.g.coord_f32),
+                      frac);
+    T x = (T)((double)(W - 1) - 2.0 * frac);
+    kernel_weights<T, W>(x, cf, w[d]);
+  }
+}
+
+template <typename T, int NDIM, int W, bool LDS>
+__global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
+  extern __shared__ unsigned char smem_raw[];
+  const uint32_t total = *a.nsub_total;
+  const int S = a.lds_side;
+  const int tile = 1 << a.g.log2tile;
+  int ncell = S;
+  if (NDIM > 1) ncell *= S;
+  if (NDIM > 2) ncell *= S;
+  T *sre = (T *)smem_raw;
+  T *sim = sre + ncell;
+  T *grid = (T *)a.grid;
+  for (uint32_t sidx = blockIdx.x; sidx < total; sidx += gridDim.x) {
+    const Sub sb = a.subs[sidx];
+    int b0[3];
+    tile_origin(a.g, sb.tile, b0);
+    if (LDS) {
+      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        sre[c] = T(0);
+        sim[c] = T(0);
+      }
+      __syncthreads();
+    }
+    for (uint32_t p = sb.start + threadIdx.x; p < sb.end; p += blockDim.x) {
+      uint32_t j;
+      int i0[3];
+      T w[NDIM][W];
+      point_setup<T, NDIM, W>(a, p, j, i0, w);
+      T vr, vi;
+      load_value<T>(a.values, j, a.val_f64, vr, vi);
+      if (LDS) {
+        int o[3];
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+          o[d] = i0[d] - b0[d];
+          o[d] = o[d] < 0 ? 0 : (o[d] >= tile ? tile - 1 : o[d]);
+        }
+        if constexpr (NDIM == 1) {
+#pragma unroll
+          for (int c0 = 0; c0 < W; ++c0) {
+            atomicAdd(&sre[o[0] + c0], vr * w[0][c0]);
+            atomicAdd(&sim[o[0] + c0], vi * w[0][c0]);
+          }
+        } else if constexpr (NDIM == 2) {
+#pragma unroll
+          for (int c0 = 0; c0 < W; ++c0) {
+            const T fr = vr * w[0][c0], fi = vi * w[0][c0];
+            const int row = (o[0] + c0) * S + o[1];
+#pragma unroll
+            for (int c1 = 0; c1 < W; ++c1) {
+              atomicAdd(&sre[row + c1], fr * w[1][c1]);
+              atomicAdd(&sim[row + c1], fi * w[1][c1]);
+            }
+          }
+        } else {
+#pragma unroll 1
+          for (int c0 = 0; c0 < W; ++c0) {
+#pragma unroll
+            for (int c1 = 0; c1 < W; ++c1) {
+              const T f01 = w[0][c0] * w[1][c1];
+              const T fr = vr * f01, fi = vi * f01;
+              const int row = ((o[0] + c0) * S + o[1] + c1) * S + o[2];
+#pragma unroll
+              for (int c2 = 0; c2 < W; ++c2) {
+                atomicAdd(&sre[row + c2], fr * w[2][c2]);
+                atomicAdd(&sim[row + c2], fi * w[2][c2]);
+              }
+            }
+          }
+        }
+      } else {
+        // direct HBM accumulation with periodic wrap
+        const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
+                  n2 = NDIM > 2 ? a.g.nover[2] : 1;
+#pragma unroll 1
+        for (int c0 = 0; c0 < W; ++c0) {
+          const int g0 = wrapi(i0[0] + c0, n0);
+          if constexpr (NDIM == 1) {
+            atomicAdd(&grid[2 * (size_t)g0], vr * w[0][c0]);
+            atomicAdd(&grid[2 * (size_t)g0 + 1], vi * w[0][c0]);
+          } else {
+#pragma unroll 1
+            for (int c1 = 0; c1 < W; ++c1) {
+              const int g1 = wrapi(i0[1] + c1, n1);
+              const T f01 = w[0][c0] * w[1][c1];
+              if constexpr (NDIM == 2) {
+                size_t gi = (size_t)g0 * n1 + g1;
+                atomicAdd(&grid[2 * gi], vr * f01);
+                atomicAdd(&grid[2 * gi + 1], vi * f01);
+              } else {
+                const size_t rb = ((size_t)g0 * n1 + g1) * n2;
+#pragma unroll
+                for (int c2 = 0; c2 < W; ++c2) {
+                  const size_t gi = rb + wrapi(i0[2] + c2, n2);
+                  atomicAdd(&grid[2 * gi], vr * f01 * w[2][c2]);
+                  atomicAdd(&grid[2 * gi + 1], vi * f01 * w[2][c2]);
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+    if (LDS) {
+      __syncthreads();
+      const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
+                n2 = NDIM > 2 ? a.g.nover[2] : 1;
+      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        const T re = sre[c], im = sim[c];
+        if (re == T(0) && im == T(0)) continue;
+        size_t gi;
+        if constexpr (NDIM == 1) {
+          gi = (size_t)wrapi(b0[0] + c, n0);
+        } else if constexpr (NDIM == 2) {
+          const int cu = c / S, cv = c - cu * S;
+          gi = (size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1);
+        } else {
+          const int cu = c / (S * S), r = c - cu * S * S, cv = r / S,
+                    cw = r - cv * S;
+          gi = ((size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1)) *
+                   n2 +
+               wrapi(b0[2] + cw, n2);
+        }
+        atomicAdd(&grid[2 * gi], re);
+        atomicAdd(&grid[2 * gi + 1], im);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// MODE 0: values (out[j] complex); MODE 1: coordinate gradient
+// (out[j*NDIM+d] complex = d/dcoord_d of the interpolated value).
+template <typename T, int NDIM, int W, bool LDS, int MODE>
+__global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
+                                                int out_f64) {
+  extern __shared__ unsigned char smem_raw[];
+  const uint32_t total = *a.nsub_total;
+  const int S = a.lds_side;
+  const int tile = 1 << a.g.log2tile;
+  int ncell = S;
+  if (NDIM > 1) ncell *= S;
+  if (NDIM > 2) ncell *= S;
+  T *sre = (T *)smem_raw;
+  T *sim = sre + ncell;
+  const T *grid = (const T *)a.grid;
+  const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
+            n2 = NDIM > 2 ? a.g.nover[2] : 1;
+  for (uint32_t sidx = blockIdx.x; sidx < total; sidx += gridDim.x) {
+    const Sub sb = a.subs[sidx];
+    int b0[3];
+    tile_origin(a.g, sb.tile, b0);
+    if (LDS) {
+      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        size_t gi;
+        if constexpr (NDIM == 1) {
+          gi = (size_t)wrapi(b0[0] + c, n0);
+        } else if constexpr (NDIM == 2) {
+          const int cu = c / S, cv = c - cu * S;
+          gi = (size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1);
+        } else {
+          const int cu = c / (S * S), r = c - cu * S * S, cv = r / S,
+                    cw = r - cv * S;
+          gi = ((size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1)) *
+                   n2 +
+               wrapi(b0[2] + cw, n2);
+        }
+        sre[c] = grid[2 * gi];
+        sim[c] = grid[2 * gi + 1];
+      }
+      __syncthreads();
+    }
+    for (uint32_t p = sb.start + threadIdx.x; p < sb.end; p += blockDim.x) {
+      uint32_t j = a.idx[p];
+      const size_t cj = a.coords_sorted ? (size_t)p : (size_t)j;
+      const T *cf = (const T *)a.coef;
+      int i0[3];
+      T w[NDIM][W];
+      T dw[MODE ? NDIM : 1][W];
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) {
+        double frac;
+        i0[d] = fold_axis(a.g, d,
+                          load_coord(a.coords, cj, NDIM, d, a.g.coord_f32),
+                          frac);
+        T x = (T)((double)(W - 1) - 2.0 * frac);
+        if constexpr (MODE == 0)
+          kernel_weights<T, W>(x, cf, w[d]);
+        else
+          kernel_weights_deriv<T, W>(x, cf, w[d], dw[d]);
+      }
+      int o[3];
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) {
+        o[d] = i0[d] - b0[d];
+        o[d] = o[d] < 0 ? 0 : (o[d] >= tile ? tile - 1 : o[d]);
+      }
+      // cell value accessor
+      auto cell = [&](int c0, int c1, int c2, T &re, T &im) {
+        if (LDS) {
+          int c;
+          if constexpr (NDIM == 1) c = o[0] + c0;
+          else if constexpr (NDIM == 2) c = (o[0] + c0) * S + o[1] + c1;
+          else c = ((o[0] + c0) * S + o[1] + c1) * S + o[2] + c2;
+          re = sre[c];
+          im = sim[c];
+        } else {
+          size_t gi;
+          if constexpr (NDIM == 1) gi = (size_t)wrapi(i0[0] + c0, n0);
+          else if constexpr (NDIM == 2)
+            gi = (size_t)wrapi(i0[0] + c0, n0) * n1 + wrapi(i0[1] + c1, n1);
+          else
+            gi = ((size_t)wrapi(i0[0] + c0, n0) * n1 + wrapi(i0[1] + c1, n1)) *
+                     n2 +
+                 wrapi(i0[2] + c2, n2);
+          re = grid[2 * gi];
+          im = grid[2 * gi + 1];
+        }
+      };
+      if constexpr (MODE == 0) {
+        T ar = 0, ai = 0;
+        if constexpr (NDIM == 1) {
+#pragma unroll
+          for (int c0 = 0; c0 < W; ++c0) {
+            T re, im;
+            cell(c0, 0, 0, re, im);
+            ar += re * w[0][c0];
+            ai += im * w[0][c0];
+          }
+        } else if constexpr (NDIM == 2) {
+#pragma unroll
+          for (int c0 = 0; c0 < W; ++c0) {
+            T rr = 0, ri = 0;
+#pragma unroll
+            for (int c1 = 0; c1 < W; ++c1) {
+              T re, im;
+              cell(c0, c1, 0, re, im);
+              rr += re * w[1][c1];
+              ri += im * w[1][c1];
+            }
+            ar += rr * w[0][c0];
+            ai += ri * w[0][c0];
+          }
+        } else {
+#pragma unroll 1
+          for (int c0 = 0; c0 < W; ++c0) {
+            T pr = 0, pi = 0;
+#pragma unroll
+            for (int c1 = 0; c1 < W; ++c1) {
+              T rr = 0, ri = 0;
+#pragma unroll
+              for (int c2 = 0; c2 < W; ++c2) {
+                T re, im;
+                cell(c0, c1, c2, re, im);
+                rr += re * w[2][c2];
+                ri += im * w[2][c2];
+              }
+              pr += rr * w[1][c1];
+              pi += ri * w[1][c1];
+            }
+            ar += pr * w[0][c0];
+            ai += pi * w[0][c0];
+          }
+        }
+        store_value<T>(out, j, out_f64, ar, ai);
+      } else {
+        // gradient: sum over cells of g * dw_d * prod_{e != d} w_e, times
+        // dx/dcoord_d = -2 * nover_d / period_d
+        T gr[NDIM], gi_[NDIM];
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) gr[d] = gi_[d] = T(0);
+#pragma unroll 1
+        for (int c0 = 0; c0 < W; ++c0) {
+#pragma unroll 1
+          for (int c1 = 0; c1 < (NDIM > 1 ? W : 1); ++c1) {
+#pragma unroll
+            for (int c2 = 0; c2 < (NDIM > 2 ? W : 1); ++c2) {
+              T re, im;
+              cell(c0, c1, c2, re, im);
+              T f[NDIM];
+#pragma unroll
+              for (int d = 0; d < NDIM; ++d) {
+                T prod = T(1);
+#pragma unroll
+                for (int e = 0; e < NDIM; ++e) {
+                  const int ce = e == 0 ? c0 : (e == 1 ? c1 : c2);
+                  prod *= (e == d) ? dw[e][ce] : w[e][ce];
+                }
+                f[d] = prod;
+              }
+#pragma unroll
+              for (int d = 0; d < NDIM; ++d) {
+                gr[d] += re * f[d];
+                gi_[d] += im * f[d];
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+          const T s = (T)(-2.0 * (double)a.g.nover[d] * a.g.coordfct[d]);
+          store_value<T>(out, (size_t)j * NDIM + d, out_f64, gr[d] * s,
+                         gi_[d] * s);
+        }
+      }
+    }
+    if (LDS) __syncthreads();
+  }
+}
+
+}  // namespace dmx

@@ -1,0 +1,147 @@
+/*
+ * ducc_amd.h -- C ABI of the MI355X-native NUFFT (types 1 and 2, 1-3 D).
+ *
+ * Drop-in boundary for ducc0.nufft (python/nufft_pymod.cc:445-471): every
+ * entry point is extern "C", noexcept in effect, returns 0 on success and a
+ * nonzero code on failure, with the message available from dmx_last_error()
+ * (thread-local).  Arguments are validated on the host before any GPU work;
+ * messages follow ducc0's MR_assert texts where ducc0 has one.
+ *
+ * Arrays are passed as dmx_array descriptors (role of ducc0's
+ * ArrayDescriptor, src/ducc0/bindings/array_descriptor.h:34-47), in C
+ * (row-major) axis order, strides in elements.  `device` = 0 means host
+ * memory (copied in/out by the library), 1 means device memory on the
+ * current HIP device (used in place).  The caller owns every buffer.
+ */
+#ifndef DUCC_AMD_H
+#define DUCC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* typecodes (role of src/ducc0/bindings/typecode.h:44-80) */
+enum {
+  DMX_F32 = 3,
+  DMX_F64 = 7,
+  DMX_C64 = 67,
+  DMX_C128 = 71,
+  DMX_U32 = 34
+};
+
+typedef struct dmx_array {
+  void *data;
+  int64_t ndim;
+  int64_t shape[10];
+  int64_t stride[10]; /* in elements */
+  int32_t typecode;
+  int32_t device; /* 0 host, 1 device */
+} dmx_array;
+
+/* Options with no ducc0 counterpart (precision policy, parameter chooser,
+   stream).  A NULL pointer means all defaults. */
+typedef struct dmx_options {
+  int32_t precision; /* 0 auto (FP32 when eps >= fp64_below and the
+                        single-precision kernel table reaches eps), 1 FP32,
+                        2 FP64 */
+  int32_t chooser;   /* 0 analytic (ducc0's cost model), 1 learned */
+  int32_t log2tile;  /* 0 automatic, else tile side 2^log2tile */
+  int32_t kernel_idx;/* -1 automatic, else force this kernel table row */
+  double fp64_below; /* auto policy switch, default 1e-6 */
+  void *stream;      /* hipStream_t to run on; NULL: library stream */
+  double sigma;      /* >0: force this oversampling factor (with kernel_idx) */
+  int32_t reserved[8];
+} dmx_options;
+
+typedef struct dmx_plan dmx_plan;
+
+/* Parameters a plan or call actually used. */
+typedef struct dmx_plan_info {
+  int32_t ndim, W, kernel_idx, fp64, log2tile, chooser_used;
+  int64_t nuni[3], nover[3];
+  double ofactor, eps_tab, epsilon;
+  int64_t npoints;
+} dmx_plan_info;
+
+/* Last error message of the calling thread ("" when none). */
+const char *dmx_last_error(void);
+
+/* Fill `o` with the defaults. */
+void dmx_default_options(dmx_options *o);
+
+/* ducc0.nufft.bestEpsilon (gridding_kernel.cc:1562-1574); negative on error. */
+double dmx_best_epsilon(size_t ndim, int singleprec, double sigma_min,
+                        double sigma_max);
+
+/* ducc0.nufft.u2nu (python/nufft_pymod.cc:52-94; nufft.h:1929-1958).
+   grid: complex nuni array; coord: (M, ndim) real; out: (M,) complex.
+   periodicity: ndim doubles.  `info` (may be NULL) receives the parameters
+   used. */
+int dmx_u2nu(const dmx_array *grid, const dmx_array *coord, int forward,
+             double epsilon, size_t nthreads, dmx_array *out,
+             size_t verbosity, double sigma_min, double sigma_max,
+             const double *periodicity, int fft_order,
+             const dmx_options *opt, dmx_plan_info *info);
+
+/* ducc0.nufft.nu2u (python/nufft_pymod.cc:96-137; nufft.h:1897-1927). */
+int dmx_nu2u(const dmx_array *points, const dmx_array *coord, int forward,
+             double epsilon, size_t nthreads, dmx_array *out,
+             size_t verbosity, double sigma_min, double sigma_max,
+             const double *periodicity, int fft_order,
+             const dmx_options *opt, dmx_plan_info *info);
+
+/* ducc0.nufft.plan (python/nufft_pymod.cc:139-261; nufft.h:437-503):
+   sorts the points once; the plan keeps the sorted index and a sorted copy
+   of the coordinates on the device. */
+int dmx_plan_create(dmx_plan **plan, int nu2u, const dmx_array *coord,
+                    const int64_t *grid_shape, size_t ndim, double epsilon,
+                    size_t nthreads, double sigma_min, double sigma_max,
+                    const double *periodicity, int fft_order,
+                    const dmx_options *opt);
+int dmx_plan_nu2u(dmx_plan *plan, int forward, size_t verbosity,
+                  const dmx_array *points, dmx_array *out);
+int dmx_plan_u2nu(dmx_plan *plan, int forward, size_t verbosity,
+                  const dmx_array *grid, dmx_array *out);
+int dmx_plan_get_info(const dmx_plan *plan, dmx_plan_info *info);
+int dmx_plan_destroy(dmx_plan *plan);
+
+/* Gradient of u2nu with respect to the point coordinates (no ducc0
+   counterpart; used by the autograd wrapper): for every point j and axis d,
+   out[j*ndim+d] = d u2nu(grid)_j / d coord[j,d], computed by interpolating
+   with the derivative of the spreading kernel.  Same arguments as dmx_u2nu;
+   out is an (M, ndim) complex array of the grid's precision. */
+int dmx_u2nu_coord_grad(const dmx_array *grid, const dmx_array *coord,
+                        int forward, double epsilon, dmx_array *out,
+                        double sigma_min, double sigma_max,
+                        const double *periodicity, int fft_order,
+                        const dmx_options *opt);
+
+/* Direct FP64 NUDFT on the device, the spot-check used by the parameter
+   chooser and the error reports (sum of test_nufft.py:32-43).
+   type 1: out[q] = sum_j pts_j exp(isign i 2pi/P k(q).x_j) for the modes
+   listed in `sel` (flat user indices, int64); type 2: out[q] = sum_k
+   grid_k exp(isign i 2pi/P k.x_{sel[q]}).  All arrays device or host,
+   complex128 / float64. */
+int dmx_nudft(int type, const dmx_array *coord, const dmx_array *values,
+              const int64_t *shape, size_t ndim, int forward,
+              const double *periodicity, int fft_order, const int64_t *sel,
+              size_t nsel, dmx_array *out);
+
+/* Stable sort of points into tiles (the index build of nufft.h:1364-1381):
+   returns the tile keys and the permutation for given geometry.  Exposed for
+   bit-exact parity tests.  keys/idx: device or host uint32 arrays of M. */
+int dmx_sort_points(const dmx_array *coord, const int64_t *nover,
+                    size_t ndim, int W, int log2tile,
+                    const double *periodicity, dmx_array *keys,
+                    dmx_array *idx);
+
+/* Number of HIP devices visible (0 when none). */
+int dmx_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
