@@ -1,0 +1,274 @@
+#!/usr/bin/env python
+"""Throughput of the MI355X NUFFT on the 2-D type-1 + type-2 workload.
+
+Workload: 2048 x 2048 uniform grid, M = 10,000,000 nonuniform points drawn
+from 8 Gaussian clusters (sigma 0.3 rad, centres uniform, wrapped to
+[-pi, pi)), float64 coordinates, complex64 values, eps = 1e-5.  One step is
+one nu2u (type 1) plus one u2nu (type 2) on that geometry, inputs resident
+in HBM.  The headline value is 2*M*steps / time for the fast path (FP32
+compute with FP64 folding, learned parameter choice when a trained network
+is present); the plain FP64 path (complex128, ducc0's analytic parameters)
+is timed on the same geometry for the speed-up, and ducc0 itself (built from
+source into oracle/_ref) is timed on the host cores as the CPU baseline.
+
+Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K]
+[--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def clustered_points(rng, M, d=2, ncl=8, sig=0.3):
+    cen = rng.uniform(-np.pi, np.pi, (ncl, d))
+    lab = rng.integers(0, ncl, M)
+    x = cen[lab] + rng.normal(0.0, sig, (M, d))
+    return (x + np.pi) % (2 * np.pi) - np.pi
+
+
+def complex_gauss(rng, shape, dtype):
+    z = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    return (z * np.sqrt(0.5)).astype(dtype)
+
+
+def alg_bytes(phase, info, M, d, sv):
+    """Algorithmic HBM bytes of one launch of a single-kernel phase:
+    spreading / interpolation read each point's coordinates (8 B per axis),
+    its sort index (4 B) and value (sv B) once and write / read the
+    oversampled grid once; an FFT axis pass reads and writes the lines it
+    transforms once."""
+    nover = info["nover"]
+    nuni = info["nuni"]
+    n_over = int(np.prod(nover))
+    if phase in ("spreading", "interpolation"):
+        return M * (d * 8 + 4 + sv) + n_over * sv
+    if phase.startswith("FFT axis"):
+        return None
+    return None
+
+
+def fft_pass_bytes(info, type1, sv):
+    """Bytes of each FFT axis pass with the reference's pruning: the lines
+    along axis k run over the kept bands (nuni of nover) of every later
+    axis and over all of every earlier axis, for both transform types."""
+    nover, nuni = info["nover"], info["nuni"]
+    nd = len(nover)
+    out = {}
+    for k in range(nd):
+        n = 1
+        for a in range(nd):
+            n *= nuni[a] if a > k else nover[a]
+        out[f"FFT axis {k}"] = 2 * n * sv
+    return out
+
+
+def time_steps(step, steps, warmup, sync):
+    for _ in range(warmup):
+        step()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    sync()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(coords, pts, grid, eps, M_total):
+    """ducc0 (compiled from the reference sources into oracle/_ref) on all
+    host cores, on the same workload."""
+    try:
+        from oracle import ref as R
+    except Exception:
+        return None
+    if not R.available():
+        return None
+    nth = os.cpu_count() or 1
+    shape = grid.shape
+    # warm-up on a small slice, then one full nu2u + u2nu
+    R.nu2u(pts[:10000], coords[:10000], True, eps, shape, nthreads=nth)
+    t0 = time.perf_counter()
+    R.nu2u(pts, coords, True, eps, shape, nthreads=nth)
+    t1 = time.perf_counter()
+    R.u2nu(grid, coords, False, eps, nthreads=nth)
+    t2 = time.perf_counter()
+    return dict(value=2 * len(pts) / (t2 - t0), unit="points/s", cores=nth,
+                kind="reference",
+                sample=(f"ducc0 nu2u + u2nu, full workload (M={len(pts)}, "
+                        f"{shape[0]}x{shape[1]}, eps={eps}, complex64, "
+                        f"{nth} threads), one timed pass after a warm-up; "
+                        f"nu2u {t1 - t0:.3f} s, u2nu {t2 - t1:.3f} s"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--M", type=int, default=10_000_000)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--eps", type=float, default=1e-5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--chooser", default="auto",
+                    choices=["auto", "learned", "analytic"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU "
+              "path (see DESIGN.md for how the NUFFT would shard); run with "
+              "--gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ducc_amd import _lib, nufft
+    from ducc_amd import chooser as learned
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(102)
+    M, n, eps = args.M, args.n, args.eps
+    c_np = clustered_points(rng, M)
+    p_np = complex_gauss(rng, M, np.complex64)
+    g_np = complex_gauss(rng, (n, n), np.complex64)
+    c = torch.from_numpy(c_np).to(dev)
+    p32 = torch.from_numpy(p_np).to(dev)
+    g32 = torch.from_numpy(g_np).to(dev)
+    og32 = torch.empty((n, n), dtype=torch.complex64, device=dev)
+    op32 = torch.empty(M, dtype=torch.complex64, device=dev)
+    chooser = args.chooser
+    if chooser == "auto":
+        chooser = "learned" if learned.available() else "analytic"
+    sync = torch.cuda.synchronize
+
+    info1, info2 = {}, {}
+
+    def step_fast():
+        nufft.nu2u(points=p32, coord=c, forward=True, epsilon=eps, out=og32,
+                   chooser=chooser, info=info1)
+        nufft.u2nu(grid=g32, coord=c, forward=False, epsilon=eps, out=op32,
+                   chooser=chooser, info=info2)
+
+    _lib.profile_enable(True)
+    for _ in range(args.warmup):
+        step_fast()
+    sync()
+    _lib.profile_reset()
+    t = time_steps(step_fast, args.steps, 0, sync)
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    value = 2 * M * args.steps / t
+
+    # plain FP64 path on the same geometry
+    fp64 = None
+    if not args.no_fp64:
+        p64 = p32.to(torch.complex128)
+        g64 = g32.to(torch.complex128)
+        og64 = torch.empty((n, n), dtype=torch.complex128, device=dev)
+        op64 = torch.empty(M, dtype=torch.complex128, device=dev)
+        info3, info4 = {}, {}
+
+        def step_fp64():
+            nufft.nu2u(points=p64, coord=c, forward=True, epsilon=eps,
+                       out=og64, precision="fp64", chooser="analytic",
+                       info=info3)
+            nufft.u2nu(grid=g64, coord=c, forward=False, epsilon=eps,
+                       out=op64, precision="fp64", chooser="analytic",
+                       info=info4)
+        t64 = time_steps(step_fp64, max(1, args.steps // 2), 1, sync)
+        v64 = 2 * M * max(1, args.steps // 2) / t64
+        # accuracy of the fast path against the FP64 path and the direct sum
+        a1 = og32.cpu().numpy().astype(np.complex128)
+        b1 = og64.cpu().numpy()
+        a2 = op32.cpu().numpy().astype(np.complex128)
+        b2 = op64.cpu().numpy()
+
+        def l2(a, b):
+            return float(np.sqrt(np.sum(np.abs(a - b) ** 2) /
+                                 max(np.sum(np.abs(a) ** 2),
+                                     np.sum(np.abs(b) ** 2))))
+        sel_m = rng.choice(n * n, 256, replace=False)
+        sel_p = rng.choice(M, 256, replace=False)
+        d1 = nufft.nudft_direct(1, c_np, p_np.astype(np.complex128), (n, n),
+                                True, sel_m)
+        d2 = nufft.nudft_direct(2, c_np, g_np.astype(np.complex128), (n, n),
+                                False, sel_p)
+        fp64 = dict(value=v64, ms_per_step=1e3 * t64 / max(1, args.steps // 2),
+                    W=info3.get("W"), nover=info3.get("nover"),
+                    speedup=value / v64,
+                    err_fast_vs_fp64=dict(nu2u=l2(a1, b1), u2nu=l2(a2, b2)),
+                    err_vs_direct_nudft=dict(
+                        nu2u_fast=l2(a1.ravel()[sel_m], d1),
+                        nu2u_fp64=l2(b1.ravel()[sel_m], d1),
+                        u2nu_fast=l2(a2[sel_p], d2),
+                        u2nu_fp64=l2(b2[sel_p], d2)))
+        del p64, g64, og64, op64
+
+    # roofline of the dominant single-kernel phase
+    sv = 8
+    d = 2
+    per_launch = {}
+    for ph, (ms, cnt) in prof.items():
+        per_launch[ph] = ms / max(cnt, 1)
+    cand = {ph: ms for ph, (ms, cnt) in prof.items()
+            if ph in ("nu2u/spreading", "u2nu/interpolation")
+            or "/FFT axis" in ph}
+    dom = max(cand, key=cand.get)
+    kind, name = dom.split("/", 1)
+    inf = info1 if kind == "nu2u" else info2
+    if name.startswith("FFT axis"):
+        b = fft_pass_bytes(inf, kind == "nu2u", sv)[name]
+    else:
+        b = alg_bytes(name, inf, M, d, sv)
+    achieved = b / (per_launch[dom] * 1e-3) / 1e9
+    traffic = None
+    tp = os.path.join(HERE, "profiles", "traffic.json")
+    if os.path.exists(tp):
+        try:
+            traffic = json.load(open(tp)).get(dom)
+        except Exception:
+            traffic = None
+    roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1),
+                    peak=PEAK_HBM_GBS, unit="GB/s",
+                    frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic,
+                    alg_bytes_per_launch=int(b),
+                    avg_launch_ms=round(per_launch[dom], 4))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(c_np, p_np, g_np, eps, M)
+
+    line = dict(
+        metric="nonuniform pts/s (type1+type2) @eps 1e-5: FP32+learned vs "
+               "plain FP64 GPU",
+        value=round(value, 1), unit="points/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=round(1e3 * t / args.steps, 4),
+        higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="fp32 (fp64 coordinate fold)", data="synthetic",
+        config=dict(workload="2D NUFFT type1+type2, 2048x2048 grid, "
+                             "M=10,000,000 points in 8 Gaussian clusters "
+                             "(sigma 0.3 rad), eps=1e-5, complex64 values, "
+                             "float64 coords",
+                    M=M, grid=[n, n], eps=eps, chooser=chooser,
+                    nu2u=dict(W=info1.get("W"), nover=info1.get("nover"),
+                              ofactor=info1.get("ofactor"),
+                              log2tile=info1.get("log2tile"),
+                              learned=info1.get("chooser_used")),
+                    u2nu=dict(W=info2.get("W"), nover=info2.get("nover"),
+                              ofactor=info2.get("ofactor"),
+                              log2tile=info2.get("log2tile"),
+                              learned=info2.get("chooser_used"))),
+        roofline=roofline, cpu_baseline=cpu, fp64_plain=fp64,
+        phases_ms_per_step={k: round(v[0] / args.steps, 4)
+                            for k, v in prof.items()})
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

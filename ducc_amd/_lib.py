@@ -80,6 +80,9 @@ _SIG = {
                              _P, ct.c_int, _P, ct.c_size_t, _AP]),
     "dmx_sort_points": (ct.c_int, [_AP, _P, ct.c_size_t, ct.c_int, ct.c_int,
                                    _P, _AP, _AP]),
+    "dmx_profile_enable": (ct.c_int, [ct.c_int]),
+    "dmx_profile_reset": (ct.c_int, []),
+    "dmx_profile_read": (ct.c_int, [ct.c_char_p, ct.c_size_t]),
     "dmx_device_count": (ct.c_int, []),
 }
 
@@ -156,6 +159,25 @@ def describe(x):
     a.typecode = _NP_TC[x.dtype]
     a.device = 0
     return a, x
+
+
+def profile_enable(on=True):
+    check(lib().dmx_profile_enable(int(bool(on))))
+
+
+def profile_reset():
+    check(lib().dmx_profile_reset())
+
+
+def profile_read():
+    """{phase: (total_ms, count)} accumulated since the last reset."""
+    buf = ct.create_string_buffer(1 << 16)
+    check(lib().dmx_profile_read(buf, len(buf)))
+    res = {}
+    for ln in buf.value.decode().splitlines():
+        name, ms, cnt = ln.split("\t")
+        res[name] = (float(ms), int(cnt))
+    return res
 
 
 def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,

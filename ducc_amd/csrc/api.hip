@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ducc_amd.h"
+#include "ducc_amd.h"
 #include "chooser.h"
 #include "grid.h"
 #include "params.h"
@@ -401,6 +401,7 @@ int dmx_plan_nu2u(dmx_plan *plan, int forward, size_t verbosity,
     DevOut uo(*out, true);
     plan->plan->nu2u(forward != 0, pin.p, f64, uo.p, f64, uo.st);
     uo.finish(s);
+    plan->plan->flush_profile();
   });
 }
 
@@ -423,6 +424,7 @@ int dmx_plan_u2nu(dmx_plan *plan, int forward, size_t verbosity,
     DevOut po(*out, false);
     plan->plan->u2nu(forward != 0, gin.p, f64, gin.st, po.p, f64);
     po.finish(s);
+    plan->plan->flush_profile();
   });
 }
 
@@ -526,6 +528,23 @@ int dmx_sort_points(const dmx_array *coord, const int64_t *nover,
     }
     ko.finish(s);
     io.finish(s);
+  });
+}
+
+int dmx_profile_enable(int on) {
+  return guard([&] { Profiler::get().enable(on != 0); });
+}
+
+int dmx_profile_reset(void) {
+  return guard([&] { Profiler::get().reset(); });
+}
+
+int dmx_profile_read(char *buf, size_t buflen) {
+  return guard([&] {
+    DMX_ASSERT(buf && buflen, "null argument");
+    std::string r = Profiler::get().report();
+    DMX_ASSERT(r.size() + 1 <= buflen, "profile buffer too small");
+    std::memcpy(buf, r.c_str(), r.size() + 1);
   });
 }
 

@@ -21,7 +21,10 @@ SOURCES = ["params.cpp", "runtime.hip", "api.hip", "sort.hip", "spread.hip",
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DMX_ARCH", "gfx950")
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+INCLUDE = os.environ.get("DMX_INCLUDE",
+                         os.path.join(os.path.dirname(PKG), "include"))
+COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{INCLUDE}", "-I" + HERE,
+          "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 DEVICE = [f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -173,6 +173,9 @@ void launch_build_subs(const uint32_t *counts, const uint32_t *tile_start,
                        size_t ntiles, uint32_t *nsub, uint32_t *sub_start,
                        Sub *subs, uint32_t *nsub_total, void *scanwork,
                        hipStream_t s);
+void launch_tile_bounds(const uint32_t *skeys, size_t M, size_t ntiles,
+                        uint32_t *tstart, uint32_t *tend, uint32_t *counts,
+                        hipStream_t s);
 void launch_gather_coords(const void *coords, int coord_f32, int ndim,
                           const uint32_t *idx, size_t M, void *out,
                           hipStream_t s);

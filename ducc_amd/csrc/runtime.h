@@ -9,7 +9,7 @@
 #include <mutex>
 #include <vector>
 
-#include "../../include/ducc_amd.h"
+#include "ducc_amd.h"
 #include "dmx.h"
 #include "fft.h"
 
@@ -50,6 +50,25 @@ struct Buf {
     bytes = 0;
   }
   template <typename T> T *as() const { return (T *)p; }
+};
+
+// Phase timing with HIP events recorded on the plan's stream, under the
+// phase names of the reference's TimerHierarchy (nufft.h:840-861,
+// 1326-1358).  Off unless enabled; totals accumulate per process.
+class Profiler {
+ public:
+  static Profiler &get();
+  bool enabled() const { return on_; }
+  void enable(bool on) { on_ = on; }
+  void add(const std::string &name, double ms);
+  void reset();
+  std::string report();
+
+ private:
+  bool on_ = false;
+  std::mutex mu_;
+  std::vector<std::string> order_;
+  std::map<std::string, std::pair<double, long>> acc_;
 };
 
 struct Params {
@@ -98,8 +117,24 @@ class Plan {
   hipStream_t stream() const { return s_; }
   size_t npoints() const { return M_; }
   Geo geo() const { return g_; }
+  // resolves recorded phase events (after the stream has been synchronised)
+  void flush_profile();
+
+  struct Phase {
+    Plan *p;
+    std::string name;
+    hipEvent_t a = nullptr, b = nullptr;
+    Phase(Plan *pl, std::string n);
+    ~Phase();
+  };
 
  private:
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pend_;
+  std::string tag_ = "plan";  // "index", "nu2u" or "u2nu": phase prefix
   void prepare_grid_side();
   void run_fft(bool forward, bool type1);
   void fill_spread_args(SpreadArgs &a) const;
@@ -118,6 +153,7 @@ class Plan {
   std::vector<Buf> tw_;
   std::vector<int> tw_len_;
   Buf idx_, subs_, nsub_total_, sorted_coords_;
+  Buf tmp_keys_, tmp_skeys_, tmp_tiles_, tmp_work_;
   Buf grid_, scratch_;
   const void *twiddle_for(int L);
 };

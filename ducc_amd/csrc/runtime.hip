@@ -71,6 +71,63 @@ void Pool::trim() {
   free_.clear();
 }
 
+// -------------------------------------------------------------- profiler
+Profiler &Profiler::get() {
+  static Profiler *p = new Profiler();
+  return *p;
+}
+void Profiler::add(const std::string &name, double ms) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = acc_.find(name);
+  if (it == acc_.end()) {
+    order_.push_back(name);
+    acc_[name] = {ms, 1};
+  } else {
+    it->second.first += ms;
+    it->second.second += 1;
+  }
+}
+void Profiler::reset() {
+  std::lock_guard<std::mutex> lk(mu_);
+  acc_.clear();
+  order_.clear();
+}
+std::string Profiler::report() {
+  std::lock_guard<std::mutex> lk(mu_);
+  std::string s;
+  for (auto &n : order_) {
+    auto &v = acc_[n];
+    s += n + "\t" + std::to_string(v.first) + "\t" + std::to_string(v.second) +
+         "\n";
+  }
+  return s;
+}
+
+Plan::Phase::Phase(Plan *pl, std::string n) : p(pl) {
+  if (!Profiler::get().enabled()) return;
+  name = p->tag_ + "/" + n;
+  DMX_HIP(hipEventCreate(&a));
+  DMX_HIP(hipEventCreate(&b));
+  DMX_HIP(hipEventRecord(a, p->s_));
+}
+Plan::Phase::~Phase() {
+  if (!a) return;
+  (void)hipEventRecord(b, p->s_);
+  p->pend_.push_back({name, a, b});
+}
+
+void Plan::flush_profile() {
+  for (auto &q : pend_) {
+    float ms = 0;
+    if (hipEventSynchronize(q.b) == hipSuccess &&
+        hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess)
+      Profiler::get().add(q.name, ms);
+    (void)hipEventDestroy(q.a);
+    (void)hipEventDestroy(q.b);
+  }
+  pend_.clear();
+}
+
 // -------------------------------------------------------- parameter choice
 namespace {
 int default_log2tile(int ndim, bool fp64) {
@@ -78,10 +135,13 @@ int default_log2tile(int ndim, bool fp64) {
   if (ndim == 2) return fp64 ? 4 : 5;
   return 3;
 }
+// LDS of the tile region: re and im planes of FP64 accumulators (spreading;
+// interpolation stages the same region in the compute type)
 size_t lds_bytes(int ndim, int tile, int W, bool fp64) {
+  (void)fp64;
   size_t S = (size_t)tile + W - 1, c = S;
   for (int d = 1; d < ndim; ++d) c *= S;
-  return c * (fp64 ? 16 : 8);  // re and im planes
+  return c * 16;
 }
 constexpr size_t LDS_LIMIT = 65536;
 
@@ -225,7 +285,10 @@ Plan::Plan(const Params &p, hipStream_t s) : p_(p), s_(s) {
   DMX_HIP(hipStreamSynchronize(s_));
 }
 
-Plan::~Plan() { (void)hipStreamSynchronize(s_); }
+Plan::~Plan() {
+  (void)hipStreamSynchronize(s_);
+  flush_profile();
+}
 
 const void *Plan::twiddle_for(int L) {
   for (size_t i = 0; i < tw_len_.size(); ++i)
@@ -242,6 +305,7 @@ const void *Plan::twiddle_for(int L) {
 
 void Plan::build_index(const void *coords, bool coord_f32, size_t M,
                        bool keep_sorted_coords) {
+  tag_ = "index";
   M_ = M;
   coord_f32_ = coord_f32;
   g_.coord_f32 = coord_f32 ? 1 : 0;
@@ -250,33 +314,36 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
   if (M == 0) return;
   int keybits = 1;
   while (((size_t)1 << keybits) < ntiles_) ++keybits;
-  Buf keys(M * 4), counts(ntiles_ * 4), tstart(ntiles_ * 4),
-      nsub(ntiles_ * 4), sstart(ntiles_ * 4);
+  // temporaries live as long as the plan, so nothing here waits for the
+  // stream (the pool may hand freed blocks to other streams)
   size_t ws = std::max(radix_sort_workspace(M), scan_workspace(ntiles_));
-  Buf work(ws);
-  launch_zero(counts.p, ntiles_ * 4, s_);
-  launch_fold_keys(g_, coords, M, keys.as<uint32_t>(), counts.as<uint32_t>(),
-                   s_);
+  tmp_keys_.reset(M * 4);
+  tmp_skeys_.reset(M * 4);
+  tmp_tiles_.reset(ntiles_ * 4 * 5);
+  tmp_work_.reset(ws);
+  uint32_t *counts = tmp_tiles_.as<uint32_t>(), *tstart = counts + ntiles_,
+           *tend = tstart + ntiles_, *nsub = tend + ntiles_,
+           *sstart = nsub + ntiles_;
+  Phase ph(this, "building index");
+  launch_fold_keys(g_, coords, M, tmp_keys_.as<uint32_t>(), nullptr, s_);
   idx_.reset(M * 4);
-  radix_sort_pairs(keys.as<uint32_t>(), M, keybits, idx_.as<uint32_t>(),
-                   nullptr, work.p, ws, s_);
-  exclusive_scan_u32(counts.as<uint32_t>(), tstart.as<uint32_t>(), ntiles_,
-                     nullptr, work.p, s_);
+  radix_sort_pairs(tmp_keys_.as<uint32_t>(), M, keybits, idx_.as<uint32_t>(),
+                   tmp_skeys_.as<uint32_t>(), tmp_work_.p, ws, s_);
+  launch_tile_bounds(tmp_skeys_.as<uint32_t>(), M, ntiles_, tstart, tend,
+                     counts, s_);
   nsub_max_ = std::min(ntiles_, M) + (M + MAXSUB - 1) / MAXSUB;
   subs_.reset(nsub_max_ * sizeof(Sub));
   nsub_total_.reset(4);
-  launch_build_subs(counts.as<uint32_t>(), tstart.as<uint32_t>(), ntiles_,
-                    nsub.as<uint32_t>(), sstart.as<uint32_t>(),
-                    subs_.as<Sub>(), nsub_total_.as<uint32_t>(), work.p, s_);
+  launch_build_subs(counts, tstart, ntiles_, nsub, sstart, subs_.as<Sub>(),
+                    nsub_total_.as<uint32_t>(), tmp_work_.p, s_);
   if (keep_sorted_coords) {
+    Phase ps(this, "sorting coords");
     sorted_coords_.reset(M * p_.ndim * (coord_f32 ? 4 : 8));
     launch_gather_coords(coords, coord_f32, p_.ndim, idx_.as<uint32_t>(), M,
                          sorted_coords_.p, s_);
     coords_ = sorted_coords_.p;
     coords_sorted_ = true;
   }
-  // temporaries go back to the pool; the stream must finish with them first
-  DMX_HIP(hipStreamSynchronize(s_));
 }
 
 uint32_t *Plan::keys_copy_to(uint32_t *dev_keys_out) {
@@ -350,15 +417,21 @@ void Plan::run_fft(bool forward, bool type1) {
     range(qax, qax > a, L.nv, L.vn1, L.va1, L.va2);
     if (L.L <= 1) continue;
     L.tw = twiddle_for(L.L);
+    Phase ph(this, std::string("FFT axis ") + std::to_string(a - f));
     fft_lineset(L, p_.fp64, scratch_.p, s_);
   }
 }
 
 void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
                 bool uni_f64, const int64_t *ust) {
+  tag_ = "nu2u";
   prepare_grid_side();
-  launch_zero(grid_.p, grid_.bytes, s_);
+  {
+    Phase ph(this, "zeroing grid");
+    launch_zero(grid_.p, grid_.bytes, s_);
+  }
   if (M_ > 0) {
+    Phase ph(this, "spreading");
     SpreadArgs a{};
     fill_spread_args(a);
     a.values = points;
@@ -385,13 +458,18 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
     cf[d] = corfac_[e].as<double>();
   }
   c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
+  Phase ph(this, "grid correction");
   launch_grid_copy(p_.fp64, false, c, s_);
 }
 
 void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
                 const int64_t *ust, void *out, bool out_f64) {
+  tag_ = "u2nu";
   prepare_grid_side();
-  launch_zero(grid_.p, grid_.bytes, s_);
+  {
+    Phase ph(this, "zeroing grid");
+    launch_zero(grid_.p, grid_.bytes, s_);
+  }
   GridCopyArgs c{};
   c.grid = grid_.p;
   c.uni = const_cast<void *>(uni);
@@ -410,20 +488,28 @@ void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
     cf[d] = corfac_[e].as<double>();
   }
   c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
-  launch_grid_copy(p_.fp64, true, c, s_);
+  {
+    Phase ph(this, "grid correction");
+    launch_grid_copy(p_.fp64, true, c, s_);
+  }
   run_fft(forward, false);
   if (M_ > 0) {
     SpreadArgs a{};
     fill_spread_args(a);
     a.grid = grid_.p;
+    Phase ph(this, "interpolation");
     launch_interp(p_.fp64, a, out, out_f64 ? 1 : 0, s_);
   }
 }
 
 void Plan::u2nu_grad(bool forward, const void *uni, bool uni_f64,
                      const int64_t *ust, void *out, bool out_f64) {
+  tag_ = "u2nu_grad";
   prepare_grid_side();
-  launch_zero(grid_.p, grid_.bytes, s_);
+  {
+    Phase ph(this, "zeroing grid");
+    launch_zero(grid_.p, grid_.bytes, s_);
+  }
   GridCopyArgs c{};
   c.grid = grid_.p;
   c.uni = const_cast<void *>(uni);
@@ -442,12 +528,16 @@ void Plan::u2nu_grad(bool forward, const void *uni, bool uni_f64,
     cf[d] = corfac_[e].as<double>();
   }
   c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
-  launch_grid_copy(p_.fp64, true, c, s_);
+  {
+    Phase ph(this, "grid correction");
+    launch_grid_copy(p_.fp64, true, c, s_);
+  }
   run_fft(forward, false);
   if (M_ > 0) {
     SpreadArgs a{};
     fill_spread_args(a);
     a.grid = grid_.p;
+    Phase ph(this, "interpolation (derivative)");
     launch_interp_grad(p_.fp64, a, out, out_f64 ? 1 : 0, s_);
   }
 }

@@ -188,6 +188,25 @@ __global__ __launch_bounds__(ST) void k_scan_add(uint32_t *out, size_t n,
     if (base + i < n) out[base + i] += a;
 }
 
+// ------------------------------------------------------------ tile bounds
+// first and one-past-last sorted position of every non-empty tile, from the
+// sorted keys (no atomics: hot tiles of clustered points would serialise a
+// histogram); empty tiles keep start = end = 0.
+__global__ void k_tile_bounds(const uint32_t *skeys, size_t M,
+                              uint32_t *tstart, uint32_t *tend) {
+  size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  const uint32_t k = skeys[p];
+  if (p == 0 || skeys[p - 1] != k) tstart[k] = (uint32_t)p;
+  if (p == M - 1 || skeys[p + 1] != k) tend[k] = (uint32_t)(p + 1);
+}
+
+__global__ void k_counts(const uint32_t *tstart, const uint32_t *tend,
+                         size_t ntiles, uint32_t *counts) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntiles) counts[t] = tend[t] - tstart[t];
+}
+
 // ------------------------------------------------------------ subproblems
 __global__ void k_nsub(const uint32_t *counts, size_t ntiles, uint32_t *nsub) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,6 +325,20 @@ void launch_build_subs(const uint32_t *counts, const uint32_t *tile_start,
   hipLaunchKernelGGL(k_fill_subs, dim3(nblk(ntiles, 256)), dim3(256), 0, s,
                      counts, tile_start, (const uint32_t *)nsub,
                      (const uint32_t *)sub_start, ntiles, subs);
+  DMX_HIP(hipGetLastError());
+}
+
+void launch_tile_bounds(const uint32_t *skeys, size_t M, size_t ntiles,
+                        uint32_t *tstart, uint32_t *tend, uint32_t *counts,
+                        hipStream_t s) {
+  launch_zero(tstart, ntiles * 4, s);
+  launch_zero(tend, ntiles * 4, s);
+  if (M)
+    hipLaunchKernelGGL(k_tile_bounds, dim3(nblk(M, 256)), dim3(256), 0, s,
+                       skeys, M, tstart, tend);
+  hipLaunchKernelGGL(k_counts, dim3(nblk(ntiles, 256)), dim3(256), 0, s,
+                     (const uint32_t *)tstart, (const uint32_t *)tend, ntiles,
+                     counts);
   DMX_HIP(hipGetLastError());
 }
 

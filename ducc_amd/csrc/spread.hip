@@ -13,7 +13,7 @@ void go(const SpreadArgs &a, hipStream_t s) {
   if (a.use_lds) {
     size_t cells = a.lds_side;
     for (int d = 1; d < NDIM; ++d) cells *= a.lds_side;
-    size_t sh = 2 * cells * sizeof(T);
+    size_t sh = 2 * cells * sizeof(Acc);
     hipLaunchKernelGGL((k_spread<T, NDIM, W, true>), dim3(nb), dim3(256), sh,
                        s, a);
   } else {

@@ -138,6 +138,16 @@ int dmx_sort_points(const dmx_array *coord, const int64_t *nover,
                     const double *periodicity, dmx_array *keys,
                     dmx_array *idx);
 
+/* Phase timing (role of ducc0's TimerHierarchy / verbosity output,
+   src/ducc0/infra/timers.h:69-262, nufft.h:469-471): when enabled, every
+   transform records HIP events around its phases ("building index",
+   "zeroing grid", "spreading", "FFT axis k", "grid correction",
+   "interpolation") on its stream; totals accumulate per process.
+   dmx_profile_read writes lines "name\ttotal_ms\tcount\n". */
+int dmx_profile_enable(int on);
+int dmx_profile_reset(void);
+int dmx_profile_read(char *buf, size_t buflen);
+
 /* Number of HIP devices visible (0 when none). */
 int dmx_device_count(void);
 

@@ -246,11 +246,12 @@ def test_sort_vs_ducc0_golden():
 def test_nufft_vs_ducc0_golden():
     z = np.load(_gold("nufft.npz"))
     cases = json.loads(str(z["cases"]))
+    gp = json.load(open(_gold("params.json")))
     for i, cs in enumerate(cases):
         c = z[f"coord_{i}"]
         p = z[f"points_{i}"]
         g = z[f"grid_{i}"]
-        vlen = 8 if cs["single"] else 4
+        vlen = gp["vlen_single"] if cs["single"] else gp["vlen_double"]
         kw = dict(single=cs["single"], vlen=vlen, nthreads=1,
                   period=cs["periodicity"], fft_order=cs["fft_order"])
         a = O.nu2u(c, p, tuple(cs["shape"]), cs["forward"], cs["eps"], **kw)
