@@ -141,6 +141,36 @@ def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
     return out
 
 
+def u2nu_coord_grad(*, grid, coord, forward, epsilon, out=None,
+                    sigma_min=1.2, sigma_max=2.51, periodicity=_TWO_PI,
+                    fft_order=False, precision="auto", chooser="analytic",
+                    log2tile=0):
+    """d u2nu(grid)_j / d coord[j, d] for every point and axis, as an
+    (M, ndim) complex array: the interpolation of the same oversampled grid
+    with the derivative of the spreading kernel (no ducc0 counterpart)."""
+    _lib.require_gpu()
+    _check_dtypes(grid, coord)
+    M, nd = coord.shape
+    per = _periodicity(periodicity, nd)
+    if out is None:
+        if _is_torch(grid):
+            import torch
+            out = torch.empty((M, nd), dtype=grid.dtype, device=grid.device)
+        else:
+            out = np.empty((M, nd), dtype=np.asarray(grid).dtype)
+    g, kg = _lib.describe(grid)
+    c, kc = _lib.describe(coord)
+    o, ko = _lib.describe(out)
+    opt = _opts(precision, chooser, log2tile, _stream_for(grid, coord, out))
+    _lib.check(_lib.lib().dmx_u2nu_coord_grad(
+        ct.byref(g), ct.byref(c), int(bool(forward)), float(epsilon),
+        ct.byref(o), float(sigma_min), float(sigma_max), per.ctypes.data,
+        int(bool(fft_order)), ct.byref(opt)))
+    if not _is_torch(out) and ko is not out:
+        out[...] = ko
+    return out
+
+
 def bestEpsilon(*, ndim, singleprec, sigma_min=1.1, sigma_max=2.6):
     """Smallest achievable epsilon (gridding_kernel.cc:1562-1574)."""
     r = _lib.lib().dmx_best_epsilon(int(ndim), int(bool(singleprec)),
