@@ -120,6 +120,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--chooser", default="auto",
                     choices=["auto", "learned", "analytic"])
+    ap.add_argument("--calib", action="store_true",
+                    help="run one 256 MiB device copy first (calibrates HBM "
+                         "byte counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "
@@ -132,6 +135,12 @@ def main():
     from ducc_amd import chooser as learned
 
     dev = torch.device("cuda:0")
+    if args.calib:
+        # a plain copy of known size: 256 MiB read + 256 MiB written
+        x = torch.empty(1 << 26, dtype=torch.float32, device=dev).fill_(1.0)
+        y = x.clone()
+        torch.cuda.synchronize()
+        del x, y
     rng = np.random.default_rng(102)
     M, n, eps = args.M, args.n, args.eps
     c_np = clustered_points(rng, M)

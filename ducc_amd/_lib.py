@@ -28,7 +28,8 @@ class DmxOptions(ct.Structure):
     _fields_ = [("precision", ct.c_int32), ("chooser", ct.c_int32),
                 ("log2tile", ct.c_int32), ("kernel_idx", ct.c_int32),
                 ("fp64_below", ct.c_double), ("stream", ct.c_void_p),
-                ("sigma", ct.c_double), ("reserved", ct.c_int32 * 8)]
+                ("sigma", ct.c_double), ("spot_check", ct.c_int32),
+                ("reserved", ct.c_int32 * 7)]
 
 
 class DmxPlanInfo(ct.Structure):
@@ -80,6 +81,18 @@ _SIG = {
                              _P, ct.c_int, _P, ct.c_size_t, _AP]),
     "dmx_sort_points": (ct.c_int, [_AP, _P, ct.c_size_t, ct.c_int, ct.c_int,
                                    _P, _AP, _AP]),
+    "dmx_mlp_nparam": (ct.c_size_t, []),
+    "dmx_mlp_forward": (ct.c_int, [_P, _P, ct.c_int, _P]),
+    "dmx_mlp_grad": (ct.c_int, [_P, _P, _P, _P, ct.c_int, _P, _P]),
+    "dmx_mlp_train": (ct.c_int, [_P, _P, _P, _P, ct.c_int, ct.c_int,
+                                 ct.c_float, ct.c_float, _P]),
+    "dmx_chooser_set": (ct.c_int, [_P, _P]),
+    "dmx_chooser_features": (ct.c_int, [ct.c_size_t, _P, ct.c_size_t,
+                                        ct.c_double, ct.c_int, ct.c_int, _P]),
+    "dmx_choose": (ct.c_int, [ct.c_size_t, _P, ct.c_size_t, ct.c_double,
+                              ct.c_int, ct.c_int, ct.c_double, ct.c_double,
+                              ct.POINTER(DmxOptions),
+                              ct.POINTER(DmxPlanInfo)]),
     "dmx_profile_enable": (ct.c_int, [ct.c_int]),
     "dmx_profile_reset": (ct.c_int, []),
     "dmx_profile_read": (ct.c_int, [ct.c_char_p, ct.c_size_t]),
@@ -181,9 +194,13 @@ def profile_read():
 
 
 def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
-            fp64_below=1e-6, stream=None, sigma=0.0):
+            fp64_below=1e-6, stream=None, sigma=0.0, spot_check=True):
     o = DmxOptions()
     lib().dmx_default_options(ct.byref(o))
+    o.spot_check = 2 if spot_check == "fail" else int(bool(spot_check))
+    if chooser == "learned":
+        from . import chooser as _ch
+        _ch.ensure_loaded()
     o.precision = {"auto": 0, "fp32": 1, "fp64": 2}[precision]
     o.chooser = {"analytic": 0, "learned": 1}[chooser]
     o.log2tile = int(log2tile)

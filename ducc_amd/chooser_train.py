@@ -1,0 +1,286 @@
+"""Train and evaluate the learned parameter chooser on the GPU.
+
+1. Labels: random requests from the mixed distribution (ndim 1-3, grid size
+   per axis log-uniform 64..4096 - 1-D up to 1e6, 3-D capped at 160 per axis
+   to fit - M log-uniform 1e3..1e7, eps log-uniform 1e-10..1e-3, uniform or
+   8-cluster Gaussian points).  For every request and transform type the
+   plain method is timed over candidate (kernel row, tile) settings: for
+   each support W the smallest tabulated oversampling factor in [1.2, 2.0]
+   reaching eps and two larger ones, times three tile sizes.  The label is
+   the fastest setting; every candidate row reaches eps by the table, and the
+   accuracy of the learned choice is re-checked at run time.
+2. Training: 6 features -> (sigma, W, log2 tile), normalised, full-batch
+   Adam with the library's HIP kernels (dmx_mlp_train).
+3. Evaluation: fresh requests timed with the analytic choice (ducc0's cost
+   model) and with the learned choice (spot-check on), plus the error of both
+   against the FP64 direct sum on sampled modes.
+
+python -m ducc_amd.chooser_train --problems 300 --eval 1000
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from ducc_amd import _lib, chooser, nufft  # noqa: E402
+
+
+def kernel_db():
+    a = np.loadtxt(os.path.join(HERE, "data", "kernel_db.csv"),
+                   delimiter=",", comments="#")
+    return dict(W=a[:, 0].astype(int), ofactor=a[:, 1], epsilon=a[:, 2],
+                ndim=a[:, 5].astype(int), single=a[:, 6].astype(int))
+
+
+def sample_request(rng):
+    nd = int(rng.integers(1, 4))
+    if nd == 1:
+        shape = (int(np.exp(rng.uniform(np.log(64), np.log(1e6)))),)
+    elif nd == 2:
+        shape = tuple(int(np.exp(rng.uniform(np.log(64), np.log(4096))))
+                      for _ in range(2))
+    else:
+        shape = tuple(int(np.exp(rng.uniform(np.log(64), np.log(160))))
+                      for _ in range(3))
+    M = int(10 ** rng.uniform(3, 7))
+    eps = float(10 ** rng.uniform(-10, -3))
+    dist = "cluster" if rng.random() < 0.5 else "uniform"
+    return dict(shape=shape, M=M, eps=eps, dist=dist)
+
+
+def make_inputs(req, seed):
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    nd, M = len(req["shape"]), req["M"]
+    if req["dist"] == "uniform":
+        c = (torch.rand((M, nd), generator=g, device="cuda",
+                        dtype=torch.float64) - 0.5) * (2 * math.pi)
+    else:
+        cen = (torch.rand((8, nd), generator=g, device="cuda",
+                          dtype=torch.float64) - 0.5) * (2 * math.pi)
+        lab = torch.randint(0, 8, (M,), generator=g, device="cuda")
+        c = cen[lab] + 0.3 * torch.randn((M, nd), generator=g, device="cuda",
+                                         dtype=torch.float64)
+        c = torch.remainder(c + math.pi, 2 * math.pi) - math.pi
+    f64 = req["eps"] < 1e-6
+    dt = torch.complex128 if f64 else torch.complex64
+    p = torch.randn(M, generator=g, device="cuda", dtype=dt)
+    grid = torch.randn(req["shape"], generator=g, device="cuda", dtype=dt)
+    return c.contiguous(), p, grid, f64
+
+
+def timed(fn, reps=2):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def candidates(req, f64, db):
+    nd = len(req["shape"])
+    sel = (db["ndim"] == nd) & (db["single"] == (0 if f64 else 1)) & \
+        (db["epsilon"] <= req["eps"]) & (db["ofactor"] >= 1.2) & \
+        (db["ofactor"] <= 2.0)
+    rows = np.nonzero(sel)[0]
+    out = []
+    for W in sorted(set(db["W"][rows])):
+        r = rows[db["W"][rows] == W]
+        r = r[np.argsort(db["ofactor"][r])]
+        smin = db["ofactor"][r[0]]
+        picks = [r[0]]
+        for dlt in (0.1, 0.25):
+            k = r[np.argmin(np.abs(db["ofactor"][r] - (smin + dlt)))]
+            if k not in picks:
+                picks.append(k)
+        out += [int(k) for k in picks]
+    if nd == 1:
+        tiles = (8, 9, 10)
+    elif nd == 2:
+        tiles = (4, 5, 6) if not f64 else (3, 4, 5)
+    else:
+        tiles = (2, 3)
+    return [(k, t) for k in out for t in tiles]
+
+
+def label_requests(n, seed, db, log):
+    X, T, meta = [], [], []
+    rng = np.random.default_rng(seed)
+    for q in range(n):
+        req = sample_request(rng)
+        c, p, g, f64 = make_inputs(req, seed * 100000 + q)
+        import torch
+        out_g = torch.empty(req["shape"], dtype=p.dtype, device="cuda")
+        out_p = torch.empty(req["M"], dtype=p.dtype, device="cuda")
+        cands = candidates(req, f64, db)
+        for typ in ("nu2u", "u2nu"):
+            best = None
+            for k, t in cands:
+                if typ == "nu2u":
+                    fn = lambda: nufft.nu2u(points=p, coord=c, forward=True,
+                                            epsilon=req["eps"], out=out_g,
+                                            kernel_idx=k, log2tile=t)
+                else:
+                    fn = lambda: nufft.u2nu(grid=g, coord=c, forward=True,
+                                            epsilon=req["eps"], out=out_p,
+                                            kernel_idx=k, log2tile=t)
+                try:
+                    tt = timed(fn)
+                except RuntimeError:
+                    continue
+                if best is None or tt < best[0]:
+                    best = (tt, k, t)
+            if best is None:
+                continue
+            tt, k, t = best
+            X.append(chooser.features(req["shape"], req["M"], req["eps"], f64,
+                                      typ == "nu2u"))
+            T.append([db["ofactor"][k], db["W"][k], t])
+            meta.append(dict(req=req, type=typ, time=tt, kidx=k, log2tile=t))
+        del c, p, g, out_g, out_p
+        if q % 10 == 0:
+            log(f"label {q}/{n}: {req['shape']} M={req['M']} "
+                f"eps={req['eps']:.1e} -> {T[-1] if T else None}")
+    return np.array(X), np.array(T, dtype=np.float64), meta
+
+
+def fit(X, T, log, seed=0):
+    fm, fs = X.mean(0), np.maximum(X.std(0), 1e-3)
+    tm, ts = T.mean(0), np.maximum(T.std(0), 1e-3)
+    Xn = ((X - fm) / fs).astype(np.float32)
+    Tn = ((T - tm) / ts).astype(np.float32)
+    P = chooser.init_params(seed)
+    P, l1 = chooser.train(P, Xn, Tn, steps=3000, lr=3e-3, weight_decay=1e-5)
+    P, l2 = chooser.train(P, Xn, Tn, steps=3000, lr=5e-4, weight_decay=1e-5)
+    log(f"train loss {l1[0]:.4f} -> {l1[-1]:.4f} -> {l2[-1]:.4f}")
+    norm = np.concatenate([fm, fs, tm, ts])
+    return P, norm, [float(l1[0]), float(l1[-1]), float(l2[-1])]
+
+
+def evaluate(n, seed, log):
+    import torch
+    rng = np.random.default_rng(seed)
+    rows = []
+    for q in range(n):
+        req = sample_request(rng)
+        c, p, g, f64 = make_inputs(req, seed * 100000 + q)
+        out_g = torch.empty(req["shape"], dtype=p.dtype, device="cuda")
+        out_p = torch.empty(req["M"], dtype=p.dtype, device="cuda")
+        rec = dict(req=req)
+        for ch in ("analytic", "learned"):
+            info1, info2 = {}, {}
+
+            def fn():
+                nufft.nu2u(points=p, coord=c, forward=True, epsilon=req["eps"],
+                           out=out_g, chooser=ch, info=info1)
+                nufft.u2nu(grid=g, coord=c, forward=True, epsilon=req["eps"],
+                           out=out_p, chooser=ch, info=info2)
+            rec[ch] = dict(time=timed(fn), nu2u=dict(info1), u2nu=dict(info2))
+            if q % 10 == 0:
+                # error of the type-1 result on 32 modes against the FP64
+                # direct sum
+                N = int(np.prod(req["shape"]))
+                sel = np.random.default_rng(q).choice(N, min(N, 32),
+                                                      replace=False)
+                S = min(req["M"], 200000)
+                cs = c[:S].cpu().numpy()
+                ps = p[:S].cpu().numpy().astype(np.complex128)
+                o = torch.empty(req["shape"], dtype=p.dtype, device="cuda")
+                nufft.nu2u(points=p[:S].contiguous(), coord=c[:S].contiguous(),
+                           forward=True, epsilon=req["eps"], out=o,
+                           chooser=ch)
+                ref = nufft.nudft_direct(1, cs, ps, req["shape"], True, sel)
+                got = o.cpu().numpy().ravel()[sel].astype(np.complex128)
+                rec[ch]["err"] = float(np.linalg.norm(got - ref) /
+                                       np.linalg.norm(ref))
+        rows.append(rec)
+        del c, p, g, out_g, out_p
+        if q % 50 == 0:
+            sp = [r["analytic"]["time"] / r["learned"]["time"] for r in rows]
+            log(f"eval {q}/{n}: geo-mean speed-up so far "
+                f"{math.exp(np.mean(np.log(sp))):.3f}")
+    return rows
+
+
+def summarize(rows):
+    sp = np.array([r["analytic"]["time"] / r["learned"]["time"]
+                   for r in rows])
+    used = np.array([r["learned"]["nu2u"]["chooser_used"] and
+                     r["learned"]["u2nu"]["chooser_used"] for r in rows])
+    errs = [(r["learned"]["err"] / r["req"]["eps"],
+             r["analytic"]["err"] / r["req"]["eps"])
+            for r in rows if "err" in r["learned"]]
+    by_dim = {}
+    for nd in (1, 2, 3):
+        s = sp[[len(r["req"]["shape"]) == nd for r in rows]]
+        if len(s):
+            by_dim[nd] = dict(n=int(len(s)),
+                              geo_mean=float(np.exp(np.mean(np.log(s)))))
+    return dict(
+        requests=len(rows),
+        speedup_geo_mean=float(np.exp(np.mean(np.log(sp)))),
+        speedup_median=float(np.median(sp)),
+        frac_faster=float(np.mean(sp > 1.0)),
+        frac_slower_10pct=float(np.mean(sp < 0.9)),
+        learned_kept_after_spot_check=float(np.mean(used)),
+        err_over_eps_learned_max=float(max(e[0] for e in errs)) if errs else None,
+        err_over_eps_analytic_max=float(max(e[1] for e in errs)) if errs else None,
+        total_time_analytic_s=float(sum(r["analytic"]["time"] for r in rows)),
+        total_time_learned_s=float(sum(r["learned"]["time"] for r in rows)),
+        by_ndim=by_dim)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--problems", type=int, default=300)
+    ap.add_argument("--eval", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=105)
+    ap.add_argument("--out", default=chooser.WEIGHTS)
+    ap.add_argument("--report", default=os.path.join(ROOT, "gpurun_out",
+                                                     "chooser_eval.json"))
+    ap.add_argument("--skip-train", action="store_true")
+    a = ap.parse_args()
+
+    def log(msg):
+        print(msg, flush=True)
+
+    db = kernel_db()
+    if not a.skip_train:
+        t0 = time.time()
+        X, T, meta = label_requests(a.problems, a.seed, db, log)
+        log(f"labelled {len(X)} samples in {time.time() - t0:.0f} s")
+        P, norm, losses = fit(X, T, log)
+        os.makedirs(os.path.dirname(a.out), exist_ok=True)
+        json.dump(dict(params=[float(v) for v in P],
+                       norm=[float(v) for v in norm],
+                       features=["ndim", "mean log2 grid size", "log10 M",
+                                 "-log10 eps", "fp64", "type 1"],
+                       targets=["oversampling factor", "W", "log2 tile"],
+                       samples=int(len(X)), losses=losses,
+                       seed=a.seed),
+                  open(a.out, "w"))
+        np.savez_compressed(os.path.splitext(a.report)[0] + "_train.npz",
+                            X=X, T=T)
+        log(f"weights written to {a.out}")
+    chooser.load(a.out)
+    rows = evaluate(a.eval, a.seed + 1, log)
+    summ = summarize(rows)
+    log(json.dumps(summ))
+    os.makedirs(os.path.dirname(a.report), exist_ok=True)
+    json.dump(dict(summary=summ, rows=rows), open(a.report, "w"))
+
+
+if __name__ == "__main__":
+    main()

@@ -11,8 +11,11 @@
 #include <vector>
 
 #include "ducc_amd.h"
+#include <cmath>
+
 #include "chooser.h"
 #include "grid.h"
+#include "mlp.h"
 #include "params.h"
 #include "runtime.h"
 
@@ -217,6 +220,95 @@ hipStream_t pick_stream(const dmx_options &o) {
   return o.stream ? (hipStream_t)o.stream : library_stream();
 }
 
+// Accuracy spot-check of a parameter set: a type-1 transform of random
+// complex128 values at up to 4096 of the request's points (evenly strided
+// sample), compared on 32 random modes with the FP64 direct sum on the GPU.
+// Passes when the relative l2 error is within the reference tests' bound
+// (4/2/1 eps in 1/2/3-D, test_nufft.py:79,140,203).
+bool spot_check(const Params &p, const void *coords, bool coord_f32, size_t M,
+                hipStream_t s) {
+  const int nd = p.ndim;
+  const size_t S = std::min<size_t>(M, 4096);
+  if (S == 0) return true;
+  std::vector<uint32_t> sel(S);
+  for (size_t i = 0; i < S; ++i) sel[i] = (uint32_t)((i * M) / S);
+  Buf dsel(S * 4), dsub(S * nd * 8);
+  DMX_HIP(hipMemcpyAsync(dsel.p, sel.data(), S * 4, hipMemcpyHostToDevice, s));
+  launch_gather_coords(coords, coord_f32, nd, dsel.as<uint32_t>(), S, dsub.p,
+                       s);
+  std::vector<double> c64(S * nd);
+  if (coord_f32) {
+    std::vector<float> c32(S * nd);
+    DMX_HIP(hipMemcpyAsync(c32.data(), dsub.p, S * nd * 4,
+                           hipMemcpyDeviceToHost, s));
+    DMX_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < S * nd; ++i) c64[i] = c32[i];
+  } else {
+    DMX_HIP(hipMemcpyAsync(c64.data(), dsub.p, S * nd * 8,
+                           hipMemcpyDeviceToHost, s));
+    DMX_HIP(hipStreamSynchronize(s));
+  }
+  // values: a fixed pseudo-random sequence
+  std::vector<double> v(2 * S);
+  uint64_t st = 0x9E3779B97F4A7C15ull;
+  for (auto &x : v) {
+    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+    x = (double)(st >> 11) / 9007199254740992.0 - 0.5;
+  }
+  size_t N = 1;
+  for (int d = 0; d < nd; ++d) N *= p.nuni[d];
+  const size_t nm = std::min<size_t>(N, 32);
+  std::vector<int64_t> modes(nm);
+  for (size_t i = 0; i < nm; ++i) {
+    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+    modes[i] = nm == N ? (int64_t)i : (int64_t)(st % N);
+  }
+  Buf dvals(2 * S * 8), dc64(S * nd * 8), dout(N * 16), dmodes(nm * 8),
+      dref(nm * 16), dgot(nm * 16);
+  DMX_HIP(hipMemcpyAsync(dvals.p, v.data(), 2 * S * 8, hipMemcpyHostToDevice, s));
+  DMX_HIP(hipMemcpyAsync(dc64.p, c64.data(), S * nd * 8, hipMemcpyHostToDevice, s));
+  DMX_HIP(hipMemcpyAsync(dmodes.p, modes.data(), nm * 8, hipMemcpyHostToDevice, s));
+  {
+    Plan sp(p, s);
+    sp.build_index(dsub.p, coord_f32, S, false);
+    int64_t ust[3] = {0, 0, 0};
+    int64_t acc = 1;
+    for (int d = nd - 1; d >= 0; --d) {
+      ust[d] = acc;
+      acc *= (int64_t)p.nuni[d];
+    }
+    sp.nu2u(true, dvals.p, true, dout.p, true, ust);
+  }
+  launch_gather_c128(dout.p, dmodes.as<int64_t>(), nm, dgot.p, s);
+  NudftArgs a{};
+  a.type = 1;
+  a.ndim = nd;
+  a.forward = 1;
+  a.fft_order = p.fft_order ? 1 : 0;
+  a.M = (int64_t)S;
+  a.nsel = (int64_t)nm;
+  for (int d = 0; d < 3; ++d) {
+    a.n[d] = d < nd ? (int64_t)p.nuni[d] : 1;
+    a.period[d] = d < nd ? p.period[d] : 1.;
+  }
+  a.coord = dc64.as<double>();
+  a.vals = dvals.as<double>();
+  a.sel = dmodes.as<int64_t>();
+  a.out = dref.as<double>();
+  launch_nudft(a, s);
+  std::vector<double> got(2 * nm), ref(2 * nm);
+  DMX_HIP(hipMemcpyAsync(got.data(), dgot.p, nm * 16, hipMemcpyDeviceToHost, s));
+  DMX_HIP(hipMemcpyAsync(ref.data(), dref.p, nm * 16, hipMemcpyDeviceToHost, s));
+  DMX_HIP(hipStreamSynchronize(s));
+  double num = 0, den = 0;
+  for (size_t i = 0; i < 2 * nm; ++i) {
+    num += (got[i] - ref[i]) * (got[i] - ref[i]);
+    den += ref[i] * ref[i];
+  }
+  const double mul = nd == 1 ? 4. : (nd == 2 ? 2. : 1.);
+  return den == 0 || std::sqrt(num / den) <= mul * p.eps;
+}
+
 int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
                   const dmx_array *pts, int forward, double epsilon,
                   double smin, double smax, const double *per, int fft_order,
@@ -233,9 +325,22 @@ int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
     Params p = choose_params(ndim, nuni, M, type1, epsilon, smin, smax, per,
                              fft_order != 0, vf64, opt);
     hipStream_t s = pick_stream(opt);
-    Plan plan(p, s);
     DevIn cin(*coord, s, false);
-    plan.build_index(cin.p, coord->typecode == DMX_F32, M, false);
+    const bool cf32 = coord->typecode == DMX_F32;
+    if (p.chooser_used && !p.chooser_verified && opt.spot_check) {
+      // spot_check == 2 forces the failure branch (tests of the fallback)
+      const bool ok = opt.spot_check == 2 ? false
+                                          : spot_check(p, cin.p, cf32, M, s);
+      chooser_verdict(ndim, nuni, M, epsilon, smin, smax, p.fp64, type1, ok);
+      if (!ok) {
+        dmx_options o2 = opt;
+        o2.chooser = 0;
+        p = choose_params(ndim, nuni, M, type1, epsilon, smin, smax, per,
+                          fft_order != 0, vf64, o2);
+      }
+    }
+    Plan plan(p, s);
+    plan.build_index(cin.p, cf32, M, false);
     if (type1) {
       DevIn pin(*pts, s, false);
       DevOut uo(*uni, true);
@@ -285,6 +390,139 @@ void dmx_default_options(dmx_options *o) {
   o->fp64_below = 1e-6;
   o->stream = nullptr;
   o->sigma = 0;
+  o->spot_check = 1;
+}
+
+size_t dmx_mlp_nparam(void) { return MLP_NPARAM; }
+
+int dmx_mlp_forward(const float *params, const float *X, int B, float *Y) {
+  return guard([&] {
+    DMX_ASSERT(params && X && Y && B > 0, "bad arguments");
+    hipStream_t s = library_stream();
+    Buf dp(MLP_NPARAM * 4), dx((size_t)B * MLP_IN * 4),
+        dy((size_t)B * MLP_OUT * 4);
+    DMX_HIP(hipMemcpyAsync(dp.p, params, MLP_NPARAM * 4, hipMemcpyHostToDevice, s));
+    DMX_HIP(hipMemcpyAsync(dx.p, X, (size_t)B * MLP_IN * 4, hipMemcpyHostToDevice, s));
+    mlp_forward(dp.as<float>(), dx.as<float>(), B, nullptr, dy.as<float>(), s);
+    DMX_HIP(hipMemcpyAsync(Y, dy.p, (size_t)B * MLP_OUT * 4, hipMemcpyDeviceToHost, s));
+    DMX_HIP(hipStreamSynchronize(s));
+  });
+}
+
+namespace {
+struct MlpState {
+  Buf P, X, T, w, act, dz, lossb, G, loss, m, v;
+  MlpState(const float *params, const float *X_, const float *T_,
+           const float *w_, int B, hipStream_t s)
+      : P(MLP_NPARAM * 4), X((size_t)B * MLP_IN * 4),
+        T((size_t)B * MLP_OUT * 4), w(MLP_OUT * 4),
+        act(mlp_act_floats(B) * 4), dz((mlp_act_floats(B) + B * MLP_OUT) * 4),
+        lossb((size_t)B * MLP_OUT * 4), G(MLP_NPARAM * 4), loss(4),
+        m(MLP_NPARAM * 4), v(MLP_NPARAM * 4) {
+    DMX_HIP(hipMemcpyAsync(P.p, params, MLP_NPARAM * 4, hipMemcpyHostToDevice, s));
+    DMX_HIP(hipMemcpyAsync(X.p, X_, (size_t)B * MLP_IN * 4, hipMemcpyHostToDevice, s));
+    DMX_HIP(hipMemcpyAsync(T.p, T_, (size_t)B * MLP_OUT * 4, hipMemcpyHostToDevice, s));
+    static const float ones[MLP_OUT] = {1.f, 1.f, 1.f};
+    DMX_HIP(hipMemcpyAsync(w.p, w_ ? w_ : ones, MLP_OUT * 4, hipMemcpyHostToDevice, s));
+    launch_zero(m.p, MLP_NPARAM * 4, s);
+    launch_zero(v.p, MLP_NPARAM * 4, s);
+    DMX_HIP(hipStreamSynchronize(s));
+  }
+  void grad(int B, hipStream_t s) {
+    mlp_grad(P.as<float>(), X.as<float>(), T.as<float>(), w.as<float>(), B,
+             act.as<float>(), dz.as<float>(), lossb.as<float>(), G.as<float>(),
+             loss.as<float>(), s);
+  }
+};
+}  // namespace
+
+int dmx_mlp_grad(const float *params, const float *X, const float *T,
+                 const float *w, int B, float *grad, float *loss) {
+  return guard([&] {
+    DMX_ASSERT(params && X && T && grad && B > 0, "bad arguments");
+    hipStream_t s = library_stream();
+    MlpState st(params, X, T, w, B, s);
+    st.grad(B, s);
+    DMX_HIP(hipMemcpyAsync(grad, st.G.p, MLP_NPARAM * 4, hipMemcpyDeviceToHost, s));
+    if (loss)
+      DMX_HIP(hipMemcpyAsync(loss, st.loss.p, 4, hipMemcpyDeviceToHost, s));
+    DMX_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dmx_mlp_train(float *params, const float *X, const float *T,
+                  const float *w, int B, int steps, float lr,
+                  float weight_decay, float *losses) {
+  return guard([&] {
+    DMX_ASSERT(params && X && T && B > 0 && steps >= 0, "bad arguments");
+    hipStream_t s = library_stream();
+    MlpState st(params, X, T, w, B, s);
+    Buf lh((size_t)std::max(steps, 1) * 4);
+    for (int t = 1; t <= steps; ++t) {
+      st.grad(B, s);
+      if (losses)
+        DMX_HIP(hipMemcpyAsync(lh.as<float>() + (t - 1), st.loss.p, 4,
+                               hipMemcpyDeviceToDevice, s));
+      mlp_adam_step(st.P.as<float>(), st.G.as<float>(), st.m.as<float>(),
+                    st.v.as<float>(), lr, t, weight_decay, s);
+    }
+    DMX_HIP(hipMemcpyAsync(params, st.P.p, MLP_NPARAM * 4, hipMemcpyDeviceToHost, s));
+    if (losses && steps)
+      DMX_HIP(hipMemcpyAsync(losses, lh.p, (size_t)steps * 4, hipMemcpyDeviceToHost, s));
+    DMX_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dmx_chooser_set(const float *params, const double *norm) {
+  return guard([&] {
+    DMX_ASSERT(params && norm, "bad arguments");
+    for (int i = 6; i < 12; ++i) DMX_ASSERT(norm[i] > 0, "bad normalisation");
+    for (int i = 15; i < 18; ++i) DMX_ASSERT(norm[i] > 0, "bad normalisation");
+    chooser_set(params, norm);
+  });
+}
+
+int dmx_chooser_features(size_t ndim, const int64_t *shape, size_t npoints,
+                         double epsilon, int fp64, int nu2u, double *f) {
+  return guard([&] {
+    DMX_ASSERT(ndim >= 1 && ndim <= 3 && shape && f && epsilon > 0,
+               "bad arguments");
+    size_t n[3] = {1, 1, 1};
+    for (size_t d = 0; d < ndim; ++d) n[d] = (size_t)shape[d];
+    chooser_features((int)ndim, n, npoints, epsilon, fp64 != 0, nu2u != 0, f);
+  });
+}
+
+int dmx_choose(size_t ndim, const int64_t *shape, size_t npoints,
+               double epsilon, int values_f64, int nu2u, double sigma_min,
+               double sigma_max, const dmx_options *opt_in,
+               dmx_plan_info *info) {
+  return guard([&] {
+    DMX_ASSERT(shape && info, "null argument");
+    DMX_ASSERT(ndim >= 1 && ndim <= 3, "transform must be 1D/2D/3D");
+    dmx_options opt = opt_in ? *opt_in : defaults();
+    size_t n[3] = {1, 1, 1};
+    double per[3] = {2 * M_PI, 2 * M_PI, 2 * M_PI};
+    for (size_t d = 0; d < ndim; ++d) n[d] = (size_t)shape[d];
+    Params p = choose_params((int)ndim, n, npoints, nu2u != 0, epsilon,
+                             sigma_min, sigma_max, per, false, values_f64 != 0,
+                             opt);
+    std::memset(info, 0, sizeof(*info));
+    info->ndim = p.ndim;
+    info->W = p.W;
+    info->kernel_idx = p.kidx;
+    info->fp64 = p.fp64;
+    info->log2tile = p.log2tile;
+    info->chooser_used = p.chooser_used;
+    for (int d = 0; d < 3; ++d) {
+      info->nuni[d] = d < p.ndim ? (int64_t)p.nuni[d] : 1;
+      info->nover[d] = d < p.ndim ? (int64_t)p.nover[d] : 1;
+    }
+    info->ofactor = p.ofactor;
+    info->eps_tab = p.eps_tab;
+    info->epsilon = p.eps;
+    info->npoints = (int64_t)npoints;
+  });
 }
 
 double dmx_best_epsilon(size_t ndim, int singleprec, double sigma_min,

@@ -26,7 +26,37 @@ struct FftLaunch {
   int64_t st_ax, st_u, st_v;
   int nu, nv;
   int un1, ua1, ua2, vn1, va1, va2;
+  // contiguous lines only: elements kn1 <= k < ka2 are read as zero (the
+  // input is known to vanish there)
+  int kprune, kn1, ka2;
 };
+
+// One half of the two-pass FFT along a strided axis (see k_fft_cols).
+struct FftCols {
+  const void *src;
+  void *dst;
+  const void *tw_sub;   // twiddles of length Lsub
+  const void *tw_full;  // twiddles of length L (inter-pass factors)
+  int L, Lsub, npass, sign, twiddle, Bl;
+  int radix[24];
+  int64_t ia, ib, oa, ob;  // row maps of input / output (in rows)
+  int64_t rs;              // row stride (elements)
+  int ncol, vn1, va1, va2; // columns (virtual -> actual, stride 1)
+  int nlines;
+  int nouter, on1, oa1, oa2;
+  int64_t st_o;
+};
+
+// FFT of length L = L1*L2 along a strided axis, as two coalesced passes:
+// A (in place): length-L1 FFTs over rows n1*L2+n2, times w_L^{n2 k1};
+// B (src -> dst): length-L2 FFTs over rows k1*L2+n2, written to row
+// k1 + L1*k2, i.e. in natural order.  `twiddle(len)` returns device
+// twiddles.  Returns false when L has no split worth using.
+bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
+                       const void *tw_L1, const void *tw_L2,
+                       const void *tw_L, hipStream_t s);
+// a divisor of L near sqrt(L) with both factors <= 128 (0: none)
+int fft_split(int L);
 
 void fft_twiddles(int L, bool fp64, std::vector<unsigned char> &out);
 // Runs the FFT over every line of the set (in place).  `scratch` must be as

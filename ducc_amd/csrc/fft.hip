@@ -201,6 +201,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lds(FftLaunch p) {
     int b, k;
     if (p.st_ax == 1) { b = e / L; k = e - b * L; }
     else { k = e / nlines; b = e - k * nlines; }
+    if (p.kprune && k >= p.kn1 && k < p.ka2) {
+      C z;
+      z.x = T(0);
+      z.y = T(0);
+      buf[b * LP + k] = z;
+      continue;
+    }
     const int64_t addr =
         ubase + map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v + k * p.st_ax;
     buf[b * LP + k] = data[addr];
@@ -228,6 +235,61 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lds(FftLaunch p) {
     const int64_t addr =
         ubase + map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v + k * p.st_ax;
     data[addr] = buf[b * LP + k];
+  }
+}
+
+// Sub-FFTs over rows of Bl adjacent columns (one half of the two-pass
+// column FFT, see fft_columns_2pass): line u, element e sits at row
+// u*ia + e*ib of the source; output element k goes to row u*oa + k*ob of
+// the destination, optionally times w_L^{(k*u) mod L}.  Loads and stores
+// move Bl contiguous columns per row (coalesced).
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(FftCols p) {
+  using C = typename V2<T>::t;
+  extern __shared__ unsigned char smem_raw[];
+  C *buf = (C *)smem_raw;
+  const C *src = (const C *)p.src;
+  C *dst = (C *)p.dst;
+  const int Ls = p.Lsub, LP = p.Lsub + 1;
+  const int u = blockIdx.y;
+  const int v0 = blockIdx.x * p.Bl;
+  const int nb = min(p.Bl, p.ncol - v0);
+  const int64_t obase =
+      map_range(blockIdx.z, p.on1, p.oa1, p.oa2) * p.st_o;
+  const int nel = nb * Ls;
+  for (int i = threadIdx.x; i < nel; i += blockDim.x) {
+    const int e = i / nb, b = i - e * nb;
+    const int64_t row = (int64_t)u * p.ia + (int64_t)e * p.ib;
+    buf[b * LP + e] =
+        src[obase + row * p.rs + map_range(v0 + b, p.vn1, p.va1, p.va2)];
+  }
+  __syncthreads();
+  int Ns = 1;
+  const C *tw = (const C *)p.tw_sub;
+  for (int s = 0; s < p.npass; ++s) {
+    const int R = p.radix[s];
+    switch (R) {
+      case 4: lds_pass<T, 4, FFT_KM(T, 4)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 2: lds_pass<T, 2, FFT_KM(T, 2)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 3: lds_pass<T, 3, FFT_KM(T, 3)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 5: lds_pass<T, 5, FFT_KM(T, 5)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 7: lds_pass<T, 7, FFT_KM(T, 7)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 11: lds_pass<T, 11, FFT_KM(T, 11)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      default: break;
+    }
+    Ns *= R;
+  }
+  const C *twf = (const C *)p.tw_full;
+  for (int i = threadIdx.x; i < nel; i += blockDim.x) {
+    const int k = i / nb, b = i - k * nb;
+    C x = buf[b * LP + k];
+    if (p.twiddle && u > 0 && k > 0) {
+      C w = twf[(int)(((int64_t)k * u) % p.L)];
+      if (p.sign > 0) w.y = -w.y;
+      x = cmul(x, w);
+    }
+    const int64_t row = (int64_t)u * p.oa + (int64_t)k * p.ob;
+    dst[obase + row * p.rs + map_range(v0 + b, p.vn1, p.va1, p.va2)] = x;
   }
 }
 
@@ -312,6 +374,68 @@ void fft_twiddles(int L, bool fp64, std::vector<unsigned char> &out) {
       p[0] = (float)c; p[1] = (float)s;
     }
   }
+}
+
+int fft_split(int L) {
+  int best = 0;
+  for (int d = 2; d * d <= L; ++d)
+    if (L % d == 0 && L / d <= 128) best = d;
+  return best;
+}
+
+bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
+                       const void *tw_L1, const void *tw_L2,
+                       const void *tw_L, hipStream_t s) {
+  const int L = p.L, L2 = L / L1;
+  if (L1 < 2 || L2 < 2 || L1 * L2 != L || p.st_v != 1) return false;
+  DMX_ASSERT(p.nu <= 65535 && L1 <= 65535 && L2 <= 65535,
+             "FFT line set too large");
+  const int maxel = fp64 ? FFT_MAXEL64 : FFT_MAXEL32;
+  int Bl = std::min(fp64 ? 32 : 64, maxel / (std::max(L1, L2) + 1));
+  Bl = std::max(1, std::min(Bl, p.nv));
+  const size_t esz = fp64 ? 16 : 8;
+  FftCols c{};
+  c.L = L;
+  c.sign = p.sign;
+  c.Bl = Bl;
+  c.rs = p.st_ax;
+  c.ncol = p.nv;
+  c.vn1 = p.vn1; c.va1 = p.va1; c.va2 = p.va2;
+  c.nouter = p.nu;
+  c.on1 = p.un1; c.oa1 = p.ua1; c.oa2 = p.ua2;
+  c.st_o = p.st_u;
+  c.tw_full = tw_L;
+  auto launch = [&](int Lsub) {
+    auto f = factorize(Lsub);
+    c.Lsub = Lsub;
+    c.npass = (int)f.size();
+    for (size_t i = 0; i < f.size(); ++i) c.radix[i] = f[i];
+    size_t sh = (size_t)Bl * (Lsub + 1) * esz;
+    dim3 grid((unsigned)((p.nv + Bl - 1) / Bl), (unsigned)c.nlines,
+              (unsigned)p.nu);
+    if (fp64)
+      hipLaunchKernelGGL(k_fft_cols<double>, grid, dim3(FFT_THREADS), sh, s, c);
+    else
+      hipLaunchKernelGGL(k_fft_cols<float>, grid, dim3(FFT_THREADS), sh, s, c);
+    DMX_HIP(hipGetLastError());
+  };
+  // pass A: in place, length L1 over rows n1*L2+n2, twiddled
+  c.src = p.data;
+  c.dst = p.data;
+  c.tw_sub = tw_L1;
+  c.twiddle = 1;
+  c.ia = 1; c.ib = L2; c.oa = 1; c.ob = L2;
+  c.nlines = L2;
+  launch(L1);
+  // pass B: length L2 over rows k1*L2+n2, to row k1 + L1*k2 of dst
+  c.src = p.data;
+  c.dst = dst;
+  c.tw_sub = tw_L2;
+  c.twiddle = 0;
+  c.ia = L2; c.ib = 1; c.oa = 1; c.ob = L1;
+  c.nlines = L1;
+  launch(L2);
+  return true;
 }
 
 void fft_lineset(const FftLaunch &base, bool fp64, void *scratch,

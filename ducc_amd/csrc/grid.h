@@ -39,5 +39,8 @@ void launch_corfac(const CorfacArgs &a, double *out, hipStream_t s);
 void launch_grid_copy(bool fp64, bool pad, const GridCopyArgs &a,
                       hipStream_t s);
 void launch_nudft(const NudftArgs &a, hipStream_t s);
+// dst[i] = src[idx[i]] for complex128 arrays
+void launch_gather_c128(const void *src, const int64_t *idx, size_t n,
+                        void *dst, hipStream_t s);
 
 }  // namespace dmx

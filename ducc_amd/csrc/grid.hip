@@ -96,17 +96,21 @@ __global__ void k_nudft(NudftArgs a) {
         kk[d] = (double)(idx - n / 2) * (2 * pi / a.period[d]);
       }
     }
-    for (int64_t base = 0; base < a.M; base += 256) {
+    // this block's share of the points (blockIdx.y of gridDim.y chunks)
+    const int64_t chunk = (a.M + gridDim.y - 1) / gridDim.y;
+    const int64_t jlo = (int64_t)blockIdx.y * chunk;
+    const int64_t jhi = min(a.M, jlo + chunk);
+    for (int64_t base = jlo; base < jhi; base += 256) {
       __syncthreads();
       int64_t jp = base + threadIdx.x;
-      if (jp < a.M) {
+      if (jp < jhi) {
         for (int d = 0; d < a.ndim; ++d)
           sh[threadIdx.x * 5 + d] = a.coord[jp * a.ndim + d];
         sh[threadIdx.x * 5 + 3] = a.vals[2 * jp];
         sh[threadIdx.x * 5 + 4] = a.vals[2 * jp + 1];
       }
       __syncthreads();
-      int cnt = (int)min((int64_t)256, a.M - base);
+      int cnt = (int)min((int64_t)256, jhi - base);
       if (q < a.nsel)
         for (int t = 0; t < cnt; ++t) {
           double ph = 0;
@@ -128,8 +132,11 @@ __global__ void k_nudft(NudftArgs a) {
     }
     int64_t ntot = (int64_t)a.n[0] * a.n[1] * a.n[2];
     int64_t n1 = a.n[1], n2 = a.n[2];
+    const int64_t chunk = (ntot + gridDim.y - 1) / gridDim.y;
+    const int64_t flo = (int64_t)blockIdx.y * chunk;
+    const int64_t fhi = min(ntot, flo + chunk);
     if (q < a.nsel)
-      for (int64_t flat = 0; flat < ntot; ++flat) {
+      for (int64_t flat = flo; flat < fhi; ++flat) {
         int64_t uu[3] = {flat / (n1 * n2), (flat / n2) % n1, flat % n2};
         double ph = 0;
         for (int d = 0; d < a.ndim; ++d) {
@@ -145,12 +152,32 @@ __global__ void k_nudft(NudftArgs a) {
       }
   }
   if (q < a.nsel) {
-    a.out[2 * q] = accr;
-    a.out[2 * q + 1] = acci;
+    if (gridDim.y == 1) {
+      a.out[2 * q] = accr;
+      a.out[2 * q + 1] = acci;
+    } else {  // partial sums of the chunks (out zeroed by the launcher)
+      atomicAdd(&a.out[2 * q], accr);
+      atomicAdd(&a.out[2 * q + 1], acci);
+    }
   }
 }
 
+__global__ void k_gather_c128(const double2 *src, const int64_t *idx,
+                              size_t n, double2 *dst) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
 }  // namespace
+
+void launch_gather_c128(const void *src, const int64_t *idx, size_t n,
+                        void *dst, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_gather_c128, dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, s, (const double2 *)src, idx, n,
+                     (double2 *)dst);
+  DMX_HIP(hipGetLastError());
+}
 
 void launch_corfac(const CorfacArgs &a, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_corfac, dim3((a.n + 255) / 256), dim3(256), 0, s, a,
@@ -175,8 +202,13 @@ void launch_grid_copy(bool fp64, bool pad, const GridCopyArgs &a,
 
 void launch_nudft(const NudftArgs &a, hipStream_t s) {
   if (!a.nsel) return;
-  hipLaunchKernelGGL(k_nudft, dim3((unsigned)((a.nsel + 255) / 256)), dim3(256),
-                     0, s, a);
+  const unsigned bx = (unsigned)((a.nsel + 255) / 256);
+  // split the inner sum so that about 4096 workgroups run
+  const int64_t inner = a.type == 1 ? a.M : a.n[0] * a.n[1] * a.n[2];
+  int64_t ny = std::max<int64_t>(1, std::min<int64_t>(4096 / bx, inner / 4096));
+  ny = std::min<int64_t>(ny, 65535);
+  if (ny > 1) launch_zero(a.out, (size_t)a.nsel * 16, s);
+  hipLaunchKernelGGL(k_nudft, dim3(bx, (unsigned)ny), dim3(256), 0, s, a);
   DMX_HIP(hipGetLastError());
 }
 

@@ -12,7 +12,7 @@ void go(const SpreadArgs &a, void *out, int out_f64, hipStream_t s) {
   unsigned nb = (unsigned)std::min<size_t>(a.nsub_max, INTERP_MAX_BLOCKS);
   if (nb == 0) return;
   if (a.use_lds) {
-    size_t cells = a.lds_side;
+    size_t cells = interp_inner_pad(a.lds_side);
     for (int d = 1; d < NDIM; ++d) cells *= a.lds_side;
     size_t sh = 2 * cells * sizeof(T);
     hipLaunchKernelGGL((k_interp<T, NDIM, W, true, MODE>), dim3(nb), dim3(256),

@@ -84,6 +84,12 @@ struct Params {
   int log2tile = 4;
   bool use_lds = true;
   int chooser_used = 0;
+  int chooser_verified = 0;
+};
+
+// device constants shared by all plans of one parameter set
+struct Setup {
+  Buf coef, corfac[3], one;
 };
 
 // choose precision, kernel, oversampled grid and tile for a request
@@ -149,12 +155,13 @@ class Plan {
   const void *coords_ = nullptr;  // points to user coords or sorted copy
   bool coords_sorted_ = false;
   bool coord_f32_ = false;
-  Buf coef_, corfac_[3], one_;
-  std::vector<Buf> tw_;
-  std::vector<int> tw_len_;
+  std::shared_ptr<Setup> su_;
+  std::vector<std::shared_ptr<Buf>> tw_keep_;
+  static std::shared_ptr<Setup> setup_for(const Params &p);
   Buf idx_, subs_, nsub_total_, sorted_coords_;
   Buf tmp_keys_, tmp_skeys_, tmp_tiles_, tmp_work_;
   Buf grid_, scratch_;
+  void *cur_ = nullptr;  // buffer holding the grid after the FFT passes
   const void *twiddle_for(int L);
 };
 

@@ -138,10 +138,12 @@ int default_log2tile(int ndim, bool fp64) {
 // LDS of the tile region: re and im planes of FP64 accumulators (spreading;
 // interpolation stages the same region in the compute type)
 size_t lds_bytes(int ndim, int tile, int W, bool fp64) {
-  (void)fp64;
-  size_t S = (size_t)tile + W - 1, c = S;
-  for (int d = 1; d < ndim; ++d) c *= S;
-  return c * 16;
+  size_t S = (size_t)tile + W - 1, c = S, ci = (S + 3) & ~(size_t)1;
+  for (int d = 1; d < ndim; ++d) {
+    c *= S;
+    ci *= S;
+  }
+  return std::max(c * 16, ci * (fp64 ? 16 : 8));
 }
 constexpr size_t LDS_LIMIT = 65536;
 
@@ -195,6 +197,7 @@ Params choose_params(int ndim, const size_t *nuni, size_t M, bool gridding,
       l2t = lc.log2tile;
       for (int d = 0; d < ndim; ++d) p.nover[d] = lc.nover[d];
       p.chooser_used = 1;
+      p.chooser_verified = lc.verified;
     }
   }
   if (kidx < 0) {
@@ -254,35 +257,7 @@ Plan::Plan(const Params &p, hipStream_t s) : p_(p), s_(s) {
   }
   lds_side_ = (1 << p.log2tile) + p.W - 1;
   ntiles_ = (size_t)g_.ntiles[0] * g_.ntiles[1] * g_.ntiles[2];
-  // kernel coefficients in the compute type
-  auto c = device_coeffs(p.W, p.beta, p.e0);
-  if (p.fp64) {
-    coef_.reset(c.size() * 8);
-    DMX_HIP(hipMemcpyAsync(coef_.p, c.data(), c.size() * 8,
-                           hipMemcpyHostToDevice, s_));
-  } else {
-    std::vector<float> cf(c.begin(), c.end());
-    coef_.reset(cf.size() * 4);
-    DMX_HIP(hipMemcpyAsync(coef_.p, cf.data(), cf.size() * 4,
-                           hipMemcpyHostToDevice, s_));
-  }
-  // correction factors, FP64 on the device
-  CorfacArgs ca{};
-  ca.nq = corr_quadrature(p.W, p.beta * p.W, p.e0, ca.x, ca.wpsi);
-  ca.W = p.W;
-  for (int d = 0; d < p.ndim; ++d) {
-    bool reuse = d > 0 && p.nuni[d] == p.nuni[d - 1] && p.nover[d] == p.nover[d - 1];
-    if (reuse) continue;
-    ca.n = (int)(p.nuni[d] / 2 + 1);
-    ca.dx = 1. / (double)p.nover[d];
-    corfac_[d].reset((size_t)ca.n * 8);
-    launch_corfac(ca, corfac_[d].as<double>(), s_);
-  }
-  static const double one = 1.0;
-  one_.reset(8);
-  DMX_HIP(hipMemcpyAsync(one_.p, &one, 8, hipMemcpyHostToDevice, s_));
-  // the host staging above is pageable; make sure it is consumed
-  DMX_HIP(hipStreamSynchronize(s_));
+  su_ = setup_for(p);
 }
 
 Plan::~Plan() {
@@ -290,17 +265,97 @@ Plan::~Plan() {
   flush_profile();
 }
 
+namespace {
+// Device constants of a parameter set (kernel coefficients in the compute
+// type, FP64 correction factors per axis) and FFT twiddles per length,
+// built once and kept in small per-process caches: a repeated request
+// touches no host-device copy and waits on no stream.
+struct SetupCache {
+  std::mutex mu;
+  std::map<std::string, std::shared_ptr<Setup>> m;
+  std::vector<std::string> order;
+  std::map<std::string, std::shared_ptr<Buf>> tw;
+};
+SetupCache &setup_cache() {
+  static SetupCache *c = new SetupCache();
+  return *c;
+}
+constexpr size_t SETUP_CACHE_MAX = 64;
+}  // namespace
+
+std::shared_ptr<Setup> Plan::setup_for(const Params &p) {
+  int dev = 0;
+  DMX_HIP(hipGetDevice(&dev));
+  std::string key = std::to_string(dev) + ":" + std::to_string(p.kidx) + ":" +
+                    std::to_string(p.W) + ":" + (p.fp64 ? "d" : "s");
+  char b[64];
+  std::snprintf(b, sizeof(b), ":%.17g:%.17g", p.beta, p.e0);
+  key += b;
+  for (int d = 0; d < p.ndim; ++d)
+    key += ":" + std::to_string(p.nuni[d]) + "/" + std::to_string(p.nover[d]);
+  auto &C = setup_cache();
+  std::lock_guard<std::mutex> lk(C.mu);
+  auto it = C.m.find(key);
+  if (it != C.m.end()) return it->second;
+  auto su = std::make_shared<Setup>();
+  // kernel coefficients in the compute type
+  auto c = device_coeffs(p.W, p.beta, p.e0);
+  if (p.fp64) {
+    su->coef.reset(c.size() * 8);
+    DMX_HIP(hipMemcpy(su->coef.p, c.data(), c.size() * 8,
+                      hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> cf(c.begin(), c.end());
+    su->coef.reset(cf.size() * 4);
+    DMX_HIP(hipMemcpy(su->coef.p, cf.data(), cf.size() * 4,
+                      hipMemcpyHostToDevice));
+  }
+  // correction factors, FP64 on the device
+  CorfacArgs ca{};
+  ca.nq = corr_quadrature(p.W, p.beta * p.W, p.e0, ca.x, ca.wpsi);
+  ca.W = p.W;
+  for (int d = 0; d < p.ndim; ++d) {
+    bool reuse = d > 0 && p.nuni[d] == p.nuni[d - 1] &&
+                 p.nover[d] == p.nover[d - 1];
+    if (reuse) continue;
+    ca.n = (int)(p.nuni[d] / 2 + 1);
+    ca.dx = 1. / (double)p.nover[d];
+    su->corfac[d].reset((size_t)ca.n * 8);
+    launch_corfac(ca, su->corfac[d].as<double>(), nullptr);
+  }
+  static const double one = 1.0;
+  su->one.reset(8);
+  DMX_HIP(hipMemcpy(su->one.p, &one, 8, hipMemcpyHostToDevice));
+  DMX_HIP(hipDeviceSynchronize());
+  C.m[key] = su;
+  C.order.push_back(key);
+  if (C.order.size() > SETUP_CACHE_MAX) {
+    C.m.erase(C.order.front());  // plans in flight keep their shared_ptr
+    C.order.erase(C.order.begin());
+  }
+  return su;
+}
+
 const void *Plan::twiddle_for(int L) {
-  for (size_t i = 0; i < tw_len_.size(); ++i)
-    if (tw_len_[i] == L) return tw_[i].p;
+  int dev = 0;
+  DMX_HIP(hipGetDevice(&dev));
+  const std::string key = std::to_string(dev) + ":" + std::to_string(L) +
+                          (p_.fp64 ? "d" : "s");
+  auto &C = setup_cache();
+  std::lock_guard<std::mutex> lk(C.mu);
+  auto it = C.tw.find(key);
+  if (it != C.tw.end()) {
+    tw_keep_.push_back(it->second);
+    return it->second->p;
+  }
   std::vector<unsigned char> h;
   fft_twiddles(L, p_.fp64, h);
-  tw_.emplace_back(h.size());
-  DMX_HIP(hipMemcpyAsync(tw_.back().p, h.data(), h.size(),
-                         hipMemcpyHostToDevice, s_));
-  DMX_HIP(hipStreamSynchronize(s_));
-  tw_len_.push_back(L);
-  return tw_.back().p;
+  auto buf = std::make_shared<Buf>(h.size());
+  DMX_HIP(hipMemcpy(buf->p, h.data(), h.size(), hipMemcpyHostToDevice));
+  if (C.tw.size() > 4 * SETUP_CACHE_MAX) C.tw.clear();
+  C.tw[key] = buf;
+  tw_keep_.push_back(buf);
+  return buf->p;
 }
 
 void Plan::build_index(const void *coords, bool coord_f32, size_t M,
@@ -357,7 +412,7 @@ void Plan::fill_spread_args(SpreadArgs &a) const {
   a.nsub_total = nsub_total_.as<uint32_t>();
   a.idx = idx_.as<uint32_t>();
   a.coords = coords_;
-  a.coef = coef_.p;
+  a.coef = su_->coef.p;
   a.nsub_max = nsub_max_;
   a.lds_side = lds_side_;
   a.use_lds = p_.use_lds ? 1 : 0;
@@ -369,11 +424,14 @@ void Plan::prepare_grid_side() {
   for (int d = 0; d < p_.ndim; ++d) n *= p_.nover[d];
   size_t bytes = n * (p_.fp64 ? 16 : 8);
   if (grid_.bytes != bytes) grid_.reset(bytes);
-  bool need_scratch = false;
+  // the second buffer serves lines longer than LDS and the out-of-place
+  // half of the two-pass column FFT (2-D)
+  bool need_scratch = p_.ndim == 2;
   for (int d = 0; d < p_.ndim; ++d)
     if ((int)p_.nover[d] > (p_.fp64 ? FFT_MAXEL64 : FFT_MAXEL32))
       need_scratch = true;
   if (need_scratch && scratch_.bytes != bytes) scratch_.reset(bytes);
+  cur_ = grid_.p;
 }
 
 void Plan::run_fft(bool forward, bool type1) {
@@ -404,10 +462,12 @@ void Plan::run_fft(bool forward, bool type1) {
   std::vector<int> order;
   if (type1) for (int a = 2; a >= f; --a) order.push_back(a);
   else for (int a = f; a <= 2; ++a) order.push_back(a);
+  const int maxel = p_.fp64 ? FFT_MAXEL64 : FFT_MAXEL32;
+  bool cols_moved = false;  // the 2-D column pass left the grid in scratch
   for (int a : order) {
     int pax = a == 0 ? 1 : 0, qax = a == 2 ? 1 : 2;
     FftLaunch L{};
-    L.data = grid_.p;
+    L.data = cur_;
     L.L = (int)E[a];
     L.st_ax = st[a];
     L.st_u = st[pax];
@@ -416,9 +476,31 @@ void Plan::run_fft(bool forward, bool type1) {
     range(pax, pax > a, L.nu, L.un1, L.ua1, L.ua2);
     range(qax, qax > a, L.nv, L.vn1, L.va1, L.va2);
     if (L.L <= 1) continue;
-    L.tw = twiddle_for(L.L);
     Phase ph(this, std::string("FFT axis ") + std::to_string(a - f));
-    fft_lineset(L, p_.fp64, scratch_.p, s_);
+    // 2-D columns too long for >= 16 per workgroup: two coalesced passes
+    // (64 columns per workgroup), the second writing natural order into
+    // the other buffer
+    const int L1 = fft_split(L.L);
+    if (nd == 2 && L.st_ax > 1 && maxel / L.L < 16 && L1 > 0 &&
+        scratch_.p) {
+      void *other = cur_ == grid_.p ? scratch_.p : grid_.p;
+      // rows longer than LDS cannot skip the unwritten columns: clear them
+      if (!type1 && E[2] > maxel) launch_zero(other, grid_.bytes, s_);
+      fft_columns_2pass(L, L1, other, p_.fp64, twiddle_for(L1),
+                        twiddle_for(L.L / L1), twiddle_for(L.L), s_);
+      cur_ = other;
+      cols_moved = true;
+      continue;
+    }
+    if (cols_moved && L.st_ax == 1) {
+      // rows after the column pass of type 2: only the kept columns were
+      // written; the others are zero by construction
+      L.kprune = 1;
+      L.kn1 = (n_act[a] + 1) / 2;
+      L.ka2 = N_act[a] - n_act[a] / 2;
+    }
+    L.tw = twiddle_for(L.L);
+    fft_lineset(L, p_.fp64, cur_ == grid_.p ? scratch_.p : grid_.p, s_);
   }
 }
 
@@ -441,7 +523,7 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
   }
   run_fft(forward, true);
   GridCopyArgs c{};
-  c.grid = grid_.p;
+  c.grid = cur_;
   c.uni = uni;
   c.u_f64 = uni_f64 ? 1 : 0;
   c.fft_order = p_.fft_order ? 1 : 0;
@@ -452,10 +534,10 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
   }
   const double *cf[3];
   for (int d = 0; d < 3; ++d) {
-    if (d >= p_.ndim) { cf[d] = one_.as<double>(); continue; }
+    if (d >= p_.ndim) { cf[d] = su_->one.as<double>(); continue; }
     int e = d;
-    while (!corfac_[e].p) --e;  // reused from the previous axis
-    cf[d] = corfac_[e].as<double>();
+    while (!su_->corfac[e].p) --e;  // reused from the previous axis
+    cf[d] = su_->corfac[e].as<double>();
   }
   c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
   Phase ph(this, "grid correction");
@@ -482,10 +564,10 @@ void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
   }
   const double *cf[3];
   for (int d = 0; d < 3; ++d) {
-    if (d >= p_.ndim) { cf[d] = one_.as<double>(); continue; }
+    if (d >= p_.ndim) { cf[d] = su_->one.as<double>(); continue; }
     int e = d;
-    while (!corfac_[e].p) --e;
-    cf[d] = corfac_[e].as<double>();
+    while (!su_->corfac[e].p) --e;
+    cf[d] = su_->corfac[e].as<double>();
   }
   c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
   {
@@ -496,7 +578,7 @@ void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
   if (M_ > 0) {
     SpreadArgs a{};
     fill_spread_args(a);
-    a.grid = grid_.p;
+    a.grid = cur_;
     Phase ph(this, "interpolation");
     launch_interp(p_.fp64, a, out, out_f64 ? 1 : 0, s_);
   }
@@ -522,10 +604,10 @@ void Plan::u2nu_grad(bool forward, const void *uni, bool uni_f64,
   }
   const double *cf[3];
   for (int d = 0; d < 3; ++d) {
-    if (d >= p_.ndim) { cf[d] = one_.as<double>(); continue; }
+    if (d >= p_.ndim) { cf[d] = su_->one.as<double>(); continue; }
     int e = d;
-    while (!corfac_[e].p) --e;
-    cf[d] = corfac_[e].as<double>();
+    while (!su_->corfac[e].p) --e;
+    cf[d] = su_->corfac[e].as<double>();
   }
   c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
   {
@@ -536,7 +618,7 @@ void Plan::u2nu_grad(bool forward, const void *uni, bool uni_f64,
   if (M_ > 0) {
     SpreadArgs a{};
     fill_spread_args(a);
-    a.grid = grid_.p;
+    a.grid = cur_;
     Phase ph(this, "interpolation (derivative)");
     launch_interp_grad(p_.fp64, a, out, out_f64 ? 1 : 0, s_);
   }

@@ -63,54 +63,74 @@ __device__ __forceinline__ unsigned lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
+// lanes of the wave holding the same 8-bit digit (ballot per bit)
+__device__ __forceinline__ unsigned long long digit_peers(uint32_t d,
+                                                          bool valid) {
+  unsigned long long peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const int bit = (d >> b) & 1;
+    const unsigned long long bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  return valid ? peers : 0ull;
+}
+
+// Stable scatter of one 4096-key block.  Wave w owns keys
+// [w*1024, (w+1)*1024) of the block: (1) each wave counts its digits,
+// (2) one prefix over the four waves gives every wave its output base per
+// digit, (3) each wave ranks its keys in order (ballot peers + a running
+// per-digit counter that only its own lanes touch), with no further
+// workgroup barrier.
 __global__ __launch_bounds__(RT) void k_radix_scatter(
     const uint32_t *keys_in, const uint32_t *idx_in, size_t M, int shift,
     const uint32_t *offs, int nblocks, uint32_t *keys_out,
     uint32_t *idx_out) {
-  __shared__ uint32_t base[RB], run[RB], wcnt[RT / 64][RB];
+  constexpr int NW = RT / 64, WCH = RTILE / NW, WR = WCH / 64;
+  __shared__ uint32_t cnt[NW][RB];
   const int tid = threadIdx.x, wave = tid >> 6;
   const unsigned lane = lane_id();
-  base[tid] = offs[(size_t)tid * nblocks + blockIdx.x];
-  run[tid] = 0;
 #pragma unroll
-  for (int w = 0; w < RT / 64; ++w) wcnt[w][tid] = 0;
+  for (int w = 0; w < NW; ++w) cnt[w][tid] = 0;
   __syncthreads();
-  const size_t b0 = (size_t)blockIdx.x * RTILE;
+  const size_t c0 = (size_t)blockIdx.x * RTILE + (size_t)wave * WCH;
   const unsigned long long below_mask =
       lane ? (~0ull >> (64 - lane)) : 0ull;
-  for (int r = 0; r < RITEMS; ++r) {
-    size_t j = b0 + (size_t)r * RT + tid;
-    bool valid = j < M;
-    uint32_t k = valid ? keys_in[j] : 0u;
-    uint32_t id = valid ? (idx_in ? idx_in[j] : (uint32_t)j) : 0u;
-    uint32_t d = (k >> shift) & 0xFFu;
-    unsigned long long peers = __ballot(valid);
+  for (int r = 0; r < WR; ++r) {
+    const size_t j = c0 + (size_t)r * 64 + lane;
+    const bool valid = j < M;
+    const uint32_t d = valid ? (keys_in[j] >> shift) & 0xFFu : 0u;
+    const unsigned long long peers = digit_peers(d, valid);
+    if (valid && (peers & below_mask) == 0)
+      cnt[wave][d] += (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  {
+    uint32_t run = offs[(size_t)tid * nblocks + blockIdx.x];
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      int bit = (d >> b) & 1;
-      unsigned long long bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t c = cnt[w][tid];
+      cnt[w][tid] = run;
+      run += c;
     }
-    if (!valid) peers = 0;
-    unsigned long long below = peers & below_mask;
-    uint32_t rank = (uint32_t)__popcll(below);
-    if (valid && below == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
+  }
+  __syncthreads();
+  for (int r = 0; r < WR; ++r) {
+    const size_t j = c0 + (size_t)r * 64 + lane;
+    const bool valid = j < M;
+    const uint32_t k = valid ? keys_in[j] : 0u;
+    const uint32_t d = (k >> shift) & 0xFFu;
+    const unsigned long long peers = digit_peers(d, valid);
+    const unsigned long long below = peers & below_mask;
     if (valid) {
-      uint32_t pos = base[d] + run[d] + rank;
-      for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+      const uint32_t pos = cnt[wave][d] + (uint32_t)__popcll(below);
       if (keys_out) keys_out[pos] = k;
-      idx_out[pos] = id;
+      idx_out[pos] = idx_in ? idx_in[j] : (uint32_t)j;
     }
-    __syncthreads();
-    uint32_t add = 0;
-#pragma unroll
-    for (int w = 0; w < RT / 64; ++w) {
-      add += wcnt[w][tid];
-      wcnt[w][tid] = 0;
-    }
-    run[tid] += add;
-    __syncthreads();
+    // every lane has read its base before the group leader advances it
+    __builtin_amdgcn_wave_barrier();
+    if (valid && below == 0) cnt[wave][d] += (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
   }
 }
 

@@ -230,21 +230,31 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   }
 }
 
+// Padded inner side of the interpolation's LDS region: even (16-byte
+// aligned rows of complex float) and two cells longer than the region, so
+// the aligned pair reads of the FP32 2-D path stay inside the row.
+__host__ __device__ inline int interp_inner_pad(int S) { return (S + 3) & ~1; }
+
 // MODE 0: values (out[j] complex); MODE 1: coordinate gradient
 // (out[j*NDIM+d] complex = d/dcoord_d of the interpolated value).
+// The region is staged in LDS as interleaved complex cells (one 8/16-byte
+// read per tap); 2-D FP32 reads aligned pairs of cells (16 bytes) and picks
+// the taps by the parity of the first column.
 template <typename T, int NDIM, int W, bool LDS, int MODE>
 __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
                                                 int out_f64) {
+  using C2 = typename cplx2<T>::type;
   extern __shared__ unsigned char smem_raw[];
   const uint32_t total = *a.nsub_total;
   const int S = a.lds_side;
+  const int SP = interp_inner_pad(S);
   const int tile = 1 << a.g.log2tile;
-  int ncell = S;
+  int ncell = SP;
   if (NDIM > 1) ncell *= S;
   if (NDIM > 2) ncell *= S;
-  T *sre = (T *)smem_raw;
-  T *sim = sre + ncell;
+  C2 *buf = (C2 *)smem_raw;
   const T *grid = (const T *)a.grid;
+  const C2 *gridc = (const C2 *)a.grid;
   const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
             n2 = NDIM > 2 ? a.g.nover[2] : 1;
   for (uint32_t sidx = blockIdx.x; sidx < total; sidx += gridDim.x) {
@@ -253,21 +263,22 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
     tile_origin(a.g, sb.tile, b0);
     if (LDS) {
       for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        // inner index over the padded side; the pad cells hold copies of
+        // in-range cells (wrapped), never used in arithmetic
         size_t gi;
         if constexpr (NDIM == 1) {
           gi = (size_t)wrapi(b0[0] + c, n0);
         } else if constexpr (NDIM == 2) {
-          const int cu = c / S, cv = c - cu * S;
+          const int cu = c / SP, cv = c - cu * SP;
           gi = (size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1);
         } else {
-          const int cu = c / (S * S), r = c - cu * S * S, cv = r / S,
-                    cw = r - cv * S;
+          const int cu = c / (S * SP), r = c - cu * S * SP, cv = r / SP,
+                    cw = r - cv * SP;
           gi = ((size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1)) *
                    n2 +
                wrapi(b0[2] + cw, n2);
         }
-        sre[c] = grid[2 * gi];
-        sim[c] = grid[2 * gi + 1];
+        buf[c] = gridc[gi];
       }
       __syncthreads();
     }
@@ -301,10 +312,11 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
         if (LDS) {
           int c;
           if constexpr (NDIM == 1) c = o[0] + c0;
-          else if constexpr (NDIM == 2) c = (o[0] + c0) * S + o[1] + c1;
-          else c = ((o[0] + c0) * S + o[1] + c1) * S + o[2] + c2;
-          re = sre[c];
-          im = sim[c];
+          else if constexpr (NDIM == 2) c = (o[0] + c0) * SP + o[1] + c1;
+          else c = ((o[0] + c0) * S + o[1] + c1) * SP + o[2] + c2;
+          const C2 v = buf[c];
+          re = v.x;
+          im = v.y;
         } else {
           size_t gi;
           if constexpr (NDIM == 1) gi = (size_t)wrapi(i0[0] + c0, n0);
@@ -327,6 +339,30 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
             cell(c0, 0, 0, re, im);
             ar += re * w[0][c0];
             ai += im * w[0][c0];
+          }
+        } else if constexpr (NDIM == 2 && LDS && sizeof(T) == 4) {
+          constexpr int P = (W + 2) / 2;  // aligned cell pairs per row
+          const bool odd = o[1] & 1;
+#pragma unroll
+          for (int c0 = 0; c0 < W; ++c0) {
+            const float4 *q =
+                (const float4 *)(buf + (o[0] + c0) * SP + (o[1] & ~1));
+            float4 v[P];
+#pragma unroll
+            for (int m = 0; m < P; ++m) v[m] = q[m];
+            T rr = 0, ri = 0;
+#pragma unroll
+            for (int c1 = 0; c1 < W; ++c1) {
+              // tap c1 is cell c1 (even start) or c1+1 (odd start)
+              const float4 ve = v[c1 >> 1], vo = v[(c1 + 1) >> 1];
+              const T er = (c1 & 1) ? ve.z : ve.x, ei = (c1 & 1) ? ve.w : ve.y;
+              const T orr = (c1 & 1) ? vo.x : vo.z, oi = (c1 & 1) ? vo.y : vo.w;
+              const T re = odd ? orr : er, im = odd ? oi : ei;
+              rr += re * w[1][c1];
+              ri += im * w[1][c1];
+            }
+            ar += rr * w[0][c0];
+            ai += ri * w[0][c0];
           }
         } else if constexpr (NDIM == 2) {
 #pragma unroll
@@ -359,8 +395,9 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
               pr += rr * w[1][c1];
               pi += ri * w[1][c1];
             }
-            ar += pr * w[0][c0];
-            ai += pi * w[0][c0];
+            const T w0 = w[0][c0];
+            ar += pr * w0;
+            ai += pi * w0;
           }
         }
         store_value<T>(out, j, out_f64, ar, ai);

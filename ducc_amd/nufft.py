@@ -5,10 +5,11 @@ messages as ducc0.nufft (python/nufft_pymod.cc:445-471), plus a few keyword
 arguments ducc0 does not have (all with defaults that keep ducc0's
 behaviour apart from the compute precision, see `u2nu`):
 
-  precision  "auto" | "fp32" | "fp64".  "auto" computes in FP32 when
-             epsilon >= fp64_below (1e-6) and the single-precision kernel
-             table reaches epsilon, otherwise in FP64; coordinates are always
-             folded in FP64 and correction factors are FP64.
+  precision  "auto" | "fp32" | "fp64".  "auto" computes in the precision
+             of the values (as ducc0 does), raised to FP64 for complex64
+             values when epsilon < 1e-6 or the single-precision kernel table
+             cannot reach epsilon; coordinates are always folded in FP64 and
+             correction factors are FP64.
   chooser    "analytic" (ducc0's cost model) | "learned" (the trained
              parameter network, with the analytic choice as fallback).
   info       a dict that receives the parameters actually used.
@@ -66,16 +67,19 @@ def _check_dtypes(vals, coord):
         raise RuntimeError("not yet supported")
 
 
-def _opts(precision, chooser, log2tile, stream, kernel_idx=-1, sigma=0.0):
+def _opts(precision, chooser, log2tile, stream, kernel_idx=-1, sigma=0.0,
+          spot_check=True):
     return _lib.options(precision=precision, chooser=chooser,
                         log2tile=log2tile, stream=stream,
-                        kernel_idx=kernel_idx, sigma=sigma)
+                        kernel_idx=kernel_idx, sigma=sigma,
+                        spot_check=spot_check)
 
 
 def u2nu(*, grid, coord, forward, epsilon, nthreads=1, out=None,
          verbosity=0, sigma_min=1.2, sigma_max=2.51,
          periodicity=_TWO_PI, fft_order=False, precision="auto",
-         chooser="analytic", log2tile=0, info=None):
+         chooser="analytic", log2tile=0, info=None, kernel_idx=-1,
+         spot_check=True):
     """Type 2 NUFFT: uniform grid -> values at nonuniform points.
 
     out_j = sum_k grid_k exp(isign i 2pi/P k.x_j), isign = -1 if forward,
@@ -95,7 +99,8 @@ def u2nu(*, grid, coord, forward, epsilon, nthreads=1, out=None,
     g, kg = _lib.describe(grid)
     c, kc = _lib.describe(coord)
     o, ko = _lib.describe(out)
-    opt = _opts(precision, chooser, log2tile, _stream_for(grid, coord, out))
+    opt = _opts(precision, chooser, log2tile, _stream_for(grid, coord, out),
+                kernel_idx=kernel_idx, spot_check=spot_check)
     inf = _lib.DmxPlanInfo()
     _lib.check(_lib.lib().dmx_u2nu(
         ct.byref(g), ct.byref(c), int(bool(forward)), float(epsilon),
@@ -112,7 +117,8 @@ def u2nu(*, grid, coord, forward, epsilon, nthreads=1, out=None,
 def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
          verbosity=0, sigma_min=1.2, sigma_max=2.51,
          periodicity=_TWO_PI, fft_order=False, precision="auto",
-         chooser="analytic", log2tile=0, info=None):
+         chooser="analytic", log2tile=0, info=None, kernel_idx=-1,
+         spot_check=True):
     """Type 1 NUFFT: values at nonuniform points -> uniform grid.
 
     out_k = sum_j points_j exp(isign i 2pi/P k.x_j).  `out` is required: its
@@ -127,7 +133,8 @@ def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
     p, kp = _lib.describe(points)
     c, kc = _lib.describe(coord)
     o, ko = _lib.describe(out)
-    opt = _opts(precision, chooser, log2tile, _stream_for(points, coord, out))
+    opt = _opts(precision, chooser, log2tile, _stream_for(points, coord, out),
+                kernel_idx=kernel_idx, spot_check=spot_check)
     inf = _lib.DmxPlanInfo()
     _lib.check(_lib.lib().dmx_nu2u(
         ct.byref(p), ct.byref(c), int(bool(forward)), float(epsilon),

@@ -53,7 +53,11 @@ typedef struct dmx_options {
   double fp64_below; /* auto policy switch, default 1e-6 */
   void *stream;      /* hipStream_t to run on; NULL: library stream */
   double sigma;      /* >0: force this oversampling factor (with kernel_idx) */
-  int32_t reserved[8];
+  int32_t spot_check;/* learned choices: verify accuracy once per request
+                        signature with an FP64 direct sum, fall back to the
+                        analytic choice on failure (default 1; 0 off; 2
+                        treats the check as failed, for tests) */
+  int32_t reserved[7];
 } dmx_options;
 
 typedef struct dmx_plan dmx_plan;
@@ -137,6 +141,36 @@ int dmx_sort_points(const dmx_array *coord, const int64_t *nover,
                     size_t ndim, int W, int log2tile,
                     const double *periodicity, dmx_array *keys,
                     dmx_array *idx);
+
+/* ---- learned parameter chooser (no ducc0 counterpart; its fallback and
+   label judge is ducc0's findNufftParameters, nufft.h:155-208) ----
+   Network: 6 features -> 64 -> 64 -> 64 (ReLU) -> 3 outputs (oversampling
+   factor, support W, log2 tile), float32, MLP_NPARAM = 8963 parameters
+   (W1 64x6, b1, W2 64x64, b2, W3 64x64, b3, W4 3x64, b4).  All arrays host
+   memory; the work runs on the GPU. */
+size_t dmx_mlp_nparam(void);
+/* Y[B][3] = net(X[B][6]) */
+int dmx_mlp_forward(const float *params, const float *X, int B, float *Y);
+/* loss = sum_b sum_k w_k (y_bk - T_bk)^2 / (3B) and its gradient */
+int dmx_mlp_grad(const float *params, const float *X, const float *T,
+                 const float *w, int B, float *grad, float *loss);
+/* `steps` full-batch Adam steps in place on params; loss per step into
+   `losses` (may be NULL) */
+int dmx_mlp_train(float *params, const float *X, const float *T,
+                  const float *w, int B, int steps, float lr,
+                  float weight_decay, float *losses);
+/* install trained weights and the normalisation (feature mean[6], std[6],
+   target mean[3], std[3]) used by chooser = learned */
+int dmx_chooser_set(const float *params, const double *norm);
+/* the 6 features of a request */
+int dmx_chooser_features(size_t ndim, const int64_t *shape, size_t npoints,
+                         double epsilon, int fp64, int nu2u, double *f);
+/* the parameters a call would use, without running it (info->chooser_used
+   tells whether the network's choice was taken) */
+int dmx_choose(size_t ndim, const int64_t *shape, size_t npoints,
+               double epsilon, int values_f64, int nu2u, double sigma_min,
+               double sigma_max, const dmx_options *opt,
+               dmx_plan_info *info);
 
 /* Phase timing (role of ducc0's TimerHierarchy / verbosity output,
    src/ducc0/infra/timers.h:69-262, nufft.h:469-471): when enabled, every
