@@ -6,9 +6,11 @@
    8-cluster Gaussian points).  For every request and transform type the
    plain method is timed over candidate (kernel row, tile) settings: for
    each support W the smallest tabulated oversampling factor in [1.2, 2.0]
-   reaching eps and two larger ones, times three tile sizes.  The label is
-   the fastest setting; every candidate row reaches eps by the table, and the
-   accuracy of the learned choice is re-checked at run time.
+   reaching eps and two larger ones, times three tile sizes, and the
+   analytic choice.  The label (pick_label) keeps the analytic setting
+   unless a candidate is more than 5% faster, and breaks near-ties (3%)
+   towards the smallest support; every candidate row reaches eps by the
+   table, and the accuracy of the learned choice is re-checked at run time.
 2. Training: 6 features -> (sigma, W, log2 tile), normalised, full-batch
    Adam with the library's HIP kernels (dmx_mlp_train).
 3. Evaluation: fresh requests timed with the analytic choice (ducc0's cost
@@ -115,6 +117,28 @@ def candidates(req, f64, db):
     return [(k, t) for k in out for t in tiles]
 
 
+def default_log2tile(nd, f64):
+    return 9 if nd == 1 else ((4 if f64 else 5) if nd == 2 else 3)
+
+
+def pick_label(times, an, db, nd, f64, margin=1.05, tie=1.03):
+    """Training target from one request's timings.  times: {(kidx, tile):
+    seconds}; an: (seconds, kidx, tile) of the analytic choice.  The
+    analytic setting stays the label unless a candidate beats it by more
+    than `margin`; among candidates within `tie` of the fastest the
+    smallest support, then the smallest oversampling, then the default tile
+    wins (timing noise otherwise flips labels between near-equal
+    settings)."""
+    best = min(times.values())
+    if best * margin > an[0]:
+        return an[1], an[2]
+    near = [kt for kt, t in times.items() if t <= best * tie]
+    dt = default_log2tile(nd, f64)
+    k, t = min(near, key=lambda kt: (db["W"][kt[0]], db["ofactor"][kt[0]],
+                                     abs(kt[1] - dt)))
+    return k, t
+
+
 def label_requests(n, seed, db, log):
     X, T, meta = [], [], []
     rng = np.random.default_rng(seed)
@@ -125,30 +149,37 @@ def label_requests(n, seed, db, log):
         out_g = torch.empty(req["shape"], dtype=p.dtype, device="cuda")
         out_p = torch.empty(req["M"], dtype=p.dtype, device="cuda")
         cands = candidates(req, f64, db)
+        nd = len(req["shape"])
         for typ in ("nu2u", "u2nu"):
-            best = None
-            for k, t in cands:
+            def run(**kw):
                 if typ == "nu2u":
-                    fn = lambda: nufft.nu2u(points=p, coord=c, forward=True,
-                                            epsilon=req["eps"], out=out_g,
-                                            kernel_idx=k, log2tile=t)
-                else:
-                    fn = lambda: nufft.u2nu(grid=g, coord=c, forward=True,
-                                            epsilon=req["eps"], out=out_p,
-                                            kernel_idx=k, log2tile=t)
+                    return lambda: nufft.nu2u(points=p, coord=c, forward=True,
+                                              epsilon=req["eps"], out=out_g,
+                                              **kw)
+                return lambda: nufft.u2nu(grid=g, coord=c, forward=True,
+                                          epsilon=req["eps"], out=out_p, **kw)
+            info = {}
+            try:
+                t_an = timed(run(chooser="analytic", info=info))
+            except RuntimeError:
+                continue
+            an = (t_an, int(info["kernel_idx"]), int(info["log2tile"]))
+            times = {}
+            for k, t in cands:
                 try:
-                    tt = timed(fn)
+                    times[(k, t)] = timed(run(kernel_idx=k, log2tile=t))
                 except RuntimeError:
                     continue
-                if best is None or tt < best[0]:
-                    best = (tt, k, t)
-            if best is None:
+            if not times:
                 continue
-            tt, k, t = best
+            k, t = pick_label(times, an, db, nd, f64)
             X.append(chooser.features(req["shape"], req["M"], req["eps"], f64,
                                       typ == "nu2u"))
             T.append([db["ofactor"][k], db["W"][k], t])
-            meta.append(dict(req=req, type=typ, time=tt, kidx=k, log2tile=t))
+            meta.append(dict(req=req, type=typ, analytic=list(an),
+                             label=[int(k), int(t)],
+                             times=[[int(kk), int(tt), v]
+                                    for (kk, tt), v in times.items()]))
         del c, p, g, out_g, out_p
         if q % 10 == 0:
             log(f"label {q}/{n}: {req['shape']} M={req['M']} "
@@ -179,15 +210,26 @@ def evaluate(n, seed, log):
         out_g = torch.empty(req["shape"], dtype=p.dtype, device="cuda")
         out_p = torch.empty(req["M"], dtype=p.dtype, device="cuda")
         rec = dict(req=req)
+        fns, infos = {}, {}
         for ch in ("analytic", "learned"):
             info1, info2 = {}, {}
 
-            def fn():
+            def fn(ch=ch, info1=info1, info2=info2):
                 nufft.nu2u(points=p, coord=c, forward=True, epsilon=req["eps"],
                            out=out_g, chooser=ch, info=info1)
                 nufft.u2nu(grid=g, coord=c, forward=True, epsilon=req["eps"],
                            out=out_p, chooser=ch, info=info2)
-            rec[ch] = dict(time=timed(fn), nu2u=dict(info1), u2nu=dict(info2))
+            fns[ch], infos[ch] = fn, (info1, info2)
+        # alternate the two settings so that drift affects both alike
+        tsum = {"analytic": 0.0, "learned": 0.0}
+        for rep in range(3):
+            for ch in (("analytic", "learned") if rep % 2 == 0
+                       else ("learned", "analytic")):
+                tsum[ch] += timed(fns[ch], reps=1)
+        for ch in ("analytic", "learned"):
+            info1, info2 = infos[ch]
+            rec[ch] = dict(time=tsum[ch] / 3, nu2u=dict(info1),
+                           u2nu=dict(info2))
             if q % 10 == 0:
                 # error of the type-1 result on 32 modes against the FP64
                 # direct sum
@@ -273,6 +315,8 @@ def main():
                   open(a.out, "w"))
         np.savez_compressed(os.path.splitext(a.report)[0] + "_train.npz",
                             X=X, T=T)
+        json.dump(meta, open(os.path.splitext(a.report)[0] + "_labels.json",
+                             "w"))
         log(f"weights written to {a.out}")
     chooser.load(a.out)
     rows = evaluate(a.eval, a.seed + 1, log)

@@ -45,6 +45,18 @@ struct Sub {
 };
 
 constexpr int MAXSUB = 1024;
+// points per LDS staging round of the spreader
+constexpr int SCHUNK = 256;
+// LDS of one spreading workgroup: re/im planes of FP64 accumulators over the
+// region of `cells` cells plus the staging of SCHUNK points (values, kernel
+// arguments, offsets, bucket counts and list)
+inline constexpr size_t spread_lds_bytes(size_t cells, int ndim, bool fp64) {
+  return 2 * cells * sizeof(double) +
+         (((size_t)SCHUNK * (2 * (fp64 ? 8 : 4) + ndim * (fp64 ? 8 : 4) + 4 +
+                             2) +
+           64 * 4 + 15) &
+          ~(size_t)15);
+}
 
 // ----------------------------------------------------------- device helpers
 #ifdef __HIPCC__
@@ -160,25 +172,33 @@ __device__ __forceinline__ int wrapi(int i, int n) {
 struct FftAxisPlan;  // defined in fft.hip
 
 // Launch wrappers (defined in the .hip files).
+// tile key of every point; hist0 (optional): the first sort pass's
+// histogram, as radix_sort_hist0 locates it in the sort workspace
 void launch_fold_keys(const Geo &g, const void *coords, size_t M,
-                      uint32_t *keys, uint32_t *counts, hipStream_t s);
+                      uint32_t *keys, uint32_t *hist0, hipStream_t s);
+// stable LSD sort of keys < 2^keybits; hist0_ready: the first pass's
+// histogram was already written by launch_fold_keys
 void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
                       uint32_t *idx_out, uint32_t *keys_out, void *work,
-                      size_t work_bytes, hipStream_t s);
+                      size_t work_bytes, hipStream_t s,
+                      bool hist0_ready = false);
 size_t radix_sort_workspace(size_t M);
+uint32_t *radix_sort_hist0(void *work, size_t M);
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n,
                         uint32_t *total, void *work, hipStream_t s);
 size_t scan_workspace(size_t n);
-void launch_build_subs(const uint32_t *counts, const uint32_t *tile_start,
-                       size_t ntiles, uint32_t *nsub, uint32_t *sub_start,
-                       Sub *subs, uint32_t *nsub_total, void *scanwork,
-                       hipStream_t s);
-void launch_tile_bounds(const uint32_t *skeys, size_t M, size_t ntiles,
-                        uint32_t *tstart, uint32_t *tend, uint32_t *counts,
-                        hipStream_t s);
+// from the sorted tile keys: per-tile [start, end) and point counts, and
+// the subproblem list (runs of at most MAXSUB points of one tile)
+void launch_tile_index(const uint32_t *skeys, size_t M, size_t ntiles,
+                       uint32_t *counts, uint32_t *tstart, uint32_t *tend,
+                       uint32_t *nsub, uint32_t *sub_start, Sub *subs,
+                       uint32_t *nsub_total, void *scanwork, hipStream_t s);
 void launch_gather_coords(const void *coords, int coord_f32, int ndim,
                           const uint32_t *idx, size_t M, void *out,
                           hipStream_t s);
+// dst[p] = src[idx[p]] for elements of 8 or 16 bytes
+void launch_gather_elems(const void *src, int elem_bytes, const uint32_t *idx,
+                         size_t n, void *dst, hipStream_t s);
 void launch_zero(void *p, size_t bytes, hipStream_t s);
 
 // spreading / interpolation (spread.hip, interp.hip)
@@ -188,14 +208,15 @@ struct SpreadArgs {
   const uint32_t *nsub_total;
   const uint32_t *idx;      // sorted order; NULL: identity (coords sorted)
   const void *coords;       // (M, ndim) float32/float64
-  const void *values;       // (M,) c64 or c128 in original order
+  const void *values;       // (M,) c64 or c128
   int val_f64;
+  int values_sorted;        // 1: values are stored in sorted order
   void *grid;               // oversampled grid, complex of compute type
   const void *coef;         // kernel coefficients of compute type
   size_t nsub_max;
   int lds_side;             // tile + W - 1
   int use_lds;
-  int coords_sorted;        // 1: coords are stored in sorted order (plan)
+  int coords_sorted;        // 1: coords are stored in sorted order
 };
 void launch_spread(bool fp64, const SpreadArgs &a, hipStream_t s);
 void launch_interp(bool fp64, const SpreadArgs &a, void *out, int out_f64,

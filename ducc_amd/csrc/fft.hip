@@ -135,6 +135,24 @@ __device__ __forceinline__ void dft(C *v, int sign) {
   }
 }
 
+// Quotient and remainder of x / d for 0 <= x < 2^22 and 0 < d, from a
+// float reciprocal and one correction step: the estimate is within one of
+// the true quotient because x * inv carries a relative error below 2^-22.
+// Integer division compiles to a long VALU sequence, which dominated the
+// butterflies before.
+__device__ __forceinline__ int fdivq(int x, int d, float inv, int &r) {
+  int q = (int)((float)x * inv);
+  r = x - q * d;
+  if (r < 0) {
+    --q;
+    r += d;
+  } else if (r >= d) {
+    ++q;
+    r -= d;
+  }
+  return q;
+}
+
 // One radix-R Stockham pass over Bl LDS-resident lines (stride LP).
 template <typename T, int R, int KM>
 __device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
@@ -145,14 +163,19 @@ __device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
   const int nbf = L / R;
   const int total = nbf * Bl;
   const int twstep = L / (Ns * R);
+  const float inv_nbf = 1.0f / (float)nbf, inv_ns = 1.0f / (float)Ns;
   C v[KM][R];
+  int boff[KM], jjk[KM];
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
     const int q = threadIdx.x + k * blockDim.x;
     if (q < total) {
-      const int b = q / nbf, jj = q - b * nbf;
+      int jj;
+      const int b = Bl == 1 ? (jj = q, 0) : fdivq(q, nbf, inv_nbf, jj);
+      boff[k] = b * LP;
+      jjk[k] = jj;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[k][r] = buf[b * LP + jj + r * nbf];
+      for (int r = 0; r < R; ++r) v[k][r] = buf[boff[k] + jj + r * nbf];
     }
   }
   __syncthreads();
@@ -160,8 +183,9 @@ __device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
   for (int k = 0; k < KM; ++k) {
     const int q = threadIdx.x + k * blockDim.x;
     if (q < total) {
-      const int b = q / nbf, jj = q - b * nbf;
-      const int kk = jj % Ns;
+      const int jj = jjk[k];
+      int kk;
+      const int jq = Ns == 1 ? (kk = 0, jj) : fdivq(jj, Ns, inv_ns, kk);
       if (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -171,9 +195,9 @@ __device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
         }
       }
       dft<T, R>(v[k], sign);
-      const int base = (jj / Ns) * Ns * R + kk;
+      const int base = jq * Ns * R + kk;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[b * LP + base + r * Ns] = v[k][r];
+      for (int r = 0; r < R; ++r) buf[boff[k] + base + r * Ns] = v[k][r];
     }
   }
   __syncthreads();
@@ -196,11 +220,12 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lds(FftLaunch p) {
   const int64_t ubase = map_range(u, p.un1, p.ua1, p.ua2) * p.st_u;
   const int nlines = min(Bl, p.nv - v0);
   const int nel = nlines * L;
+  const float inv_L = 1.0f / (float)L, inv_nl = 1.0f / (float)nlines;
   // load
   for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     int b, k;
-    if (p.st_ax == 1) { b = e / L; k = e - b * L; }
-    else { k = e / nlines; b = e - k * nlines; }
+    if (p.st_ax == 1) b = fdivq(e, L, inv_L, k);
+    else k = fdivq(e, nlines, inv_nl, b);
     if (p.kprune && k >= p.kn1 && k < p.ka2) {
       C z;
       z.x = T(0);
@@ -230,8 +255,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lds(FftLaunch p) {
   // store
   for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     int b, k;
-    if (p.st_ax == 1) { b = e / L; k = e - b * L; }
-    else { k = e / nlines; b = e - k * nlines; }
+    if (p.st_ax == 1) b = fdivq(e, L, inv_L, k);
+    else k = fdivq(e, nlines, inv_nl, b);
     const int64_t addr =
         ubase + map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v + k * p.st_ax;
     data[addr] = buf[b * LP + k];
@@ -257,8 +282,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(FftCols p) {
   const int64_t obase =
       map_range(blockIdx.z, p.on1, p.oa1, p.oa2) * p.st_o;
   const int nel = nb * Ls;
+  const float inv_nb = 1.0f / (float)nb, inv_L = 1.0f / (float)p.L;
   for (int i = threadIdx.x; i < nel; i += blockDim.x) {
-    const int e = i / nb, b = i - e * nb;
+    int b;
+    const int e = fdivq(i, nb, inv_nb, b);
     const int64_t row = (int64_t)u * p.ia + (int64_t)e * p.ib;
     buf[b * LP + e] =
         src[obase + row * p.rs + map_range(v0 + b, p.vn1, p.va1, p.va2)];
@@ -281,10 +308,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(FftCols p) {
   }
   const C *twf = (const C *)p.tw_full;
   for (int i = threadIdx.x; i < nel; i += blockDim.x) {
-    const int k = i / nb, b = i - k * nb;
+    int b;
+    const int k = fdivq(i, nb, inv_nb, b);
     C x = buf[b * LP + k];
     if (p.twiddle && u > 0 && k > 0) {
-      C w = twf[(int)(((int64_t)k * u) % p.L)];
+      int m;
+      fdivq(k * u, p.L, inv_L, m);  // k * u < L1 * L2 = L
+      C w = twf[m];
       if (p.sign > 0) w.y = -w.y;
       x = cmul(x, w);
     }
@@ -450,7 +480,9 @@ void fft_lineset(const FftLaunch &base, bool fp64, void *scratch,
   const int maxel = fp64 ? FFT_MAXEL64 : FFT_MAXEL32;
   if (p.L <= maxel) {
     int Bl = maxel / p.L;
-    if (p.st_ax == 1) Bl = std::min(Bl, 8);
+    // contiguous lines: about 32 KiB of LDS per workgroup, so that several
+    // workgroups per CU overlap their loads with the passes of the others
+    if (p.st_ax == 1) Bl = std::min(Bl, std::max(1, (maxel / 2) / p.L));
     Bl = std::max(1, std::min(Bl, p.nv));
     // cap lines per block by the register staging capacity: the staged
     // butterflies per thread never exceed FFT_KM for any radix

@@ -102,8 +102,10 @@ class Plan {
  public:
   Plan(const Params &p, hipStream_t s);
   ~Plan();
-  // sorts the points; coords: device (M, ndim), f32 or f64, contiguous.
-  // keep_sorted_coords: store a sorted copy (planned use).
+  // sorts the points; coords: device (M, ndim), f32 or f64, contiguous,
+  // alive while the plan executes unless keep_sorted_coords.
+  // keep_sorted_coords: store a copy in sorted order (for plans executed
+  // several times: one random gather instead of one per execution).
   void build_index(const void *coords, bool coord_f32, size_t M,
                    bool keep_sorted_coords);
   // type 1: points (device, M complex, c64/c128) -> uniform (device, strided)
@@ -152,7 +154,8 @@ class Plan {
   size_t ntiles_ = 0;
   size_t nsub_max_ = 0;
   int lds_side_ = 0;
-  const void *coords_ = nullptr;  // points to user coords or sorted copy
+  const void *coords_ = nullptr;  // user coords or the sorted copy
+  const void *user_coords_ = nullptr;  // caller's coordinates (input order)
   bool coords_sorted_ = false;
   bool coord_f32_ = false;
   std::shared_ptr<Setup> su_;

@@ -135,7 +135,7 @@ int default_log2tile(int ndim, bool fp64) {
   if (ndim == 2) return fp64 ? 4 : 5;
   return 3;
 }
-// LDS of the tile region: re and im planes of FP64 accumulators (spreading;
+// LDS of the tile region: FP64 accumulators and point staging (spreading;
 // interpolation stages the same region in the compute type)
 size_t lds_bytes(int ndim, int tile, int W, bool fp64) {
   size_t S = (size_t)tile + W - 1, c = S, ci = (S + 3) & ~(size_t)1;
@@ -143,7 +143,7 @@ size_t lds_bytes(int ndim, int tile, int W, bool fp64) {
     c *= S;
     ci *= S;
   }
-  return std::max(c * 16, ci * (fp64 ? 16 : 8));
+  return std::max(spread_lds_bytes(c, ndim, fp64), ci * (fp64 ? 16 : 8));
 }
 constexpr size_t LDS_LIMIT = 65536;
 
@@ -365,6 +365,7 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
   coord_f32_ = coord_f32;
   g_.coord_f32 = coord_f32 ? 1 : 0;
   coords_ = coords;
+  user_coords_ = coords;
   coords_sorted_ = false;
   if (M == 0) return;
   int keybits = 1;
@@ -379,19 +380,23 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
   uint32_t *counts = tmp_tiles_.as<uint32_t>(), *tstart = counts + ntiles_,
            *tend = tstart + ntiles_, *nsub = tend + ntiles_,
            *sstart = nsub + ntiles_;
-  Phase ph(this, "building index");
-  launch_fold_keys(g_, coords, M, tmp_keys_.as<uint32_t>(), nullptr, s_);
-  idx_.reset(M * 4);
-  radix_sort_pairs(tmp_keys_.as<uint32_t>(), M, keybits, idx_.as<uint32_t>(),
-                   tmp_skeys_.as<uint32_t>(), tmp_work_.p, ws, s_);
-  launch_tile_bounds(tmp_skeys_.as<uint32_t>(), M, ntiles_, tstart, tend,
-                     counts, s_);
-  nsub_max_ = std::min(ntiles_, M) + (M + MAXSUB - 1) / MAXSUB;
-  subs_.reset(nsub_max_ * sizeof(Sub));
-  nsub_total_.reset(4);
-  launch_build_subs(counts, tstart, ntiles_, nsub, sstart, subs_.as<Sub>(),
-                    nsub_total_.as<uint32_t>(), tmp_work_.p, s_);
+  {
+    Phase ph(this, "building index");
+    launch_fold_keys(g_, coords, M, tmp_keys_.as<uint32_t>(),
+                     radix_sort_hist0(tmp_work_.p, M), s_);
+    idx_.reset(M * 4);
+    radix_sort_pairs(tmp_keys_.as<uint32_t>(), M, keybits, idx_.as<uint32_t>(),
+                     tmp_skeys_.as<uint32_t>(), tmp_work_.p, ws, s_, true);
+    nsub_max_ = std::min(ntiles_, M) + (M + MAXSUB - 1) / MAXSUB;
+    subs_.reset(nsub_max_ * sizeof(Sub));
+    nsub_total_.reset(4);
+    launch_tile_index(tmp_skeys_.as<uint32_t>(), M, ntiles_, counts, tstart,
+                      tend, nsub, sstart, subs_.as<Sub>(),
+                      nsub_total_.as<uint32_t>(), tmp_work_.p, s_);
+  }
   if (keep_sorted_coords) {
+    // a random gather of 1e7 points costs about as much as the gathers it
+    // saves in one spreading plus one interpolation, so only plans keep it
     Phase ps(this, "sorting coords");
     sorted_coords_.reset(M * p_.ndim * (coord_f32 ? 4 : 8));
     launch_gather_coords(coords, coord_f32, p_.ndim, idx_.as<uint32_t>(), M,
@@ -402,7 +407,7 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
 }
 
 uint32_t *Plan::keys_copy_to(uint32_t *dev_keys_out) {
-  launch_fold_keys(g_, coords_, M_, dev_keys_out, nullptr, s_);
+  launch_fold_keys(g_, user_coords_, M_, dev_keys_out, nullptr, s_);
   return dev_keys_out;
 }
 

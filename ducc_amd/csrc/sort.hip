@@ -9,7 +9,8 @@
 // histogram per 4096-key block), one exclusive scan of the digit-major
 // block table, and one scatter kernel that ranks equal digits inside a
 // wavefront with ballots (stable) and across the block's four wavefronts in
-// wave order, so every pass is stable.
+// wave order, so every pass is stable; the ranked block is staged in LDS and
+// written out in digit runs.
 #include "dmx.h"
 
 namespace dmx {
@@ -20,26 +21,44 @@ constexpr int RT = 256;         // threads per block
 constexpr int RITEMS = 16;      // keys per thread
 constexpr int RTILE = RT * RITEMS;
 
-__global__ __launch_bounds__(256) void k_fold_keys(Geo g, const void *coords,
-                                                   size_t M, uint32_t *keys,
-                                                   uint32_t *counts) {
-  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
-  uint32_t t[3] = {0, 0, 0};
-  for (int d = 0; d < g.ndim; ++d) {
-    double fr;
-    int i0 = fold_axis(g, d, load_coord(coords, j, g.ndim, d, g.coord_f32), fr);
-    t[d] = (uint32_t)((i0 + g.nsafe) >> g.log2tile);
+// Tile key per point (nufft.h:1364-1381 role); one 4096-point block per
+// workgroup, so that the first sort pass's per-block digit histogram comes
+// out of the same sweep.
+__global__ __launch_bounds__(RT) void k_fold_keys(Geo g, const void *coords,
+                                                  size_t M, uint32_t *keys,
+                                                  uint32_t *hist0,
+                                                  int nblocks) {
+  __shared__ uint32_t h[RB];
+  const int tid = threadIdx.x;
+  if (hist0) h[tid] = 0;
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * RTILE;
+#pragma unroll 4
+  for (int i = 0; i < RITEMS; ++i) {
+    const size_t j = base + (size_t)i * RT + tid;
+    if (j >= M) break;
+    uint32_t t[3] = {0, 0, 0};
+    for (int d = 0; d < g.ndim; ++d) {
+      double fr;
+      int i0 = fold_axis(g, d, load_coord(coords, j, g.ndim, d, g.coord_f32),
+                         fr);
+      t[d] = (uint32_t)((i0 + g.nsafe) >> g.log2tile);
+    }
+    uint32_t key;
+    if (g.ndim == 1)
+      key = t[0];
+    else if (g.ndim == 2)
+      key = t[0] * (uint32_t)g.ntiles[1] + t[1];
+    else
+      key = (t[0] * (uint32_t)g.ntiles[1] + t[1]) * (uint32_t)g.ntiles[2] +
+            t[2];
+    keys[j] = key;
+    if (hist0) atomicAdd(&h[key & 0xFFu], 1u);
   }
-  uint32_t key;
-  if (g.ndim == 1)
-    key = t[0];
-  else if (g.ndim == 2)
-    key = t[0] * (uint32_t)g.ntiles[1] + t[1];
-  else
-    key = (t[0] * (uint32_t)g.ntiles[1] + t[1]) * (uint32_t)g.ntiles[2] + t[2];
-  keys[j] = key;
-  if (counts) atomicAdd(&counts[key], 1u);
+  if (hist0) {
+    __syncthreads();
+    hist0[(size_t)tid * nblocks + blockIdx.x] = h[tid];
+  }
 }
 
 __global__ __launch_bounds__(RT) void k_radix_hist(const uint32_t *keys,
@@ -76,64 +95,6 @@ __device__ __forceinline__ unsigned long long digit_peers(uint32_t d,
   return valid ? peers : 0ull;
 }
 
-// Stable scatter of one 4096-key block.  Wave w owns keys
-// [w*1024, (w+1)*1024) of the block: (1) each wave counts its digits,
-// (2) one prefix over the four waves gives every wave its output base per
-// digit, (3) each wave ranks its keys in order (ballot peers + a running
-// per-digit counter that only its own lanes touch), with no further
-// workgroup barrier.
-__global__ __launch_bounds__(RT) void k_radix_scatter(
-    const uint32_t *keys_in, const uint32_t *idx_in, size_t M, int shift,
-    const uint32_t *offs, int nblocks, uint32_t *keys_out,
-    uint32_t *idx_out) {
-  constexpr int NW = RT / 64, WCH = RTILE / NW, WR = WCH / 64;
-  __shared__ uint32_t cnt[NW][RB];
-  const int tid = threadIdx.x, wave = tid >> 6;
-  const unsigned lane = lane_id();
-#pragma unroll
-  for (int w = 0; w < NW; ++w) cnt[w][tid] = 0;
-  __syncthreads();
-  const size_t c0 = (size_t)blockIdx.x * RTILE + (size_t)wave * WCH;
-  const unsigned long long below_mask =
-      lane ? (~0ull >> (64 - lane)) : 0ull;
-  for (int r = 0; r < WR; ++r) {
-    const size_t j = c0 + (size_t)r * 64 + lane;
-    const bool valid = j < M;
-    const uint32_t d = valid ? (keys_in[j] >> shift) & 0xFFu : 0u;
-    const unsigned long long peers = digit_peers(d, valid);
-    if (valid && (peers & below_mask) == 0)
-      cnt[wave][d] += (uint32_t)__popcll(peers);
-  }
-  __syncthreads();
-  {
-    uint32_t run = offs[(size_t)tid * nblocks + blockIdx.x];
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      const uint32_t c = cnt[w][tid];
-      cnt[w][tid] = run;
-      run += c;
-    }
-  }
-  __syncthreads();
-  for (int r = 0; r < WR; ++r) {
-    const size_t j = c0 + (size_t)r * 64 + lane;
-    const bool valid = j < M;
-    const uint32_t k = valid ? keys_in[j] : 0u;
-    const uint32_t d = (k >> shift) & 0xFFu;
-    const unsigned long long peers = digit_peers(d, valid);
-    const unsigned long long below = peers & below_mask;
-    if (valid) {
-      const uint32_t pos = cnt[wave][d] + (uint32_t)__popcll(below);
-      if (keys_out) keys_out[pos] = k;
-      idx_out[pos] = idx_in ? idx_in[j] : (uint32_t)j;
-    }
-    // every lane has read its base before the group leader advances it
-    __builtin_amdgcn_wave_barrier();
-    if (valid && below == 0) cnt[wave][d] += (uint32_t)__popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-  }
-}
-
 // ------------------------------------------------------------------ scan
 constexpr int ST = 256, SITEMS = 8, STILE = ST * SITEMS;
 
@@ -153,6 +114,107 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh,
   total = sh[ST - 1];
   __syncthreads();
   return incl - v;
+}
+
+// Stable scatter of one 4096-key block, staged through LDS so that the
+// global writes go out in digit runs (consecutive lanes -> consecutive
+// addresses):
+//  (1) wave w ranks its 1024 keys [w*1024, (w+1)*1024) in order, 64 at a
+//      time: ballot peers give the rank inside the 64, a per-wave per-digit
+//      counter in LDS (touched only by the group leader) the rank before;
+//  (2) one prefix over (digit, wave) gives each key its position in the
+//      block's locally sorted order, and each digit its global base;
+//  (3) keys, then indices, are written to LDS in local order and read back
+//      by position, so lane t of round i stores element i*256+t of the
+//      sorted block at global base[digit] + (position - local start).
+__global__ __launch_bounds__(RT) void k_radix_scatter(
+    const uint32_t *keys_in, const uint32_t *idx_in, size_t M, int shift,
+    const uint32_t *offs, int nblocks, uint32_t *keys_out,
+    uint32_t *idx_out) {
+  constexpr int NW = RT / 64;
+  __shared__ uint32_t cnt[NW][RB];
+  __shared__ uint32_t gbase[RB];
+  __shared__ uint32_t scan_sh[RB];
+  __shared__ uint32_t stage[RTILE];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const unsigned lane = lane_id();
+#pragma unroll
+  for (int w = 0; w < NW; ++w) cnt[w][tid] = 0;
+  const size_t blk0 = (size_t)blockIdx.x * RTILE;
+  const size_t c0 = blk0 + (size_t)wave * (RTILE / NW);
+  uint32_t key[RITEMS], rank[RITEMS];
+#pragma unroll
+  for (int r = 0; r < RITEMS; ++r) {
+    const size_t j = c0 + (size_t)r * 64 + lane;
+    key[r] = j < M ? keys_in[j] : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  const unsigned long long below_mask =
+      lane ? (~0ull >> (64 - lane)) : 0ull;
+#pragma unroll
+  for (int r = 0; r < RITEMS; ++r) {
+    const size_t j = c0 + (size_t)r * 64 + lane;
+    const bool valid = j < M;
+    const uint32_t d = (key[r] >> shift) & 0xFFu;
+    const unsigned long long peers = digit_peers(d, valid);
+    const unsigned long long below = peers & below_mask;
+    const uint32_t old = cnt[wave][d];
+    // every lane has read the counter before the group leader advances it
+    __builtin_amdgcn_wave_barrier();
+    if (valid && below == 0) cnt[wave][d] = old + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    rank[r] = old + (uint32_t)__popcll(below);
+  }
+  __syncthreads();
+  {
+    // thread tid owns digit tid: wave prefix, block-local start, global base
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t c = cnt[w][tid];
+      cnt[w][tid] = run;
+      run += c;
+    }
+    uint32_t tot;
+    const uint32_t start = block_exclusive_scan(run, scan_sh, tot);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) cnt[w][tid] += start;
+    gbase[tid] = offs[(size_t)tid * nblocks + blockIdx.x] - start;
+  }
+  __syncthreads();
+  const int nvalid = (int)min((size_t)RTILE, M - blk0);
+  uint32_t lpos[RITEMS];
+#pragma unroll
+  for (int r = 0; r < RITEMS; ++r) {
+    const size_t j = c0 + (size_t)r * 64 + lane;
+    const uint32_t d = (key[r] >> shift) & 0xFFu;
+    lpos[r] = cnt[wave][d] + rank[r];
+    if (j < M) stage[lpos[r]] = key[r];
+  }
+  __syncthreads();
+  uint32_t gpos[RITEMS];
+#pragma unroll
+  for (int i = 0; i < RITEMS; ++i) {
+    const int q = i * RT + tid;
+    gpos[i] = 0;
+    if (q < nvalid) {
+      const uint32_t k = stage[q];
+      gpos[i] = gbase[(k >> shift) & 0xFFu] + (uint32_t)q;
+      if (keys_out) keys_out[gpos[i]] = k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RITEMS; ++r) {
+    const size_t j = c0 + (size_t)r * 64 + lane;
+    if (j < M) stage[lpos[r]] = idx_in ? idx_in[j] : (uint32_t)j;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < RITEMS; ++i) {
+    const int q = i * RT + tid;
+    if (q < nvalid) idx_out[gpos[i]] = stage[q];
+  }
 }
 
 __global__ __launch_bounds__(ST) void k_scan_blocks(const uint32_t *in,
@@ -176,23 +238,26 @@ __global__ __launch_bounds__(ST) void k_scan_blocks(const uint32_t *in,
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(ST) void k_scan_top(uint32_t *bsum, size_t nb,
-                                                 uint32_t *total) {
+// one workgroup scans the whole array (in may equal out); used for the
+// block sums of the multi-block scan and for short arrays in one launch
+__global__ __launch_bounds__(ST) void k_scan_single(const uint32_t *in,
+                                                    uint32_t *out, size_t n,
+                                                    uint32_t *total) {
   __shared__ uint32_t sh[ST];
   uint32_t carry = 0;
-  for (size_t c = 0; c < nb; c += STILE) {
+  for (size_t c = 0; c < n; c += STILE) {
     size_t base = c + (size_t)threadIdx.x * SITEMS;
     uint32_t v[SITEMS], s = 0;
 #pragma unroll
     for (int i = 0; i < SITEMS; ++i) {
-      v[i] = (base + i < nb) ? bsum[base + i] : 0u;
+      v[i] = (base + i < n) ? in[base + i] : 0u;
       s += v[i];
     }
     uint32_t tot;
     uint32_t ex = block_exclusive_scan(s, sh, tot) + carry;
 #pragma unroll
     for (int i = 0; i < SITEMS; ++i) {
-      if (base + i < nb) bsum[base + i] = ex;
+      if (base + i < n) out[base + i] = ex;
       ex += v[i];
     }
     carry += tot;
@@ -221,16 +286,16 @@ __global__ void k_tile_bounds(const uint32_t *skeys, size_t M,
   if (p == M - 1 || skeys[p + 1] != k) tend[k] = (uint32_t)(p + 1);
 }
 
-__global__ void k_counts(const uint32_t *tstart, const uint32_t *tend,
-                         size_t ntiles, uint32_t *counts) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < ntiles) counts[t] = tend[t] - tstart[t];
-}
-
 // ------------------------------------------------------------ subproblems
-__global__ void k_nsub(const uint32_t *counts, size_t ntiles, uint32_t *nsub) {
+// points per tile and subproblems per tile (runs of at most MAXSUB points)
+__global__ void k_counts_nsub(const uint32_t *tstart, const uint32_t *tend,
+                              size_t ntiles, uint32_t *counts,
+                              uint32_t *nsub) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < ntiles) nsub[t] = (counts[t] + MAXSUB - 1) / MAXSUB;
+  if (t >= ntiles) return;
+  const uint32_t c = tend[t] - tstart[t];
+  counts[t] = c;
+  nsub[t] = (c + MAXSUB - 1) / MAXSUB;
 }
 
 __global__ void k_fill_subs(const uint32_t *counts, const uint32_t *tstart,
@@ -258,6 +323,13 @@ __global__ void k_gather_coords(const T *c, int ndim, const uint32_t *idx,
   for (int d = 0; d < ndim; ++d) out[j * ndim + d] = c[s * ndim + d];
 }
 
+template <typename E>
+__global__ void k_gather_elems(const E *src, const uint32_t *idx, size_t n,
+                               E *dst) {
+  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) dst[j] = src[idx[j]];
+}
+
 __global__ void k_zero16(uint4 *p, size_t n16) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -272,10 +344,11 @@ inline unsigned nblk(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t);
 }  // namespace
 
 void launch_fold_keys(const Geo &g, const void *coords, size_t M,
-                      uint32_t *keys, uint32_t *counts, hipStream_t s) {
+                      uint32_t *keys, uint32_t *hist0, hipStream_t s) {
   if (!M) return;
-  hipLaunchKernelGGL(k_fold_keys, dim3(nblk(M, 256)), dim3(256), 0, s, g,
-                     coords, M, keys, counts);
+  const size_t nblocks = (M + RTILE - 1) / RTILE;
+  hipLaunchKernelGGL(k_fold_keys, dim3((unsigned)nblocks), dim3(RT), 0, s, g,
+                     coords, M, keys, hist0, (int)nblocks);
   DMX_HIP(hipGetLastError());
 }
 
@@ -288,10 +361,17 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n,
                         uint32_t *total, void *work, hipStream_t s) {
   if (!n) return;
   size_t nb = (n + STILE - 1) / STILE;
+  if (nb <= 32) {
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(ST), 0, s, in, out, n,
+                       total);
+    DMX_HIP(hipGetLastError());
+    return;
+  }
   uint32_t *bsum = (uint32_t *)work;
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(ST), 0, s, in,
                      out, n, bsum);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(ST), 0, s, bsum, nb, total);
+  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(ST), 0, s,
+                     (const uint32_t *)bsum, bsum, nb, total);
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(ST), 0, s, out, n,
                      (const uint32_t *)bsum);
   DMX_HIP(hipGetLastError());
@@ -304,9 +384,13 @@ size_t radix_sort_workspace(size_t M) {
          scan_workspace(hist) + 256;
 }
 
+uint32_t *radix_sort_hist0(void *work, size_t M) {
+  return (uint32_t *)work + 4 * M;
+}
+
 void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
                       uint32_t *idx_out, uint32_t *keys_out, void *work,
-                      size_t work_bytes, hipStream_t s) {
+                      size_t work_bytes, hipStream_t s, bool hist0_ready) {
   if (!M) return;
   DMX_ASSERT(work_bytes >= radix_sort_workspace(M), "sort workspace too small");
   size_t nblocks = (M + RTILE - 1) / RTILE;
@@ -323,8 +407,9 @@ void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
     bool last = p == npass - 1;
     uint32_t *kout = last ? keys_out : kbuf[p & 1];
     uint32_t *iout = last ? idx_out : ibuf[p & 1];
-    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nblocks), dim3(RT), 0, s,
-                       kin, M, 8 * p, hist, (int)nblocks);
+    if (p > 0 || !hist0_ready)
+      hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nblocks), dim3(RT), 0,
+                         s, kin, M, 8 * p, hist, (int)nblocks);
     exclusive_scan_u32(hist, offs, hist_n, nullptr, sw, s);
     hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nblocks), dim3(RT), 0,
                        s, kin, iin, M, 8 * p, (const uint32_t *)offs,
@@ -335,30 +420,26 @@ void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
   }
 }
 
-void launch_build_subs(const uint32_t *counts, const uint32_t *tile_start,
-                       size_t ntiles, uint32_t *nsub, uint32_t *sub_start,
-                       Sub *subs, uint32_t *nsub_total, void *scanwork,
-                       hipStream_t s) {
-  hipLaunchKernelGGL(k_nsub, dim3(nblk(ntiles, 256)), dim3(256), 0, s, counts,
-                     ntiles, nsub);
-  exclusive_scan_u32(nsub, sub_start, ntiles, nsub_total, scanwork, s);
-  hipLaunchKernelGGL(k_fill_subs, dim3(nblk(ntiles, 256)), dim3(256), 0, s,
-                     counts, tile_start, (const uint32_t *)nsub,
-                     (const uint32_t *)sub_start, ntiles, subs);
-  DMX_HIP(hipGetLastError());
-}
-
-void launch_tile_bounds(const uint32_t *skeys, size_t M, size_t ntiles,
-                        uint32_t *tstart, uint32_t *tend, uint32_t *counts,
-                        hipStream_t s) {
-  launch_zero(tstart, ntiles * 4, s);
-  launch_zero(tend, ntiles * 4, s);
+void launch_tile_index(const uint32_t *skeys, size_t M, size_t ntiles,
+                       uint32_t *counts, uint32_t *tstart, uint32_t *tend,
+                       uint32_t *nsub, uint32_t *sub_start, Sub *subs,
+                       uint32_t *nsub_total, void *scanwork, hipStream_t s) {
+  if (tend == tstart + ntiles) {
+    launch_zero(tstart, ntiles * 8, s);
+  } else {
+    launch_zero(tstart, ntiles * 4, s);
+    launch_zero(tend, ntiles * 4, s);
+  }
   if (M)
     hipLaunchKernelGGL(k_tile_bounds, dim3(nblk(M, 256)), dim3(256), 0, s,
                        skeys, M, tstart, tend);
-  hipLaunchKernelGGL(k_counts, dim3(nblk(ntiles, 256)), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_counts_nsub, dim3(nblk(ntiles, 256)), dim3(256), 0, s,
                      (const uint32_t *)tstart, (const uint32_t *)tend, ntiles,
-                     counts);
+                     counts, nsub);
+  exclusive_scan_u32(nsub, sub_start, ntiles, nsub_total, scanwork, s);
+  hipLaunchKernelGGL(k_fill_subs, dim3(nblk(ntiles, 256)), dim3(256), 0, s,
+                     counts, tstart, (const uint32_t *)nsub,
+                     (const uint32_t *)sub_start, ntiles, subs);
   DMX_HIP(hipGetLastError());
 }
 
@@ -374,6 +455,19 @@ void launch_gather_coords(const void *coords, int coord_f32, int ndim,
     hipLaunchKernelGGL(k_gather_coords<double>, dim3(nblk(M, 256)), dim3(256),
                        0, s, (const double *)coords, ndim, idx, M,
                        (double *)out);
+  DMX_HIP(hipGetLastError());
+}
+
+void launch_gather_elems(const void *src, int elem_bytes, const uint32_t *idx,
+                         size_t n, void *dst, hipStream_t s) {
+  if (!n) return;
+  DMX_ASSERT(elem_bytes == 8 || elem_bytes == 16, "gather element size");
+  if (elem_bytes == 8)
+    hipLaunchKernelGGL(k_gather_elems<uint2>, dim3(nblk(n, 256)), dim3(256),
+                       0, s, (const uint2 *)src, idx, n, (uint2 *)dst);
+  else
+    hipLaunchKernelGGL(k_gather_elems<uint4>, dim3(nblk(n, 256)), dim3(256),
+                       0, s, (const uint4 *)src, idx, n, (uint4 *)dst);
   DMX_HIP(hipGetLastError());
 }
 

@@ -8,12 +8,15 @@ constexpr unsigned SPREAD_MAX_BLOCKS = 8192;
 
 template <typename T, int NDIM, int W>
 void go(const SpreadArgs &a, hipStream_t s) {
+  static_assert(sizeof(SpreadStash<T, NDIM>) <=
+                    spread_lds_bytes(0, NDIM, sizeof(T) == 8),
+                "LDS staging size");
   unsigned nb = (unsigned)std::min<size_t>(a.nsub_max, SPREAD_MAX_BLOCKS);
   if (nb == 0) return;
   if (a.use_lds) {
     size_t cells = a.lds_side;
     for (int d = 1; d < NDIM; ++d) cells *= a.lds_side;
-    size_t sh = 2 * cells * sizeof(Acc);
+    size_t sh = spread_lds_bytes(cells, NDIM, sizeof(T) == 8);
     hipLaunchKernelGGL((k_spread<T, NDIM, W, true>), dim3(nb), dim3(256), sh,
                        s, a);
   } else {

@@ -65,7 +65,8 @@ __device__ __forceinline__ void tile_origin(const Geo &g, uint32_t tile,
 template <typename T, int NDIM, int W>
 __device__ __forceinline__ void point_setup(const SpreadArgs &a, uint32_t p,
                                             uint32_t &j, int *i0, T (*w)[W]) {
-  j = a.idx[p];
+  // the original index is only needed for data still in input order
+  j = (a.coords_sorted && a.values_sorted) ? p : a.idx[p];
   const size_t cj = a.coords_sorted ? (size_t)p : (size_t)j;
   const T *cf = (const T *)a.coef;
 #pragma unroll
@@ -94,6 +95,32 @@ __device__ __forceinline__ void spread_row3(Acc *sre, Acc *sim, int row,
   }
 }
 
+// Points of a subproblem are spread in chunks of SCHUNK.  Phase 1 (thread
+// t <-> point t of the chunk, coalesced): fold, kernel argument, offset in
+// the region and value go to an LDS stash, and the points are bucketed by
+// (first cell mod 32), the LDS bank pair of an FP64 accumulator.  Phase 2:
+// lane l of every wave takes points from the l-th 1/64 of the bucketed
+// list, so the 64 lanes of one atomic start in (nearly) distinct bank pairs;
+// every tap adds the same offset to all lanes, so all W^d atomics of a point
+// stay conflict-free.  Random lane-to-cell assignment costs several-way bank
+// conflicts on every LDS atomic, the spreader's bottleneck.
+template <typename T, int NDIM>
+struct SpreadStash {
+  typename cplx2<T>::type v[SCHUNK];
+  T x[NDIM][SCHUNK];
+  uint32_t o[SCHUNK];  // region offsets, 10 bits per axis
+  uint32_t cnt[32], off[32];
+  uint16_t list[SCHUNK];
+};
+
+template <typename T, int NDIM>
+__device__ __forceinline__ int region_cell(uint32_t o, int S) {
+  const int o0 = o & 1023, o1 = (o >> 10) & 1023, o2 = (o >> 20) & 1023;
+  if constexpr (NDIM == 1) return o0;
+  else if constexpr (NDIM == 2) return o0 * S + o1;
+  else return (o0 * S + o1) * S + o2;
+}
+
 template <typename T, int NDIM, int W, bool LDS>
 __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   extern __shared__ unsigned char smem_raw[];
@@ -105,74 +132,145 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   if (NDIM > 2) ncell *= S;
   Acc *sre = (Acc *)smem_raw;
   Acc *sim = sre + ncell;
+  SpreadStash<T, NDIM> &st =
+      *(SpreadStash<T, NDIM> *)(smem_raw + 2 * (size_t)ncell * sizeof(Acc));
   T *grid = (T *)a.grid;
+  const T *cf = (const T *)a.coef;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (uint32_t sidx = blockIdx.x; sidx < total; sidx += gridDim.x) {
     const Sub sb = a.subs[sidx];
     int b0[3];
     tile_origin(a.g, sb.tile, b0);
-    if (LDS) {
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    if constexpr (LDS) {
+      for (int c = tid; c < ncell; c += blockDim.x) {
         sre[c] = Acc(0);
         sim[c] = Acc(0);
       }
-      __syncthreads();
-    }
-    for (uint32_t p = sb.start + threadIdx.x; p < sb.end; p += blockDim.x) {
-      uint32_t j;
-      int i0[3];
-      T w[NDIM][W];
-      point_setup<T, NDIM, W>(a, p, j, i0, w);
-      T vr, vi;
-      load_value<T>(a.values, j, a.val_f64, vr, vi);
-      if (LDS) {
-        int o[3];
+      for (uint32_t c0 = sb.start; c0 < sb.end; c0 += SCHUNK) {
+        const int nc = (int)min((uint32_t)SCHUNK, sb.end - c0);
+        if (tid < 32) st.cnt[tid] = 0;
+        // the previous chunk's phase 2 is done with the stash (and the
+        // accumulators are zero before the first chunk)
+        __syncthreads();
+        int bkt = 0;
+        uint32_t rnk = 0;
+        if (tid < nc) {
+          const uint32_t p = c0 + tid;
+          const uint32_t j = a.idx[p];
+          const size_t cj = a.coords_sorted ? (size_t)p : (size_t)j;
+          uint32_t o = 0;
 #pragma unroll
-        for (int d = 0; d < NDIM; ++d) {
-          o[d] = i0[d] - b0[d];
-          o[d] = o[d] < 0 ? 0 : (o[d] >= tile ? tile - 1 : o[d]);
+          for (int d = 0; d < NDIM; ++d) {
+            double frac;
+            const int i0 = fold_axis(
+                a.g, d, load_coord(a.coords, cj, NDIM, d, a.g.coord_f32),
+                frac);
+            st.x[d][tid] = (T)((double)(W - 1) - 2.0 * frac);
+            int od = i0 - b0[d];
+            od = od < 0 ? 0 : (od >= tile ? tile - 1 : od);
+            o |= (uint32_t)od << (10 * d);
+          }
+          st.o[tid] = o;
+          T vr, vi;
+          load_value<T>(a.values, a.values_sorted ? (size_t)p : (size_t)j,
+                        a.val_f64, vr, vi);
+          st.v[tid].x = vr;
+          st.v[tid].y = vi;
+          bkt = region_cell<T, NDIM>(o, S) & 31;
+          rnk = atomicAdd(&st.cnt[bkt], 1u);
         }
-        if constexpr (NDIM == 1) {
+        __syncthreads();
+        if (tid < 32) {
+          uint32_t run = 0;
+          for (int k = 0; k < tid; ++k) run += st.cnt[k];
+          st.off[tid] = run;
+        }
+        __syncthreads();
+        if (tid < nc) st.list[st.off[bkt] + rnk] = (uint16_t)tid;
+        __syncthreads();
+        const int lo = (lane * nc) >> 6, hi = ((lane + 1) * nc) >> 6;
+        for (int e = lo + wave; e < hi; e += 4) {
+          const int i = st.list[e];
+          const uint32_t o = st.o[i];
+          const T vr = st.v[i].x, vi = st.v[i].y;
+          T w[NDIM][W];
 #pragma unroll
-          for (int c0 = 0; c0 < W; ++c0) {
-            atomicAdd(&sre[o[0] + c0], (Acc)(vr * w[0][c0]));
-            atomicAdd(&sim[o[0] + c0], (Acc)(vi * w[0][c0]));
-          }
-        } else if constexpr (NDIM == 2) {
+          for (int d = 0; d < NDIM; ++d)
+            kernel_weights<T, W>(st.x[d][i], cf, w[d]);
+          const int cell0 = region_cell<T, NDIM>(o, S);
+          if constexpr (NDIM == 1) {
 #pragma unroll
-          for (int c0 = 0; c0 < W; ++c0) {
-            const T fr = vr * w[0][c0], fi = vi * w[0][c0];
-            const int row = (o[0] + c0) * S + o[1];
-#pragma unroll
-            for (int c1 = 0; c1 < W; ++c1) {
-              atomicAdd(&sre[row + c1], (Acc)(fr * w[1][c1]));
-              atomicAdd(&sim[row + c1], (Acc)(fi * w[1][c1]));
+            for (int q0 = 0; q0 < W; ++q0) {
+              atomicAdd(&sre[cell0 + q0], (Acc)(vr * w[0][q0]));
+              atomicAdd(&sim[cell0 + q0], (Acc)(vi * w[0][q0]));
             }
-          }
-        } else if constexpr (W <= 8) {
+          } else if constexpr (NDIM == 2) {
 #pragma unroll
-          for (int c0 = 0; c0 < W; ++c0)
+            for (int q0 = 0; q0 < W; ++q0) {
+              const T fr = vr * w[0][q0], fi = vi * w[0][q0];
+              const int row = cell0 + q0 * S;
 #pragma unroll
-            for (int c1 = 0; c1 < W; ++c1) {
-              const T f01 = w[0][c0] * w[1][c1];
-              spread_row3<T, W>(sre, sim,
-                                ((o[0] + c0) * S + o[1] + c1) * S + o[2],
-                                vr * f01, vi * f01, w[2]);
+              for (int q1 = 0; q1 < W; ++q1) {
+                atomicAdd(&sre[row + q1], (Acc)(fr * w[1][q1]));
+                atomicAdd(&sim[row + q1], (Acc)(fi * w[1][q1]));
+              }
             }
-        } else {
+          } else if constexpr (W <= 8) {
+#pragma unroll
+            for (int q0 = 0; q0 < W; ++q0)
+#pragma unroll
+              for (int q1 = 0; q1 < W; ++q1) {
+                const T f01 = w[0][q0] * w[1][q1];
+                spread_row3<T, W>(sre, sim, cell0 + (q0 * S + q1) * S,
+                                  vr * f01, vi * f01, w[2]);
+              }
+          } else {
 #pragma unroll 1
-          for (int c0 = 0; c0 < W; ++c0)
+            for (int q0 = 0; q0 < W; ++q0)
 #pragma unroll
-            for (int c1 = 0; c1 < W; ++c1) {
-              const T f01 = w[0][c0] * w[1][c1];
-              spread_row3<T, W>(sre, sim,
-                                ((o[0] + c0) * S + o[1] + c1) * S + o[2],
-                                vr * f01, vi * f01, w[2]);
-            }
+              for (int q1 = 0; q1 < W; ++q1) {
+                const T f01 = w[0][q0] * w[1][q1];
+                spread_row3<T, W>(sre, sim, cell0 + (q0 * S + q1) * S,
+                                  vr * f01, vi * f01, w[2]);
+              }
+          }
         }
-      } else {
-        // direct HBM accumulation with periodic wrap
-        const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
-                  n2 = NDIM > 2 ? a.g.nover[2] : 1;
+      }
+      __syncthreads();
+      const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
+                n2 = NDIM > 2 ? a.g.nover[2] : 1;
+      for (int c = tid; c < ncell; c += blockDim.x) {
+        const T re = (T)sre[c], im = (T)sim[c];
+        if (re == T(0) && im == T(0)) continue;
+        size_t gi;
+        if constexpr (NDIM == 1) {
+          gi = (size_t)wrapi(b0[0] + c, n0);
+        } else if constexpr (NDIM == 2) {
+          const int cu = c / S, cv = c - cu * S;
+          gi = (size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1);
+        } else {
+          const int cu = c / (S * S), r = c - cu * S * S, cv = r / S,
+                    cw = r - cv * S;
+          gi = ((size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1)) *
+                   n2 +
+               wrapi(b0[2] + cw, n2);
+        }
+        atomicAdd(&grid[2 * gi], re);
+        atomicAdd(&grid[2 * gi + 1], im);
+      }
+      __syncthreads();
+    } else {
+      // direct HBM accumulation with periodic wrap (regions beyond LDS)
+      const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
+                n2 = NDIM > 2 ? a.g.nover[2] : 1;
+      for (uint32_t p = sb.start + tid; p < sb.end; p += blockDim.x) {
+        uint32_t j;
+        int i0[3];
+        T w[NDIM][W];
+        point_setup<T, NDIM, W>(a, p, j, i0, w);
+        T vr, vi;
+        load_value<T>(a.values, a.values_sorted ? (size_t)p : (size_t)j,
+                      a.val_f64, vr, vi);
 #pragma unroll 1
         for (int c0 = 0; c0 < W; ++c0) {
           const int g0 = wrapi(i0[0] + c0, n0);
@@ -201,31 +299,6 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
           }
         }
       }
-    }
-    if (LDS) {
-      __syncthreads();
-      const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
-                n2 = NDIM > 2 ? a.g.nover[2] : 1;
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const T re = (T)sre[c], im = (T)sim[c];
-        if (re == T(0) && im == T(0)) continue;
-        size_t gi;
-        if constexpr (NDIM == 1) {
-          gi = (size_t)wrapi(b0[0] + c, n0);
-        } else if constexpr (NDIM == 2) {
-          const int cu = c / S, cv = c - cu * S;
-          gi = (size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1);
-        } else {
-          const int cu = c / (S * S), r = c - cu * S * S, cv = r / S,
-                    cw = r - cv * S;
-          gi = ((size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1)) *
-                   n2 +
-               wrapi(b0[2] + cw, n2);
-        }
-        atomicAdd(&grid[2 * gi], re);
-        atomicAdd(&grid[2 * gi + 1], im);
-      }
-      __syncthreads();
     }
   }
 }
