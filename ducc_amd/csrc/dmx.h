@@ -45,16 +45,20 @@ struct Sub {
 };
 
 constexpr int MAXSUB = 1024;
-// points per LDS staging round of the spreader
-constexpr int SCHUNK = 256;
+// points per LDS staging round of the spreader: a whole subproblem in 1-D
+// and 2-D (FP32), half of one in FP64, 256 in 3-D where the region itself
+// needs most of the LDS
+__host__ __device__ constexpr int spread_chunk(int ndim, bool fp64) {
+  return ndim == 3 ? 256 : (fp64 ? MAXSUB / 2 : MAXSUB);
+}
 // LDS of one spreading workgroup: re/im planes of FP64 accumulators over the
-// region of `cells` cells plus the staging of SCHUNK points (values, kernel
-// arguments, offsets, bucket counts and list)
+// region of `cells` cells plus the staging of one chunk of points (values,
+// kernel arguments, offsets, class counts)
 inline constexpr size_t spread_lds_bytes(size_t cells, int ndim, bool fp64) {
   return 2 * cells * sizeof(double) +
-         (((size_t)SCHUNK * (2 * (fp64 ? 8 : 4) + ndim * (fp64 ? 8 : 4) + 4 +
-                             2) +
-           64 * 4 + 15) &
+         (((size_t)spread_chunk(ndim, fp64) *
+               (2 * (fp64 ? 8 : 4) + ndim * (fp64 ? 8 : 4) + 4) +
+           32 * 4 + 15) &
           ~(size_t)15);
 }
 

@@ -11,6 +11,9 @@ void go(const SpreadArgs &a, hipStream_t s) {
   static_assert(sizeof(SpreadStash<T, NDIM>) <=
                     spread_lds_bytes(0, NDIM, sizeof(T) == 8),
                 "LDS staging size");
+  static_assert(SpreadStash<T, NDIM>::SC % 256 == 0 &&
+                    SpreadStash<T, NDIM>::SC <= 65536,
+                "staging chunk: whole rounds of 256 threads, 16-bit list");
   unsigned nb = (unsigned)std::min<size_t>(a.nsub_max, SPREAD_MAX_BLOCKS);
   if (nb == 0) return;
   if (a.use_lds) {

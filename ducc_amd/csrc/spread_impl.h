@@ -95,22 +95,29 @@ __device__ __forceinline__ void spread_row3(Acc *sre, Acc *sim, int row,
   }
 }
 
-// Points of a subproblem are spread in chunks of SCHUNK.  Phase 1 (thread
-// t <-> point t of the chunk, coalesced): fold, kernel argument, offset in
-// the region and value go to an LDS stash, and the points are bucketed by
-// (first cell mod 32), the LDS bank pair of an FP64 accumulator.  Phase 2:
-// lane l of every wave takes points from the l-th 1/64 of the bucketed
-// list, so the 64 lanes of one atomic start in (nearly) distinct bank pairs;
-// every tap adds the same offset to all lanes, so all W^d atomics of a point
-// stay conflict-free.  Random lane-to-cell assignment costs several-way bank
-// conflicts on every LDS atomic, the spreader's bottleneck.
+// Points of a subproblem are spread in chunks (spread_chunk: the whole
+// subproblem in 1-D/2-D FP32).  ds_add_f64 resolves bank conflicts per
+// quarter-wave: 16 lanes whose cells are distinct mod 16 run at ~7 lane-
+// atomics/CU/clock, random cells at ~2.5 (measured on gfx950), and these
+// atomics bound the spreader.  So:
+// phase 1 (thread t <-> points t, t+256, ... of the chunk, all loads in
+//   flight first): fold, kernel argument, region offset and value; a rank
+//   inside the point's class (first cell mod 16); after a prefix over the 16
+//   classes each point has a position in the class-sorted order, which is
+//   split into 64 runs of K = ceil(n/64) positions.  The point is written to
+//   stash slot j*64 + r (run r, place j in the run);
+// phase 2: lane l takes run r(l) = 4*(l mod 16) + l/16, so the 16 lanes of
+//   a quarter-wave hold 16 consecutive runs, i.e. (nearly) all 16 classes,
+//   and all W^d taps of the point, which shift every lane by the same
+//   offset, stay conflict-free; stash slots j*64 + r are consecutive across
+//   the wave, so the stash reads are conflict-free as well.
 template <typename T, int NDIM>
 struct SpreadStash {
-  typename cplx2<T>::type v[SCHUNK];
-  T x[NDIM][SCHUNK];
-  uint32_t o[SCHUNK];  // region offsets, 10 bits per axis
-  uint32_t cnt[32], off[32];
-  uint16_t list[SCHUNK];
+  static constexpr int SC = spread_chunk(NDIM, sizeof(T) == 8);
+  typename cplx2<T>::type v[SC];
+  T x[NDIM][SC];
+  uint32_t o[SC];  // region offsets, 10 bits per axis
+  uint32_t cnt[16], off[16];
 };
 
 template <typename T, int NDIM>
@@ -146,51 +153,89 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
         sre[c] = Acc(0);
         sim[c] = Acc(0);
       }
-      for (uint32_t c0 = sb.start; c0 < sb.end; c0 += SCHUNK) {
-        const int nc = (int)min((uint32_t)SCHUNK, sb.end - c0);
-        if (tid < 32) st.cnt[tid] = 0;
+      constexpr int SC = SpreadStash<T, NDIM>::SC, PT = SC / 256;
+      for (uint32_t c0 = sb.start; c0 < sb.end; c0 += SC) {
+        const int nc = (int)min((uint32_t)SC, sb.end - c0);
+        const int K = (nc + 63) >> 6;            // run length
+        const uint32_t kmul = (65536u + K - 1) / K;  // pos / K == pos*kmul >> 16
+        if (tid < 16) st.cnt[tid] = 0;
         // the previous chunk's phase 2 is done with the stash (and the
         // accumulators are zero before the first chunk)
         __syncthreads();
-        int bkt = 0;
-        uint32_t rnk = 0;
-        if (tid < nc) {
-          const uint32_t p = c0 + tid;
-          const uint32_t j = a.idx[p];
-          const size_t cj = a.coords_sorted ? (size_t)p : (size_t)j;
-          uint32_t o = 0;
+        // loads of all this thread's points first (memory-level parallelism)
+        uint32_t jj[PT];
+        double cc[PT][NDIM];
+        T vr_[PT], vi_[PT];
 #pragma unroll
-          for (int d = 0; d < NDIM; ++d) {
-            double frac;
-            const int i0 = fold_axis(
-                a.g, d, load_coord(a.coords, cj, NDIM, d, a.g.coord_f32),
-                frac);
-            st.x[d][tid] = (T)((double)(W - 1) - 2.0 * frac);
-            int od = i0 - b0[d];
-            od = od < 0 ? 0 : (od >= tile ? tile - 1 : od);
-            o |= (uint32_t)od << (10 * d);
+        for (int k = 0; k < PT; ++k) {
+          const int i = tid + 256 * k;
+          jj[k] = i < nc ? a.idx[c0 + i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < PT; ++k) {
+          const int i = tid + 256 * k;
+          if (i < nc) {
+            const uint32_t p = c0 + i;
+            const size_t cj = a.coords_sorted ? (size_t)p : (size_t)jj[k];
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d)
+              cc[k][d] = load_coord(a.coords, cj, NDIM, d, a.g.coord_f32);
+            load_value<T>(a.values,
+                          a.values_sorted ? (size_t)p : (size_t)jj[k],
+                          a.val_f64, vr_[k], vi_[k]);
           }
-          st.o[tid] = o;
-          T vr, vi;
-          load_value<T>(a.values, a.values_sorted ? (size_t)p : (size_t)j,
-                        a.val_f64, vr, vi);
-          st.v[tid].x = vr;
-          st.v[tid].y = vi;
-          bkt = region_cell<T, NDIM>(o, S) & 31;
-          rnk = atomicAdd(&st.cnt[bkt], 1u);
+        }
+        T xs[PT][NDIM];
+        uint32_t os[PT], rnk[PT];
+        int cls[PT];
+#pragma unroll
+        for (int k = 0; k < PT; ++k) {
+          const int i = tid + 256 * k;
+          os[k] = 0;
+          cls[k] = 0;
+          rnk[k] = 0;
+          if (i < nc) {
+            uint32_t o = 0;
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) {
+              double frac;
+              const int i0 = fold_axis(a.g, d, cc[k][d], frac);
+              xs[k][d] = (T)((double)(W - 1) - 2.0 * frac);
+              int od = i0 - b0[d];
+              od = od < 0 ? 0 : (od >= tile ? tile - 1 : od);
+              o |= (uint32_t)od << (10 * d);
+            }
+            os[k] = o;
+            cls[k] = region_cell<T, NDIM>(o, S) & 15;
+            rnk[k] = atomicAdd(&st.cnt[cls[k]], 1u);
+          }
         }
         __syncthreads();
-        if (tid < 32) {
+        if (tid < 16) {
           uint32_t run = 0;
           for (int k = 0; k < tid; ++k) run += st.cnt[k];
           st.off[tid] = run;
         }
         __syncthreads();
-        if (tid < nc) st.list[st.off[bkt] + rnk] = (uint16_t)tid;
+#pragma unroll
+        for (int k = 0; k < PT; ++k) {
+          const int i = tid + 256 * k;
+          if (i < nc) {
+            const uint32_t pos = st.off[cls[k]] + rnk[k];
+            const uint32_t r = (pos * kmul) >> 16, jr = pos - r * K;
+            const int slot = (int)(jr * 64 + r);
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) st.x[d][slot] = xs[k][d];
+            st.o[slot] = os[k];
+            st.v[slot].x = vr_[k];
+            st.v[slot].y = vi_[k];
+          }
+        }
         __syncthreads();
-        const int lo = (lane * nc) >> 6, hi = ((lane + 1) * nc) >> 6;
-        for (int e = lo + wave; e < hi; e += 4) {
-          const int i = st.list[e];
+        const int run = ((lane & 15) << 2) | (lane >> 4);
+        for (int jr = wave; jr < K; jr += 4) {
+          if (run * K + jr >= nc) continue;
+          const int i = jr * 64 + run;
           const uint32_t o = st.o[i];
           const T vr = st.v[i].x, vi = st.v[i].y;
           T w[NDIM][W];
