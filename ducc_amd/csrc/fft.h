@@ -10,9 +10,9 @@ namespace dmx {
 #define FFT_THREADS 256
 #define FFT_MAXEL32 8192  // 64 KiB of complex float per workgroup
 #define FFT_MAXEL64 4096  // 64 KiB of complex double per workgroup
-#define FFT_KM(T, R)                                                   \
-  (((sizeof(T) == 8 ? FFT_MAXEL64 : FFT_MAXEL32) + (R)*FFT_THREADS - 1) / \
-   ((R)*FFT_THREADS))
+// butterflies per thread and pass for workgroups of at most MAXEL elements
+#define FFT_KM_EL(MAXEL, R) (((MAXEL) + (R)*FFT_THREADS - 1) / ((R)*FFT_THREADS))
+#define FFT_KM(T, R) FFT_KM_EL(sizeof(T) == 8 ? FFT_MAXEL64 : FFT_MAXEL32, R)
 
 // Lines along one axis of a row-major array.  Line (u, v) starts at
 // U(u)*st_u + V(v)*st_v, element k at + k*st_ax, where the virtual indices

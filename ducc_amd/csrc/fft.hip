@@ -153,7 +153,10 @@ __device__ __forceinline__ int fdivq(int x, int d, float inv, int &r) {
   return q;
 }
 
-// One radix-R Stockham pass over Bl LDS-resident lines (stride LP).
+// One radix-R Stockham pass over Bl LDS-resident lines (stride LP).  The
+// inputs of all butterflies a thread owns are read and twiddled before the
+// barrier (tw: L twiddles, in LDS when the launch staged them), so after it
+// only the DFTs and the writes remain.
 template <typename T, int R, int KM>
 __device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
                                          int Bl, int Ns,
@@ -165,25 +168,15 @@ __device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
   const int twstep = L / (Ns * R);
   const float inv_nbf = 1.0f / (float)nbf, inv_ns = 1.0f / (float)Ns;
   C v[KM][R];
-  int boff[KM], jjk[KM];
+  int obase[KM];
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
     const int q = threadIdx.x + k * blockDim.x;
     if (q < total) {
       int jj;
       const int b = Bl == 1 ? (jj = q, 0) : fdivq(q, nbf, inv_nbf, jj);
-      boff[k] = b * LP;
-      jjk[k] = jj;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[k][r] = buf[boff[k] + jj + r * nbf];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < KM; ++k) {
-    const int q = threadIdx.x + k * blockDim.x;
-    if (q < total) {
-      const int jj = jjk[k];
+      for (int r = 0; r < R; ++r) v[k][r] = buf[b * LP + jj + r * nbf];
       int kk;
       const int jq = Ns == 1 ? (kk = 0, jj) : fdivq(jj, Ns, inv_ns, kk);
       if (Ns > 1) {
@@ -194,10 +187,17 @@ __device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
           v[k][r] = cmul(v[k][r], w);
         }
       }
-      dft<T, R>(v[k], sign);
-      const int base = jq * Ns * R + kk;
+      obase[k] = b * LP + jq * Ns * R + kk;
+    }
+  }
+  __syncthreads();
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[boff[k] + base + r * Ns] = v[k][r];
+  for (int k = 0; k < KM; ++k) {
+    const int q = threadIdx.x + k * blockDim.x;
+    if (q < total) {
+      dft<T, R>(v[k], sign);
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[obase[k] + r * Ns] = v[k][r];
     }
   }
   __syncthreads();
@@ -207,14 +207,19 @@ __device__ __forceinline__ int64_t map_range(int v, int n1, int a1, int a2) {
   return v < n1 ? (int64_t)(a1 + v) : (int64_t)(a2 + (v - n1));
 }
 
-template <typename T>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_lds(FftLaunch p) {
+// MAXEL: the most elements (lines x length) the launch puts in one
+// workgroup; it sizes the register staging of the passes, and the half-size
+// variant leaves room for more workgroups per CU
+template <typename T, int MAXEL>
+__global__ __launch_bounds__(FFT_THREADS) __attribute__((
+    amdgpu_waves_per_eu(sizeof(T) == 4 && MAXEL <= 4096 ? 5 : 1))) void
+k_fft_lds(FftLaunch p) {
   using C = typename V2<T>::t;
   extern __shared__ unsigned char smem_raw[];
   C *buf = (C *)smem_raw;
   C *data = (C *)p.data;
-  const C *tw = (const C *)p.tw;
   const int L = p.L, LP = p.LP, Bl = p.Bl;
+  const C *tw = (const C *)p.tw;
   const int u = blockIdx.y;
   const int v0 = blockIdx.x * Bl;
   const int64_t ubase = map_range(u, p.un1, p.ua1, p.ua2) * p.st_u;
@@ -242,12 +247,12 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lds(FftLaunch p) {
   for (int s = 0; s < p.npass; ++s) {
     const int R = p.radix[s];
     switch (R) {
-      case 4: lds_pass<T, 4, FFT_KM(T, 4)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      case 2: lds_pass<T, 2, FFT_KM(T, 2)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      case 3: lds_pass<T, 3, FFT_KM(T, 3)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      case 5: lds_pass<T, 5, FFT_KM(T, 5)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      case 7: lds_pass<T, 7, FFT_KM(T, 7)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      case 11: lds_pass<T, 11, FFT_KM(T, 11)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 4: lds_pass<T, 4, FFT_KM_EL(MAXEL, 4)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 2: lds_pass<T, 2, FFT_KM_EL(MAXEL, 2)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 3: lds_pass<T, 3, FFT_KM_EL(MAXEL, 3)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 5: lds_pass<T, 5, FFT_KM_EL(MAXEL, 5)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 7: lds_pass<T, 7, FFT_KM_EL(MAXEL, 7)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
+      case 11: lds_pass<T, 11, FFT_KM_EL(MAXEL, 11)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
       default: break;
     }
     Ns *= R;
@@ -268,14 +273,20 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lds(FftLaunch p) {
 // u*ia + e*ib of the source; output element k goes to row u*oa + k*ob of
 // the destination, optionally times w_L^{(k*u) mod L}.  Loads and stores
 // move Bl contiguous columns per row (coalesced).
-template <typename T>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(FftCols p) {
+template <typename T, int MAXEL>
+__global__ __launch_bounds__(FFT_THREADS) __attribute__((
+    amdgpu_waves_per_eu(sizeof(T) == 4 && MAXEL <= 4096 ? 5 : 1))) void
+k_fft_cols(FftCols p) {
   using C = typename V2<T>::t;
   extern __shared__ unsigned char smem_raw[];
   C *buf = (C *)smem_raw;
   const C *src = (const C *)p.src;
   C *dst = (C *)p.dst;
   const int Ls = p.Lsub, LP = p.Lsub + 1;
+  // the Lsub twiddles of the sub-FFTs sit behind the columns in LDS
+  C *tws = buf + (size_t)p.Bl * LP;
+  for (int i = threadIdx.x; i < Ls; i += blockDim.x)
+    tws[i] = ((const C *)p.tw_sub)[i];
   const int u = blockIdx.y;
   const int v0 = blockIdx.x * p.Bl;
   const int nb = min(p.Bl, p.ncol - v0);
@@ -292,16 +303,16 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(FftCols p) {
   }
   __syncthreads();
   int Ns = 1;
-  const C *tw = (const C *)p.tw_sub;
+  const C *tw = tws;
   for (int s = 0; s < p.npass; ++s) {
     const int R = p.radix[s];
     switch (R) {
-      case 4: lds_pass<T, 4, FFT_KM(T, 4)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      case 2: lds_pass<T, 2, FFT_KM(T, 2)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      case 3: lds_pass<T, 3, FFT_KM(T, 3)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      case 5: lds_pass<T, 5, FFT_KM(T, 5)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      case 7: lds_pass<T, 7, FFT_KM(T, 7)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      case 11: lds_pass<T, 11, FFT_KM(T, 11)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 4: lds_pass<T, 4, FFT_KM_EL(MAXEL, 4)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 2: lds_pass<T, 2, FFT_KM_EL(MAXEL, 2)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 3: lds_pass<T, 3, FFT_KM_EL(MAXEL, 3)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 5: lds_pass<T, 5, FFT_KM_EL(MAXEL, 5)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 7: lds_pass<T, 7, FFT_KM_EL(MAXEL, 7)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
+      case 11: lds_pass<T, 11, FFT_KM_EL(MAXEL, 11)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
       default: break;
     }
     Ns *= R;
@@ -421,7 +432,13 @@ bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
   DMX_ASSERT(p.nu <= 65535 && L1 <= 65535 && L2 <= 65535,
              "FFT line set too large");
   const int maxel = fp64 ? FFT_MAXEL64 : FFT_MAXEL32;
-  int Bl = std::min(fp64 ? 32 : 64, maxel / (std::max(L1, L2) + 1));
+  // columns per workgroup: at least 32 (coalesced row segments), within
+  // half the LDS budget when that allows 32 (more workgroups per CU)
+  const int Lm = std::max(L1, L2);
+  int Bl = (maxel / 2 - Lm) / (Lm + 1);
+  const bool half = Bl >= 32;
+  if (!half) Bl = (maxel - Lm) / (Lm + 1);
+  Bl = std::min(Bl, fp64 ? 32 : 64);
   Bl = std::max(1, std::min(Bl, p.nv));
   const size_t esz = fp64 ? 16 : 8;
   FftCols c{};
@@ -440,13 +457,16 @@ bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
     c.Lsub = Lsub;
     c.npass = (int)f.size();
     for (size_t i = 0; i < f.size(); ++i) c.radix[i] = f[i];
-    size_t sh = (size_t)Bl * (Lsub + 1) * esz;
+    size_t sh = ((size_t)Bl * (Lsub + 1) + Lsub) * esz;
     dim3 grid((unsigned)((p.nv + Bl - 1) / Bl), (unsigned)c.nlines,
               (unsigned)p.nu);
-    if (fp64)
-      hipLaunchKernelGGL(k_fft_cols<double>, grid, dim3(FFT_THREADS), sh, s, c);
-    else
-      hipLaunchKernelGGL(k_fft_cols<float>, grid, dim3(FFT_THREADS), sh, s, c);
+    if (fp64) {
+      if (half) hipLaunchKernelGGL((k_fft_cols<double, FFT_MAXEL64 / 2>), grid, dim3(FFT_THREADS), sh, s, c);
+      else hipLaunchKernelGGL((k_fft_cols<double, FFT_MAXEL64>), grid, dim3(FFT_THREADS), sh, s, c);
+    } else {
+      if (half) hipLaunchKernelGGL((k_fft_cols<float, FFT_MAXEL32 / 2>), grid, dim3(FFT_THREADS), sh, s, c);
+      else hipLaunchKernelGGL((k_fft_cols<float, FFT_MAXEL32>), grid, dim3(FFT_THREADS), sh, s, c);
+    }
     DMX_HIP(hipGetLastError());
   };
   // pass A: in place, length L1 over rows n1*L2+n2, twiddled
@@ -484,17 +504,19 @@ void fft_lineset(const FftLaunch &base, bool fp64, void *scratch,
     // workgroups per CU overlap their loads with the passes of the others
     if (p.st_ax == 1) Bl = std::min(Bl, std::max(1, (maxel / 2) / p.L));
     Bl = std::max(1, std::min(Bl, p.nv));
-    // cap lines per block by the register staging capacity: the staged
-    // butterflies per thread never exceed FFT_KM for any radix
     p.Bl = Bl;
     p.LP = p.L;
     size_t sh = (size_t)Bl * p.LP * esz;
     dim3 grid((unsigned)((p.nv + Bl - 1) / Bl), (unsigned)p.nu);
     DMX_ASSERT(p.nu <= 65535, "FFT line set too large");
-    if (fp64)
-      hipLaunchKernelGGL(k_fft_lds<double>, grid, dim3(FFT_THREADS), sh, s, p);
-    else
-      hipLaunchKernelGGL(k_fft_lds<float>, grid, dim3(FFT_THREADS), sh, s, p);
+    const bool half = (size_t)Bl * p.L <= (size_t)maxel / 2;
+    if (fp64) {
+      if (half) hipLaunchKernelGGL((k_fft_lds<double, FFT_MAXEL64 / 2>), grid, dim3(FFT_THREADS), sh, s, p);
+      else hipLaunchKernelGGL((k_fft_lds<double, FFT_MAXEL64>), grid, dim3(FFT_THREADS), sh, s, p);
+    } else {
+      if (half) hipLaunchKernelGGL((k_fft_lds<float, FFT_MAXEL32 / 2>), grid, dim3(FFT_THREADS), sh, s, p);
+      else hipLaunchKernelGGL((k_fft_lds<float, FFT_MAXEL32>), grid, dim3(FFT_THREADS), sh, s, p);
+    }
     DMX_HIP(hipGetLastError());
     return;
   }
