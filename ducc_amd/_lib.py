@@ -110,6 +110,13 @@ def lib():
             raise RuntimeError(
                 f"{LIBPATH} is missing: build it with "
                 "`python ducc_amd/csrc/build.py` (there is no CPU fallback)")
+        # torch ships its own libamdhip64.so.7.  Loaded after ours, it would
+        # be a second HIP runtime in the process and torch would find no
+        # device; loaded first, our library binds to the same runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ct.CDLL(LIBPATH)
         for name, (res, args) in _SIG.items():
             f = getattr(L, name)

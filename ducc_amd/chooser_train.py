@@ -6,13 +6,14 @@
    8-cluster Gaussian points).  For every request and transform type the
    plain method is timed over candidate (kernel row, tile) settings: for
    each support W the smallest tabulated oversampling factor in [1.2, 2.0]
-   reaching eps and two larger ones, times three tile sizes, and the
+   reaching eps and one 0.25 larger, times two or three tile sizes, and the
    analytic choice.  The label (pick_label) keeps the analytic setting
    unless a candidate is more than 5% faster, and breaks near-ties (3%)
    towards the smallest support; every candidate row reaches eps by the
    table, and the accuracy of the learned choice is re-checked at run time.
-2. Training: 6 features -> (sigma, W, log2 tile), normalised, full-batch
-   Adam with the library's HIP kernels (dmx_mlp_train).
+2. Training: 6 features -> offsets of (sigma, W, log2 tile) from the
+   analytic setting, normalised, full-batch Adam with the library's HIP
+   kernels (dmx_mlp_train), checkpoint chosen on a held-out 20 %.
 3. Evaluation: fresh requests timed with the analytic choice (ducc0's cost
    model) and with the learned choice (spot-check on), plus the error of both
    against the FP64 direct sum on sampled modes.
@@ -103,7 +104,7 @@ def candidates(req, f64, db):
         r = r[np.argsort(db["ofactor"][r])]
         smin = db["ofactor"][r[0]]
         picks = [r[0]]
-        for dlt in (0.1, 0.25):
+        for dlt in (0.25,):
             k = r[np.argmin(np.abs(db["ofactor"][r] - (smin + dlt)))]
             if k not in picks:
                 picks.append(k)
@@ -139,10 +140,14 @@ def pick_label(times, an, db, nd, f64, margin=1.05, tie=1.03):
     return k, t
 
 
-def label_requests(n, seed, db, log):
-    X, T, meta = [], [], []
+def label_requests(n, seed, db, log, start=0):
+    """Requests start .. start+n-1 of the seeded stream (batches of one
+    stream can be labelled in separate processes)."""
+    X, T, A, meta = [], [], [], []
     rng = np.random.default_rng(seed)
-    for q in range(n):
+    for q in range(start):
+        sample_request(rng)
+    for q in range(start, start + n):
         req = sample_request(rng)
         c, p, g, f64 = make_inputs(req, seed * 100000 + q)
         import torch
@@ -176,28 +181,49 @@ def label_requests(n, seed, db, log):
             X.append(chooser.features(req["shape"], req["M"], req["eps"], f64,
                                       typ == "nu2u"))
             T.append([db["ofactor"][k], db["W"][k], t])
+            A.append([db["ofactor"][an[1]], db["W"][an[1]],
+                      default_log2tile(nd, f64)])
             meta.append(dict(req=req, type=typ, analytic=list(an),
                              label=[int(k), int(t)],
                              times=[[int(kk), int(tt), v]
                                     for (kk, tt), v in times.items()]))
         del c, p, g, out_g, out_p
-        if q % 10 == 0:
-            log(f"label {q}/{n}: {req['shape']} M={req['M']} "
-                f"eps={req['eps']:.1e} -> {T[-1] if T else None}")
-    return np.array(X), np.array(T, dtype=np.float64), meta
+        log(f"label {q}/{start + n}: {req['shape']} M={req['M']} "
+            f"eps={req['eps']:.1e} -> {T[-1] if T else None}")
+    return (np.array(X), np.array(T, dtype=np.float64),
+            np.array(A, dtype=np.float64), meta)
 
 
-def fit(X, T, log, seed=0):
-    fm, fs = X.mean(0), np.maximum(X.std(0), 1e-3)
-    tm, ts = T.mean(0), np.maximum(T.std(0), 1e-3)
+def fit(X, T, A, log, seed=0, wd=1e-4, chunk=250):
+    """Adam on 80 % of the samples in chunks of `chunk` steps; the
+    checkpoint with the lowest loss on the held-out 20 % is kept (the
+    network has far more weights than there are labels).  Targets are the
+    offsets T - A of the labels from the analytic setting A."""
+    T = T - A
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(len(X))
+    nv = max(1, len(X) // 5)
+    va, tr = perm[:nv], perm[nv:]
+    fm, fs = X[tr].mean(0), np.maximum(X[tr].std(0), 1e-3)
+    tm, ts = T[tr].mean(0), np.maximum(T[tr].std(0), 1e-3)
     Xn = ((X - fm) / fs).astype(np.float32)
     Tn = ((T - tm) / ts).astype(np.float32)
     P = chooser.init_params(seed)
-    P, l1 = chooser.train(P, Xn, Tn, steps=3000, lr=3e-3, weight_decay=1e-5)
-    P, l2 = chooser.train(P, Xn, Tn, steps=3000, lr=5e-4, weight_decay=1e-5)
-    log(f"train loss {l1[0]:.4f} -> {l1[-1]:.4f} -> {l2[-1]:.4f}")
+    best = (np.inf, P.copy(), 0)
+    step, first = 0, None
+    for steps, lr in ((3000, 3e-3), (3000, 5e-4)):
+        for _ in range(steps // chunk):
+            P, lt = chooser.train(P, Xn[tr], Tn[tr], steps=chunk, lr=lr,
+                                  weight_decay=wd)
+            step += chunk
+            first = float(lt[0]) if first is None else first
+            vl = float(np.mean((chooser.forward(P, Xn[va]) - Tn[va]) ** 2))
+            if vl < best[0]:
+                best = (vl, P.copy(), step)
+            log(f"step {step}: train {float(lt[-1]):.4f} val {vl:.4f}")
+    log(f"kept step {best[2]} (validation loss {best[0]:.4f})")
     norm = np.concatenate([fm, fs, tm, ts])
-    return P, norm, [float(l1[0]), float(l1[-1]), float(l2[-1])]
+    return best[1], norm, [first, float(lt[-1]), float(best[0]), best[2]]
 
 
 def evaluate(n, seed, log):
@@ -249,6 +275,9 @@ def evaluate(n, seed, log):
                                        np.linalg.norm(ref))
         rows.append(rec)
         del c, p, g, out_g, out_p
+        log(f"eval {q}: {req['shape']} M={req['M']} eps={req['eps']:.1e} "
+            f"analytic {rec['analytic']['time']*1e3:.3f} ms, learned "
+            f"{rec['learned']['time']*1e3:.3f} ms")
         if q % 50 == 0:
             sp = [r["analytic"]["time"] / r["learned"]["time"] for r in rows]
             log(f"eval {q}/{n}: geo-mean speed-up so far "
@@ -287,11 +316,17 @@ def summarize(rows):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--problems", type=int, default=300)
+    ap.add_argument("--start", type=int, default=0,
+                    help="first request of the labelling batch")
     ap.add_argument("--eval", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=105)
     ap.add_argument("--out", default=chooser.WEIGHTS)
     ap.add_argument("--report", default=os.path.join(ROOT, "gpurun_out",
                                                      "chooser_eval.json"))
+    ap.add_argument("--label-only", default=None, metavar="NPZ",
+                    help="label a batch, write it to NPZ (+ .json timings)")
+    ap.add_argument("--labels", default=None,
+                    help="comma-separated label batches to train on")
     ap.add_argument("--skip-train", action="store_true")
     a = ap.parse_args()
 
@@ -299,24 +334,39 @@ def main():
         print(msg, flush=True)
 
     db = kernel_db()
-    if not a.skip_train:
+    if a.label_only:
         t0 = time.time()
-        X, T, meta = label_requests(a.problems, a.seed, db, log)
+        X, T, A, meta = label_requests(a.problems, a.seed, db, log, a.start)
+        np.savez_compressed(a.label_only, X=X, T=T, A=A)
+        json.dump(meta, open(os.path.splitext(a.label_only)[0] + ".json",
+                             "w"))
         log(f"labelled {len(X)} samples in {time.time() - t0:.0f} s")
-        P, norm, losses = fit(X, T, log)
+        return
+    if not a.skip_train:
+        if a.labels:
+            parts = [np.load(f) for f in a.labels.split(",")]
+            X = np.concatenate([q["X"] for q in parts])
+            T = np.concatenate([q["T"] for q in parts])
+            A = np.concatenate([q["A"] for q in parts])
+        else:
+            t0 = time.time()
+            X, T, A, meta = label_requests(a.problems, a.seed, db, log)
+            log(f"labelled {len(X)} samples in {time.time() - t0:.0f} s")
+            json.dump(meta, open(os.path.splitext(a.report)[0] +
+                                 "_labels.json", "w"))
+        P, norm, losses = fit(X, T, A, log)
         os.makedirs(os.path.dirname(a.out), exist_ok=True)
         json.dump(dict(params=[float(v) for v in P],
                        norm=[float(v) for v in norm],
                        features=["ndim", "mean log2 grid size", "log10 M",
                                  "-log10 eps", "fp64", "type 1"],
-                       targets=["oversampling factor", "W", "log2 tile"],
+                       targets=["oversampling factor - analytic",
+                                "W - analytic", "log2 tile - default"],
                        samples=int(len(X)), losses=losses,
                        seed=a.seed),
                   open(a.out, "w"))
         np.savez_compressed(os.path.splitext(a.report)[0] + "_train.npz",
                             X=X, T=T)
-        json.dump(meta, open(os.path.splitext(a.report)[0] + "_labels.json",
-                             "w"))
         log(f"weights written to {a.out}")
     chooser.load(a.out)
     rows = evaluate(a.eval, a.seed + 1, log)

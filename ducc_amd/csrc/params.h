@@ -20,6 +20,9 @@ double best_epsilon(int ndim, bool single, double smin, double smax);
 // ducc0's cost model; returns the row index (or -1) and the oversampled
 // shape.  vlen / nthreads are the host SIMD width and thread count that the
 // model assumes.
+// tile side (log2) of the spreading/interpolation subproblems before the
+// LDS limit is applied: 2^9 (1-D), 2^5 / 2^4 (2-D FP32 / FP64), 2^3 (3-D)
+int default_log2tile(int ndim, bool fp64);
 int find_params(double eps, double smin, double smax, int ndim,
                 const size_t *dims, size_t npoints, bool gridding,
                 bool single, int vlen, size_t nthreads, size_t *nover);

@@ -129,12 +129,13 @@ void Plan::flush_profile() {
 }
 
 // -------------------------------------------------------- parameter choice
-namespace {
 int default_log2tile(int ndim, bool fp64) {
   if (ndim == 1) return 9;
   if (ndim == 2) return fp64 ? 4 : 5;
   return 3;
 }
+
+namespace {
 // LDS of the tile region: FP64 accumulators and point staging (spreading;
 // interpolation stages the same region in the compute type)
 size_t lds_bytes(int ndim, int tile, int W, bool fp64) {
