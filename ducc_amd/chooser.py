@@ -2,7 +2,10 @@
 
 The network (6 -> 64 -> 64 -> 64 -> 3, ReLU) maps a request's features
 (ndim, mean log2 grid size, log10 M, -log10 eps, fp64 flag, type-1 flag) to
-(oversampling factor, support W, log2 tile).  Its forward pass, gradient and
+offsets of (oversampling factor, support W, log2 tile) from ducc0's
+analytic choice (and default tile); the library adds them to the analytic
+setting and maps the sum onto the kernel table, so an output near zero
+reproduces the analytic choice.  Its forward pass, gradient and
 Adam training run as HIP kernels in libducc_amd.so (mlp.hip).  Trained
 weights and normalisation live in ducc_amd/data/chooser_weights.json; without
 that file `chooser="learned"` falls back to ducc0's analytic choice.

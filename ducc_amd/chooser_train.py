@@ -8,12 +8,13 @@
    each support W the smallest tabulated oversampling factor in [1.2, 2.0]
    reaching eps and one 0.25 larger, times two or three tile sizes, and the
    analytic choice.  The label (pick_label) keeps the analytic setting
-   unless a candidate is more than 10% faster, and breaks near-ties (3%)
+   unless a candidate is more than 35% faster, and breaks near-ties (3%)
    towards the smallest support; every candidate row reaches eps by the
    table, and the accuracy of the learned choice is re-checked at run time.
-2. Training: 6 features -> (sigma, W, log2 tile), normalised, full-batch
-   Adam with the library's HIP kernels (dmx_mlp_train), checkpoint chosen
-   on a held-out 20 %.
+2. Training: 6 features -> offsets of (sigma, W, log2 tile) from the
+   analytic setting (default tile), normalised, full-batch Adam with the
+   library's HIP kernels (dmx_mlp_train), checkpoint chosen on a held-out
+   20 %.
 3. Evaluation: fresh requests timed with the analytic choice (ducc0's cost
    model) and with the learned choice (spot-check on), plus the error of both
    against the FP64 direct sum on sampled modes.
@@ -122,12 +123,12 @@ def default_log2tile(nd, f64):
     return 9 if nd == 1 else ((4 if f64 else 5) if nd == 2 else 3)
 
 
-def pick_label(times, an, db, nd, f64, margin=1.10, tie=1.03):
+def pick_label(times, an, db, nd, f64, margin=1.35, tie=1.03):
     """Training target from one request's timings.  times: {(kidx, tile):
     seconds}; an: (seconds, kidx, tile) of the analytic choice.  The
     analytic setting stays the label unless a candidate beats it by more
-    than `margin` (smaller gains are within what the network can learn
-    from the features and timing noise); among candidates within `tie` of the fastest the
+    than `margin` (smaller gains are within timing noise and what the six
+    features can explain: learning them made unseen requests slower); among candidates within `tie` of the fastest the
     smallest support, then the smallest oversampling, then the default tile
     wins (timing noise otherwise flips labels between near-equal
     settings)."""
@@ -218,8 +219,10 @@ def relabel(npz, db, log):
 def fit(X, T, A, log, seed=0, wd=1e-4, chunk=250):
     """Adam on 80 % of the samples in chunks of `chunk` steps; the
     checkpoint with the lowest loss on the held-out 20 % is kept (the
-    network has far more weights than there are labels).  A (the analytic
-    setting of each sample) is kept with the labels for analysis only."""
+    network has far more weights than there are labels).  Targets are the
+    offsets T - A of the labels from the analytic setting A (default tile):
+    the library adds the prediction to the analytic setting."""
+    T = T - A
     rng = np.random.default_rng(seed)
     perm = rng.permutation(len(X))
     nv = max(1, len(X) // 5)
@@ -388,7 +391,8 @@ def main():
                        norm=[float(v) for v in norm],
                        features=["ndim", "mean log2 grid size", "log10 M",
                                  "-log10 eps", "fp64", "type 1"],
-                       targets=["oversampling factor", "W", "log2 tile"],
+                       targets=["oversampling factor - analytic",
+                                "W - analytic", "log2 tile - default"],
                        samples=int(len(X)), losses=losses,
                        seed=a.seed),
                   open(a.out, "w"))

@@ -158,6 +158,21 @@ bool learned_choose(int ndim, const size_t *nuni, size_t M, double eps,
   DMX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   bool ok = chooser_predict(f, pred, s);
   (void)hipStreamDestroy(s);
+  // the heads are offsets from the analytic choice (oversampling factor,
+  // support, log2 tile): where the network has learned nothing better it
+  // outputs about zero and the mapping lands on the analytic setting
+  if (ok) {
+    size_t nover_an[3];
+    const int kan = find_params(eps, smin, smax, ndim, nuni, M, gridding,
+                                !fp64, 8, 8, nover_an);
+    if (kan < 0) {
+      ok = false;
+    } else {
+      pred[0] += kernel_db()[kan].ofactor;
+      pred[1] += kernel_db()[kan].W;
+      pred[2] += default_log2tile(ndim, fp64);
+    }
+  }
   LearnedChoice lc;
   if (ok) ok = map_choice(ndim, nuni, eps, smin, smax, fp64, pred, lc);
   if (!ok) lc.kidx = -1;

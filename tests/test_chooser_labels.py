@@ -14,11 +14,11 @@ def test_analytic_kept_unless_clearly_faster():
     db = kernel_db()
     a = int(_rows(db, 2, 1, 7)[3])
     b = int(_rows(db, 2, 1, 6)[3])
-    # 4% faster than the analytic setting: not enough, analytic stays
-    times = {(a, 5): 1.0, (b, 5): 0.96}
+    # 20% faster than the analytic setting: not enough, analytic stays
+    times = {(a, 5): 1.0, (b, 5): 0.80}
     assert pick_label(times, (1.0, a, 5), db, 2, False) == (a, 5)
-    # 10% faster: the candidate wins
-    times = {(a, 5): 1.0, (b, 5): 0.90}
+    # 30% less time (1.43x): the candidate wins
+    times = {(a, 5): 1.0, (b, 5): 0.70}
     assert pick_label(times, (1.0, a, 5), db, 2, False) == (b, 5)
 
 
@@ -31,7 +31,7 @@ def test_near_ties_prefer_small_support_then_small_sigma_then_default_tile():
              (k6a, 4): 0.95}
     # all but the last are within 3% of the best; W=6 beats W=7, the smaller
     # oversampling factor beats the larger, the default tile (5) beats 6
-    assert pick_label(times, (1.0, k7, 4), db, 2, False) == (k6a, 5)
+    assert pick_label(times, (1.2, k7, 4), db, 2, False) == (k6a, 5)
 
 
 def test_candidates_reach_eps_and_stay_in_sigma_range():

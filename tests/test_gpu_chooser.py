@@ -73,27 +73,31 @@ def test_learned_choice_is_valid_and_accurate(gpu):
 
 def test_network_choice_and_fallback(gpu):
     """A network whose output is fixed (zero weights: outputs = the target
-    means) asks for sigma 1.3, W 5, tile 2^4: the request uses the nearest
-    tabulated kernel reaching eps.  When the FP64 spot-check is forced to
-    fail from the host, the call takes ducc0's analytic choice instead, and
-    the result still meets eps."""
+    means) asks for the analytic setting with one more support point and a
+    tile of half the default side: the request uses the tabulated kernel of
+    that support nearest the analytic oversampling factor.  When the FP64
+    spot-check is forced to fail from the host, the call takes ducc0's
+    analytic choice instead, and the result still meets eps."""
     from ducc_amd import chooser, nufft
+    rng = np.random.default_rng(8)
+    shape, M, eps = (64, 64), 5000, 1e-3
+    c = rng.uniform(-np.pi, np.pi, (M, 2))
+    p = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    ref = O.nudft_type1(c, p, shape, True)
+    an = {}
+    out = np.empty(shape, np.complex128)
+    nufft.nu2u(points=p, coord=c, forward=True, epsilon=eps, out=out,
+               chooser="analytic", info=an)
     params = np.zeros(chooser.nparam(), np.float32)
-    norm = np.concatenate([np.zeros(6), np.ones(6), [1.3, 5.0, 4.0],
+    norm = np.concatenate([np.zeros(6), np.ones(6), [0.0, 1.0, -1.0],
                            np.ones(3)])
     chooser.install(params, norm)
     try:
-        rng = np.random.default_rng(8)
-        shape, M, eps = (64, 64), 5000, 1e-3
-        c = rng.uniform(-np.pi, np.pi, (M, 2))
-        p = rng.standard_normal(M) + 1j * rng.standard_normal(M)
-        ref = O.nudft_type1(c, p, shape, True)
         info = {}
-        out = np.empty(shape, np.complex128)
         nufft.nu2u(points=p, coord=c, forward=True, epsilon=eps, out=out,
                    chooser="learned", info=info)
-        assert info["chooser_used"] and info["W"] == 5
-        assert info["log2tile"] == 4
+        assert info["chooser_used"] and info["W"] == an["W"] + 1
+        assert info["log2tile"] == 4 - 1   # default 2-D FP64 tile: 2^4
         assert info["eps_tab"] <= eps
         assert O.l2error(out, ref) <= 2 * eps
         info2 = {}
