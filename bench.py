@@ -26,6 +26,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
+# ds_add_f64 with the lanes of each quarter-wave on distinct bank classes,
+# measured on MI355X by ducc_amd/csrc/probes/lds_atomics.hip (the rate the
+# spreader's LDS accumulation can reach; random addresses reach a third)
+PEAK_LDS_F64_ATOMICS = 4.66e12
 
 
 def clustered_points(rng, M, d=2, ncl=8, sig=0.3):
@@ -236,9 +240,13 @@ def main():
     else:
         b = alg_bytes(name, inf, M, d, sv)
     achieved = b / (per_launch[dom] * 1e-3) / 1e9
+    # HBM bytes per launch from the profiler's FETCH_SIZE + WRITE_SIZE of
+    # this workload (calibrated by a plain copy), when recorded for the
+    # parameters in use
     traffic = None
     tp = os.path.join(HERE, "profiles", "traffic.json")
-    if os.path.exists(tp):
+    if os.path.exists(tp) and inf.get("W") == 7 and \
+            list(inf.get("nover", [])) == [3388, 3388]:
         try:
             traffic = json.load(open(tp)).get(dom)
         except Exception:
@@ -248,6 +256,19 @@ def main():
                     frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic,
                     alg_bytes_per_launch=int(b),
                     avg_launch_ms=round(per_launch[dom], 4))
+    if traffic:
+        hb = traffic / (per_launch[dom] * 1e-3) / 1e9
+        roofline["traffic_GBps"] = round(hb, 1)
+        roofline["traffic_frac"] = round(hb / PEAK_HBM_GBS, 4)
+    if name == "spreading":
+        # LDS atomics: W^d taps x (re, im) per point, plus one ranking
+        # atomic per point
+        na = M * (2 * inf["W"] ** d + 1)
+        ra = na / (per_launch[dom] * 1e-3)
+        roofline["lds_atomics"] = dict(
+            per_launch=int(na), achieved=float(f"{ra:.4g}"),
+            peak=PEAK_LDS_F64_ATOMICS, unit="atomics/s",
+            frac=round(ra / PEAK_LDS_F64_ATOMICS, 4))
 
     cpu = None
     if not args.no_cpu:

@@ -203,9 +203,10 @@ void launch_grid_copy(bool fp64, bool pad, const GridCopyArgs &a,
 void launch_nudft(const NudftArgs &a, hipStream_t s) {
   if (!a.nsel) return;
   const unsigned bx = (unsigned)((a.nsel + 255) / 256);
-  // split the inner sum so that about 4096 workgroups run
+  // split the inner sum (chunks of at least 256 terms) so that about 4096
+  // workgroups run
   const int64_t inner = a.type == 1 ? a.M : a.n[0] * a.n[1] * a.n[2];
-  int64_t ny = std::max<int64_t>(1, std::min<int64_t>(4096 / bx, inner / 4096));
+  int64_t ny = std::max<int64_t>(1, std::min<int64_t>(4096 / bx, inner / 256));
   ny = std::min<int64_t>(ny, 65535);
   if (ny > 1) launch_zero(a.out, (size_t)a.nsel * 16, s);
   hipLaunchKernelGGL(k_nudft, dim3(bx, (unsigned)ny), dim3(256), 0, s, a);
