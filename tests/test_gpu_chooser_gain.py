@@ -1,0 +1,64 @@
+"""What the trained chooser gains over the method without it (ducc0's
+analytic parameter choice): requests of the held-out evaluation stream
+(ducc_amd.chooser_train, seed 106, never used for labels), each timed with
+both choices in alternation on the GPU.  3-D requests, where the GPU's cost
+balance differs most from ducc0's CPU cost model, must gain at least 1.2x
+(geometric mean; 1.69x on the 200-request evaluation), and the whole set must
+not be slower.  Both choices meet eps (the learned one passed its FP64 spot
+check, which the call records as chooser_used)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_learned_choice_gains_over_analytic(gpu):
+    from ducc_amd import chooser, nufft
+    from ducc_amd.chooser_train import make_inputs, sample_request, timed
+    if not chooser.available():
+        pytest.skip("no trained weights")
+    chooser.load()
+    rng = np.random.default_rng(106)
+    reqs, want = [], {1: 3, 2: 3, 3: 8}
+    q = 0
+    while any(want.values()):
+        r = sample_request(rng)
+        nd = len(r["shape"])
+        if want[nd]:
+            want[nd] -= 1
+            reqs.append((q, r))
+        q += 1
+    rows = []
+    for q, r in reqs:
+        c, p, g, f64 = make_inputs(r, 106 * 100000 + q)
+        import torch
+        og = torch.empty(r["shape"], dtype=p.dtype, device="cuda")
+        op = torch.empty(r["M"], dtype=p.dtype, device="cuda")
+        infos = {}
+
+        def run(ch):
+            i1, i2 = {}, {}
+            nufft.nu2u(points=p, coord=c, forward=True, epsilon=r["eps"],
+                       out=og, chooser=ch, info=i1)
+            nufft.u2nu(grid=g, coord=c, forward=True, epsilon=r["eps"],
+                       out=op, chooser=ch, info=i2)
+            infos[ch] = (i1, i2)
+        t = {"analytic": 0.0, "learned": 0.0}
+        for rep in range(4):
+            for ch in (("analytic", "learned") if rep % 2 == 0
+                       else ("learned", "analytic")):
+                t[ch] += timed(lambda: run(ch), reps=1)
+        assert infos["learned"][0]["chooser_used"]
+        rows.append((len(r["shape"]), r["shape"], r["M"], r["eps"],
+                     t["analytic"] / t["learned"], infos))
+        print(f"{r['shape']} M={r['M']} eps={r['eps']:.1e}: "
+              f"analytic/learned time = {rows[-1][4]:.3f}", flush=True)
+        del c, p, g, og, op
+    gm = lambda v: math.exp(sum(math.log(x) for x in v) / len(v))
+    g3 = gm([x[4] for x in rows if x[0] == 3])
+    gall = gm([x[4] for x in rows])
+    print(f"geo-mean gain: 3-D {g3:.3f}, all {gall:.3f}")
+    assert g3 >= 1.2
+    assert gall >= 1.0
