@@ -75,6 +75,14 @@ def fft_pass_bytes(info, type1, sv):
     return out
 
 
+def l2rel(a, b):
+    """ducc0's l2error: sqrt(sum|a-b|^2 / max(sum|a|^2, sum|b|^2))"""
+    a = np.asarray(a, np.complex128).ravel()
+    b = np.asarray(b, np.complex128).ravel()
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) /
+                         max(np.sum(np.abs(a) ** 2), np.sum(np.abs(b) ** 2))))
+
+
 def time_steps(step, steps, warmup, sync):
     for _ in range(warmup):
         step()
@@ -86,9 +94,11 @@ def time_steps(step, steps, warmup, sync):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(coords, pts, grid, eps, M_total):
+def cpu_baseline(coords, pts, grid, eps, M_total, fast=None):
     """ducc0 (compiled from the reference sources into oracle/_ref) on all
-    host cores, on the same workload."""
+    host cores, on the same workload; with `fast` = (type-1 result, type-2
+    result) of the GPU fast path, also their L2 errors against ducc0's own
+    double-precision nu2u / u2nu of the same inputs."""
     try:
         from oracle import ref as R
     except Exception:
@@ -104,8 +114,16 @@ def cpu_baseline(coords, pts, grid, eps, M_total):
     t1 = time.perf_counter()
     R.u2nu(grid, coords, False, eps, nthreads=nth)
     t2 = time.perf_counter()
+    err = None
+    if fast is not None:
+        # eps 1e-5 in double precision (W from ducc0's FP64 table)
+        r1 = R.nu2u(pts.astype(np.complex128), coords, True, eps, shape,
+                    nthreads=nth)
+        r2 = R.u2nu(grid.astype(np.complex128), coords, False, eps,
+                    nthreads=nth)
+        err = dict(nu2u=l2rel(fast[0], r1), u2nu=l2rel(fast[1], r2))
     return dict(value=2 * len(pts) / (t2 - t0), unit="points/s", cores=nth,
-                kind="reference",
+                kind="reference", err_fast_vs_ducc0_fp64=err,
                 sample=(f"ducc0 nu2u + u2nu, full workload (M={len(pts)}, "
                         f"{shape[0]}x{shape[1]}, eps={eps}, complex64, "
                         f"{nth} threads), one timed pass after a warm-up; "
@@ -202,10 +220,7 @@ def main():
         a2 = op32.cpu().numpy().astype(np.complex128)
         b2 = op64.cpu().numpy()
 
-        def l2(a, b):
-            return float(np.sqrt(np.sum(np.abs(a - b) ** 2) /
-                                 max(np.sum(np.abs(a) ** 2),
-                                     np.sum(np.abs(b) ** 2))))
+        l2 = l2rel
         sel_m = rng.choice(n * n, 256, replace=False)
         sel_p = rng.choice(M, 256, replace=False)
         d1 = nufft.nudft_direct(1, c_np, p_np.astype(np.complex128), (n, n),
@@ -272,7 +287,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(c_np, p_np, g_np, eps, M)
+        cpu = cpu_baseline(c_np, p_np, g_np, eps, M,
+                           fast=(og32.cpu().numpy(), op32.cpu().numpy()))
 
     line = dict(
         metric="nonuniform pts/s (type1+type2) @eps 1e-5: FP32+learned vs "
