@@ -36,7 +36,7 @@ struct Geo {
   int nover[3], ntiles[3], maxi0[3];
   double coordfct[3], shift[3];
   int coord_f32;  // 1: coordinates stored as float32
-  int pad_;
+  int coord_v2;   // 1: 2-D float64 coordinates 16-byte aligned (one load)
 };
 
 // One subproblem: a run of at most MAXSUB sorted points of one tile.
@@ -108,6 +108,18 @@ __device__ __forceinline__ double load_coord(const void *c, size_t j, int ndim,
                                              int d, int f32) {
   return f32 ? (double)((const float *)c)[j * ndim + d]
              : ((const double *)c)[j * ndim + d];
+}
+
+// all coordinates of point j (one 16-byte load for aligned 2-D float64)
+__device__ __forceinline__ void load_point(const Geo &g, const void *c,
+                                           size_t j, int ndim, double *x) {
+  if (ndim == 2 && g.coord_v2) {
+    const double2 v = ((const double2 *)c)[j];
+    x[0] = v.x;
+    x[1] = v.y;
+  } else {
+    for (int d = 0; d < ndim; ++d) x[d] = load_coord(c, j, ndim, d, g.coord_f32);
+  }
 }
 
 // Values of the W kernel pieces at local coordinate x (x = W-1-2*frac),

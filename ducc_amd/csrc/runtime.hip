@@ -368,6 +368,10 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
   coords_ = coords;
   user_coords_ = coords;
   coords_sorted_ = false;
+  g_.coord_v2 = (p_.ndim == 2 && !coord_f32 &&
+                 reinterpret_cast<uintptr_t>(coords) % 16 == 0)
+                    ? 1
+                    : 0;
   if (M == 0) return;
   int keybits = 1;
   while (((size_t)1 << keybits) < ntiles_) ++keybits;
@@ -404,11 +408,17 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
                          sorted_coords_.p, s_);
     coords_ = sorted_coords_.p;
     coords_sorted_ = true;
+    g_.coord_v2 = (p_.ndim == 2 && !coord_f32) ? 1 : 0;  // pool: aligned
   }
 }
 
 uint32_t *Plan::keys_copy_to(uint32_t *dev_keys_out) {
-  launch_fold_keys(g_, user_coords_, M_, dev_keys_out, nullptr, s_);
+  Geo g = g_;
+  g.coord_v2 = (p_.ndim == 2 && !coord_f32_ &&
+                reinterpret_cast<uintptr_t>(user_coords_) % 16 == 0)
+                   ? 1
+                   : 0;
+  launch_fold_keys(g, user_coords_, M_, dev_keys_out, nullptr, s_);
   return dev_keys_out;
 }
 

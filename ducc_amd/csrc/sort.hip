@@ -38,10 +38,11 @@ __global__ __launch_bounds__(RT) void k_fold_keys(Geo g, const void *coords,
     const size_t j = base + (size_t)i * RT + tid;
     if (j >= M) break;
     uint32_t t[3] = {0, 0, 0};
+    double x[3];
+    load_point(g, coords, j, g.ndim, x);
     for (int d = 0; d < g.ndim; ++d) {
       double fr;
-      int i0 = fold_axis(g, d, load_coord(coords, j, g.ndim, d, g.coord_f32),
-                         fr);
+      int i0 = fold_axis(g, d, x[d], fr);
       t[d] = (uint32_t)((i0 + g.nsafe) >> g.log2tile);
     }
     uint32_t key;

@@ -69,11 +69,12 @@ __device__ __forceinline__ void point_setup(const SpreadArgs &a, uint32_t p,
   j = (a.coords_sorted && a.values_sorted) ? p : a.idx[p];
   const size_t cj = a.coords_sorted ? (size_t)p : (size_t)j;
   const T *cf = (const T *)a.coef;
+  double x_[NDIM];
+  load_point(a.g, a.coords, cj, NDIM, x_);
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     double frac;
-    i0[d] = fold_axis(a.g, d, load_coord(a.coords, cj, NDIM, d, a.g.coord_f32),
-                      frac);
+    i0[d] = fold_axis(a.g, d, x_[d], frac);
     T x = (T)((double)(W - 1) - 2.0 * frac);
     kernel_weights<T, W>(x, cf, w[d]);
   }
@@ -177,9 +178,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
           if (i < nc) {
             const uint32_t p = c0 + i;
             const size_t cj = a.coords_sorted ? (size_t)p : (size_t)jj[k];
-#pragma unroll
-            for (int d = 0; d < NDIM; ++d)
-              cc[k][d] = load_coord(a.coords, cj, NDIM, d, a.g.coord_f32);
+            load_point(a.g, a.coords, cj, NDIM, cc[k]);
             load_value<T>(a.values,
                           a.values_sorted ? (size_t)p : (size_t)jj[k],
                           a.val_f64, vr_[k], vi_[k]);
@@ -407,12 +406,12 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
       int i0[3];
       T w[NDIM][W];
       T dw[MODE ? NDIM : 1][W];
+      double x_[NDIM];
+      load_point(a.g, a.coords, cj, NDIM, x_);
 #pragma unroll
       for (int d = 0; d < NDIM; ++d) {
         double frac;
-        i0[d] = fold_axis(a.g, d,
-                          load_coord(a.coords, cj, NDIM, d, a.g.coord_f32),
-                          frac);
+        i0[d] = fold_axis(a.g, d, x_[d], frac);
         T x = (T)((double)(W - 1) - 2.0 * frac);
         if constexpr (MODE == 0)
           kernel_weights<T, W>(x, cf, w[d]);
