@@ -14,6 +14,16 @@ namespace dmx {
 #define FFT_KM_EL(MAXEL, R) (((MAXEL) + (R)*FFT_THREADS - 1) / ((R)*FFT_THREADS))
 #define FFT_KM(T, R) FFT_KM_EL(sizeof(T) == 8 ? FFT_MAXEL64 : FFT_MAXEL32, R)
 
+// Constants of one Stockham pass, computed on the host: radix, butterflies
+// per line (L/R), sub-transform length before the pass (Ns), twiddle stride
+// L/(Ns*R), and float reciprocals for the division-free index math.
+struct FftPass {
+  int radix, nbf, ns, twstep;
+  float inv_nbf, inv_ns;
+};
+// fills pass[0..npass) for a line of length L; returns npass
+int fft_plan_passes(int L, FftPass *pass, int maxpass);
+
 // Lines along one axis of a row-major array.  Line (u, v) starts at
 // U(u)*st_u + V(v)*st_v, element k at + k*st_ax, where the virtual indices
 // map to actual ones through at most two ranges:
@@ -22,7 +32,7 @@ struct FftLaunch {
   void *data;
   const void *tw;  // exp(-2 pi i m / L), m < L, compute type
   int L, LP, Bl, npass, sign;
-  int radix[24];
+  FftPass pass[24];
   int64_t st_ax, st_u, st_v;
   int nu, nv;
   int un1, ua1, ua2, vn1, va1, va2;
@@ -38,7 +48,7 @@ struct FftCols {
   const void *tw_sub;   // twiddles of length Lsub
   const void *tw_full;  // twiddles of length L (inter-pass factors)
   int L, Lsub, npass, sign, twiddle, Bl;
-  int radix[24];
+  FftPass pass[24];
   int64_t ia, ib, oa, ob;  // row maps of input / output (in rows)
   int64_t rs;              // row stride (elements)
   int ncol, vn1, va1, va2; // columns (virtual -> actual, stride 1)

@@ -153,20 +153,17 @@ __device__ __forceinline__ int fdivq(int x, int d, float inv, int &r) {
   return q;
 }
 
-// One radix-R Stockham pass over Bl LDS-resident lines (stride LP).  The
-// inputs of all butterflies a thread owns are read and twiddled before the
-// barrier (tw: L twiddles, in LDS when the launch staged them), so after it
-// only the DFTs and the writes remain.
-template <typename T, int R, int KM>
-__device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
-                                         int Bl, int Ns,
-                                         const typename V2<T>::t *tw,
-                                         int sign) {
+// One radix-R Stockham pass over Bl LDS-resident lines (stride LP), with
+// the transform sign SIGN known at compile time.  The inputs of all
+// butterflies a thread owns are read and twiddled before the barrier (tw: L
+// twiddles), so after it only the DFTs and the writes remain.
+template <typename T, int R, int KM, int SIGN>
+__device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int LP,
+                                         int Bl, const FftPass &ps,
+                                         const typename V2<T>::t *tw) {
   using C = typename V2<T>::t;
-  const int nbf = L / R;
+  const int nbf = ps.nbf, Ns = ps.ns, twstep = ps.twstep;
   const int total = nbf * Bl;
-  const int twstep = L / (Ns * R);
-  const float inv_nbf = 1.0f / (float)nbf, inv_ns = 1.0f / (float)Ns;
   C v[KM][R];
   int obase[KM];
 #pragma unroll
@@ -174,16 +171,17 @@ __device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
     const int q = threadIdx.x + k * blockDim.x;
     if (q < total) {
       int jj;
-      const int b = Bl == 1 ? (jj = q, 0) : fdivq(q, nbf, inv_nbf, jj);
+      const int b = Bl == 1 ? (jj = q, 0) : fdivq(q, nbf, ps.inv_nbf, jj);
 #pragma unroll
       for (int r = 0; r < R; ++r) v[k][r] = buf[b * LP + jj + r * nbf];
       int kk;
-      const int jq = Ns == 1 ? (kk = 0, jj) : fdivq(jj, Ns, inv_ns, kk);
+      const int jq = Ns == 1 ? (kk = 0, jj) : fdivq(jj, Ns, ps.inv_ns, kk);
       if (Ns > 1) {
+        const int t1 = kk * twstep;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          C w = tw[r * kk * twstep];
-          if (sign > 0) w.y = -w.y;
+          C w = tw[r * t1];
+          if (SIGN > 0) w.y = -w.y;
           v[k][r] = cmul(v[k][r], w);
         }
       }
@@ -195,12 +193,32 @@ __device__ __forceinline__ void lds_pass(typename V2<T>::t *buf, int L, int LP,
   for (int k = 0; k < KM; ++k) {
     const int q = threadIdx.x + k * blockDim.x;
     if (q < total) {
-      dft<T, R>(v[k], sign);
+      dft<T, R>(v[k], SIGN);
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[obase[k] + r * Ns] = v[k][r];
+      for (int r = 0; r < R; ++r) buf[obase[k] + r * ps.ns] = v[k][r];
     }
   }
   __syncthreads();
+}
+
+// all passes of one line set staged in LDS
+template <typename T, int MAXEL, int SIGN>
+__device__ __forceinline__ void lds_passes(typename V2<T>::t *buf, int LP,
+                                           int Bl, int npass,
+                                           const FftPass *pass,
+                                           const typename V2<T>::t *tw) {
+  for (int s = 0; s < npass; ++s) {
+    const FftPass ps = pass[s];
+    switch (ps.radix) {
+      case 4: lds_pass<T, 4, FFT_KM_EL(MAXEL, 4), SIGN>(buf, LP, Bl, ps, tw); break;
+      case 2: lds_pass<T, 2, FFT_KM_EL(MAXEL, 2), SIGN>(buf, LP, Bl, ps, tw); break;
+      case 3: lds_pass<T, 3, FFT_KM_EL(MAXEL, 3), SIGN>(buf, LP, Bl, ps, tw); break;
+      case 5: lds_pass<T, 5, FFT_KM_EL(MAXEL, 5), SIGN>(buf, LP, Bl, ps, tw); break;
+      case 7: lds_pass<T, 7, FFT_KM_EL(MAXEL, 7), SIGN>(buf, LP, Bl, ps, tw); break;
+      case 11: lds_pass<T, 11, FFT_KM_EL(MAXEL, 11), SIGN>(buf, LP, Bl, ps, tw); break;
+      default: break;
+    }
+  }
 }
 
 __device__ __forceinline__ int64_t map_range(int v, int n1, int a1, int a2) {
@@ -210,7 +228,7 @@ __device__ __forceinline__ int64_t map_range(int v, int n1, int a1, int a2) {
 // MAXEL: the most elements (lines x length) the launch puts in one
 // workgroup; it sizes the register staging of the passes, and the half-size
 // variant leaves room for more workgroups per CU
-template <typename T, int MAXEL>
+template <typename T, int MAXEL, int SIGN>
 __global__ __launch_bounds__(FFT_THREADS) __attribute__((
     amdgpu_waves_per_eu(sizeof(T) == 4 && MAXEL <= 4096 ? 5 : 1))) void
 k_fft_lds(FftLaunch p) {
@@ -243,20 +261,7 @@ k_fft_lds(FftLaunch p) {
     buf[b * LP + k] = data[addr];
   }
   __syncthreads();
-  int Ns = 1;
-  for (int s = 0; s < p.npass; ++s) {
-    const int R = p.radix[s];
-    switch (R) {
-      case 4: lds_pass<T, 4, FFT_KM_EL(MAXEL, 4)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      case 2: lds_pass<T, 2, FFT_KM_EL(MAXEL, 2)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      case 3: lds_pass<T, 3, FFT_KM_EL(MAXEL, 3)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      case 5: lds_pass<T, 5, FFT_KM_EL(MAXEL, 5)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      case 7: lds_pass<T, 7, FFT_KM_EL(MAXEL, 7)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      case 11: lds_pass<T, 11, FFT_KM_EL(MAXEL, 11)>(buf, L, LP, nlines, Ns, tw, p.sign); break;
-      default: break;
-    }
-    Ns *= R;
-  }
+  lds_passes<T, MAXEL, SIGN>(buf, LP, nlines, p.npass, p.pass, tw);
   // store
   for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     int b, k;
@@ -273,7 +278,7 @@ k_fft_lds(FftLaunch p) {
 // u*ia + e*ib of the source; output element k goes to row u*oa + k*ob of
 // the destination, optionally times w_L^{(k*u) mod L}.  Loads and stores
 // move Bl contiguous columns per row (coalesced).
-template <typename T, int MAXEL>
+template <typename T, int MAXEL, int SIGN>
 __global__ __launch_bounds__(FFT_THREADS) __attribute__((
     amdgpu_waves_per_eu(sizeof(T) == 4 && MAXEL <= 4096 ? 5 : 1))) void
 k_fft_cols(FftCols p) {
@@ -302,21 +307,7 @@ k_fft_cols(FftCols p) {
         src[obase + row * p.rs + map_range(v0 + b, p.vn1, p.va1, p.va2)];
   }
   __syncthreads();
-  int Ns = 1;
-  const C *tw = tws;
-  for (int s = 0; s < p.npass; ++s) {
-    const int R = p.radix[s];
-    switch (R) {
-      case 4: lds_pass<T, 4, FFT_KM_EL(MAXEL, 4)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      case 2: lds_pass<T, 2, FFT_KM_EL(MAXEL, 2)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      case 3: lds_pass<T, 3, FFT_KM_EL(MAXEL, 3)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      case 5: lds_pass<T, 5, FFT_KM_EL(MAXEL, 5)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      case 7: lds_pass<T, 7, FFT_KM_EL(MAXEL, 7)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      case 11: lds_pass<T, 11, FFT_KM_EL(MAXEL, 11)>(buf, Ls, LP, nb, Ns, tw, p.sign); break;
-      default: break;
-    }
-    Ns *= R;
-  }
+  lds_passes<T, MAXEL, SIGN>(buf, LP, nb, p.npass, p.pass, tws);
   const C *twf = (const C *)p.tw_full;
   for (int i = threadIdx.x; i < nel; i += blockDim.x) {
     int b;
@@ -326,7 +317,7 @@ k_fft_cols(FftCols p) {
       int m;
       fdivq(k * u, p.L, inv_L, m);  // k * u < L1 * L2 = L
       C w = twf[m];
-      if (p.sign > 0) w.y = -w.y;
+      if (SIGN > 0) w.y = -w.y;
       x = cmul(x, w);
     }
     const int64_t row = (int64_t)u * p.oa + (int64_t)k * p.ob;
@@ -377,6 +368,27 @@ __global__ __launch_bounds__(256) void k_fft_global_pass(FftLaunch p,
   }
 }
 
+std::vector<int> factorize(int L);
+}  // namespace
+
+int fft_plan_passes(int L, FftPass *pass, int maxpass) {
+  const auto f = factorize(L);
+  DMX_ASSERT((int)f.size() <= maxpass, "too many FFT passes");
+  int Ns = 1;
+  for (size_t i = 0; i < f.size(); ++i) {
+    FftPass &q = pass[i];
+    q.radix = f[i];
+    q.nbf = L / f[i];
+    q.ns = Ns;
+    q.twstep = L / (Ns * f[i]);
+    q.inv_nbf = 1.0f / (float)q.nbf;
+    q.inv_ns = 1.0f / (float)Ns;
+    Ns *= f[i];
+  }
+  return (int)f.size();
+}
+
+namespace {
 std::vector<int> factorize(int L) {
   std::vector<int> f;
   for (int r : {4, 2, 3, 5, 7, 11})
@@ -453,20 +465,22 @@ bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
   c.st_o = p.st_u;
   c.tw_full = tw_L;
   auto launch = [&](int Lsub) {
-    auto f = factorize(Lsub);
     c.Lsub = Lsub;
-    c.npass = (int)f.size();
-    for (size_t i = 0; i < f.size(); ++i) c.radix[i] = f[i];
+    c.npass = fft_plan_passes(Lsub, c.pass, 24);
     size_t sh = ((size_t)Bl * (Lsub + 1) + Lsub) * esz;
     dim3 grid((unsigned)((p.nv + Bl - 1) / Bl), (unsigned)c.nlines,
               (unsigned)p.nu);
+#define DMX_COLS(TT, ME, SG) \
+  hipLaunchKernelGGL((k_fft_cols<TT, ME, SG>), grid, dim3(FFT_THREADS), sh, s, c)
+    const bool neg = c.sign < 0;
     if (fp64) {
-      if (half) hipLaunchKernelGGL((k_fft_cols<double, FFT_MAXEL64 / 2>), grid, dim3(FFT_THREADS), sh, s, c);
-      else hipLaunchKernelGGL((k_fft_cols<double, FFT_MAXEL64>), grid, dim3(FFT_THREADS), sh, s, c);
+      if (half) { if (neg) DMX_COLS(double, FFT_MAXEL64 / 2, -1); else DMX_COLS(double, FFT_MAXEL64 / 2, 1); }
+      else { if (neg) DMX_COLS(double, FFT_MAXEL64, -1); else DMX_COLS(double, FFT_MAXEL64, 1); }
     } else {
-      if (half) hipLaunchKernelGGL((k_fft_cols<float, FFT_MAXEL32 / 2>), grid, dim3(FFT_THREADS), sh, s, c);
-      else hipLaunchKernelGGL((k_fft_cols<float, FFT_MAXEL32>), grid, dim3(FFT_THREADS), sh, s, c);
+      if (half) { if (neg) DMX_COLS(float, FFT_MAXEL32 / 2, -1); else DMX_COLS(float, FFT_MAXEL32 / 2, 1); }
+      else { if (neg) DMX_COLS(float, FFT_MAXEL32, -1); else DMX_COLS(float, FFT_MAXEL32, 1); }
     }
+#undef DMX_COLS
     DMX_HIP(hipGetLastError());
   };
   // pass A: in place, length L1 over rows n1*L2+n2, twiddled
@@ -492,10 +506,7 @@ void fft_lineset(const FftLaunch &base, bool fp64, void *scratch,
                  hipStream_t s) {
   FftLaunch p = base;
   if (p.nu <= 0 || p.nv <= 0 || p.L <= 1) return;
-  auto f = factorize(p.L);
-  DMX_ASSERT(f.size() <= 24, "too many FFT passes");
-  p.npass = (int)f.size();
-  for (size_t i = 0; i < f.size(); ++i) p.radix[i] = f[i];
+  p.npass = fft_plan_passes(p.L, p.pass, 24);
   const size_t esz = fp64 ? 16 : 8;
   const int maxel = fp64 ? FFT_MAXEL64 : FFT_MAXEL32;
   if (p.L <= maxel) {
@@ -510,13 +521,17 @@ void fft_lineset(const FftLaunch &base, bool fp64, void *scratch,
     dim3 grid((unsigned)((p.nv + Bl - 1) / Bl), (unsigned)p.nu);
     DMX_ASSERT(p.nu <= 65535, "FFT line set too large");
     const bool half = (size_t)Bl * p.L <= (size_t)maxel / 2;
+#define DMX_LDS(TT, ME, SG) \
+  hipLaunchKernelGGL((k_fft_lds<TT, ME, SG>), grid, dim3(FFT_THREADS), sh, s, p)
+    const bool neg = p.sign < 0;
     if (fp64) {
-      if (half) hipLaunchKernelGGL((k_fft_lds<double, FFT_MAXEL64 / 2>), grid, dim3(FFT_THREADS), sh, s, p);
-      else hipLaunchKernelGGL((k_fft_lds<double, FFT_MAXEL64>), grid, dim3(FFT_THREADS), sh, s, p);
+      if (half) { if (neg) DMX_LDS(double, FFT_MAXEL64 / 2, -1); else DMX_LDS(double, FFT_MAXEL64 / 2, 1); }
+      else { if (neg) DMX_LDS(double, FFT_MAXEL64, -1); else DMX_LDS(double, FFT_MAXEL64, 1); }
     } else {
-      if (half) hipLaunchKernelGGL((k_fft_lds<float, FFT_MAXEL32 / 2>), grid, dim3(FFT_THREADS), sh, s, p);
-      else hipLaunchKernelGGL((k_fft_lds<float, FFT_MAXEL32>), grid, dim3(FFT_THREADS), sh, s, p);
+      if (half) { if (neg) DMX_LDS(float, FFT_MAXEL32 / 2, -1); else DMX_LDS(float, FFT_MAXEL32 / 2, 1); }
+      else { if (neg) DMX_LDS(float, FFT_MAXEL32, -1); else DMX_LDS(float, FFT_MAXEL32, 1); }
     }
+#undef DMX_LDS
     DMX_HIP(hipGetLastError());
     return;
   }
@@ -548,9 +563,9 @@ void fft_lineset(const FftLaunch &base, bool fp64, void *scratch,
     DMX_HIP(hipGetLastError());
     std::swap(src, dst);
   };
-  for (int R : f) {
-    run(R);
-    Ns *= R;
+  for (int i = 0; i < p.npass; ++i) {
+    run(p.pass[i].radix);
+    Ns *= p.pass[i].radix;
   }
   if (src != p.data) {
     // odd number of passes: copy back with an identity pass
