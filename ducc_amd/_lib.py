@@ -29,7 +29,9 @@ class DmxOptions(ct.Structure):
                 ("log2tile", ct.c_int32), ("kernel_idx", ct.c_int32),
                 ("fp64_below", ct.c_double), ("stream", ct.c_void_p),
                 ("sigma", ct.c_double), ("spot_check", ct.c_int32),
-                ("reserved", ct.c_int32 * 7)]
+                ("flags", ct.c_int32), ("reserved", ct.c_int32 * 6)]
+
+STREAM_GIVEN, ASYNC = 1, 2
 
 
 class DmxPlanInfo(ct.Structure):
@@ -213,6 +215,11 @@ def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
     o.log2tile = int(log2tile)
     o.kernel_idx = int(kernel_idx)
     o.fp64_below = float(fp64_below)
-    o.stream = stream
+    # a stream (torch tensors on the GPU: torch's current stream, NULL for
+    # the default stream) means: run on it and return without waiting when
+    # all arrays are on the device, as torch operations do
+    if stream is not None:
+        o.stream = stream
+        o.flags = STREAM_GIVEN | ASYNC
     o.sigma = float(sigma)
     return o

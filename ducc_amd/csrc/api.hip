@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <vector>
@@ -164,7 +166,9 @@ struct DevOut {
       p = tmp.p;
     }
   }
-  void finish(hipStream_t s) {
+  // wait = false: device outputs are left to complete in stream order
+  void finish(hipStream_t s, bool wait = true) {
+    if (a->device && !wait) return;
     DMX_HIP(hipStreamSynchronize(s));
     if (a->device || !tmp.p) return;
     std::vector<char> h(tmp.bytes);
@@ -217,7 +221,57 @@ int validate(const dmx_array &uni, const dmx_array &coord,
 }
 
 hipStream_t pick_stream(const dmx_options &o) {
+  if (o.flags & DMX_STREAM_GIVEN) return (hipStream_t)o.stream;
   return o.stream ? (hipStream_t)o.stream : library_stream();
+}
+
+// Plans of calls that returned before their work finished (DMX_ASYNC): kept,
+// with an event recorded after their last kernel, until that event has
+// completed, so that their buffers go back to the pool only when no kernel
+// can still touch them.  At most MAX_RETIRED are kept; beyond that the
+// oldest is waited for.
+struct Retired {
+  hipEvent_t ev;
+  std::unique_ptr<Plan> plan;
+};
+std::mutex g_retire_mu;
+// never destroyed: plans must not be torn down after the HIP runtime at exit
+std::deque<Retired> &g_retired = *new std::deque<Retired>();
+constexpr size_t MAX_RETIRED = 4;
+
+// frees retired plans whose work has finished; wait_all: wait for all
+void reap(bool wait_all) {
+  std::vector<Retired> done;
+  {
+    std::lock_guard<std::mutex> lk(g_retire_mu);
+    while (!g_retired.empty()) {
+      Retired &r = g_retired.front();
+      const bool wait = wait_all || g_retired.size() > MAX_RETIRED;
+      hipError_t q = wait ? hipEventSynchronize(r.ev) : hipEventQuery(r.ev);
+      if (q == hipErrorNotReady) {
+        (void)hipGetLastError();  // "not ready" is not an error here
+        break;
+      }
+      r.plan->mark_done();
+      done.push_back(std::move(r));
+      g_retired.pop_front();
+    }
+  }
+  for (auto &r : done) {
+    r.plan.reset();
+    (void)hipEventDestroy(r.ev);
+  }
+}
+
+void retire(std::unique_ptr<Plan> plan, hipStream_t s) {
+  hipEvent_t ev;
+  DMX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  DMX_HIP(hipEventRecord(ev, s));
+  {
+    std::lock_guard<std::mutex> lk(g_retire_mu);
+    g_retired.push_back({ev, std::move(plan)});
+  }
+  reap(false);
 }
 
 // Accuracy spot-check of a parameter set: a type-1 transform of random
@@ -316,6 +370,7 @@ int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
                   bool grad = false, const dmx_array *gradout = nullptr) {
   return guard([&] {
     DMX_ASSERT(uni && coord && pts && per, "null argument");
+    reap(false);
     dmx_options opt = opt_in ? *opt_in : defaults();
     const int ndim = validate(*uni, *coord, *pts);
     const size_t M = (size_t)coord->shape[0];
@@ -339,25 +394,31 @@ int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
                           fft_order != 0, vf64, o2);
       }
     }
-    Plan plan(p, s);
-    plan.build_index(cin.p, cf32, M, false);
+    // return before the work finishes only when no host array is staged
+    // (staging buffers die with this call)
+    const bool async = (opt.flags & DMX_ASYNC) && coord->device &&
+                       uni->device && pts->device &&
+                       (!grad || gradout->device);
+    auto plan = std::make_unique<Plan>(p, s);
+    plan->build_index(cin.p, cf32, M, false);
     if (type1) {
       DevIn pin(*pts, s, false);
       DevOut uo(*uni, true);
-      plan.nu2u(forward != 0, pin.p, vf64, uo.p, vf64, uo.st);
-      uo.finish(s);
+      plan->nu2u(forward != 0, pin.p, vf64, uo.p, vf64, uo.st);
+      uo.finish(s, !async);
     } else if (!grad) {
       DevIn ui(*uni, s, true);
       DevOut po(*pts, false);
-      plan.u2nu(forward != 0, ui.p, vf64, ui.st, po.p, vf64);
-      po.finish(s);
+      plan->u2nu(forward != 0, ui.p, vf64, ui.st, po.p, vf64);
+      po.finish(s, !async);
     } else {
       DevIn ui(*uni, s, true);
       DevOut go(*gradout, false);
-      plan.u2nu_grad(forward != 0, ui.p, vf64, ui.st, go.p, vf64);
-      go.finish(s);
+      plan->u2nu_grad(forward != 0, ui.p, vf64, ui.st, go.p, vf64);
+      go.finish(s, !async);
     }
-    if (info) plan.info(info);
+    if (info) plan->info(info);
+    if (async) retire(std::move(plan), s);
   });
 }
 }  // namespace
@@ -369,6 +430,7 @@ void set_error(const char *msg) { g_err = msg; }
 struct dmx_plan {
   std::unique_ptr<dmx::Plan> plan;
   bool nu2u;
+  bool async;  // DMX_ASYNC at creation
   int ndim;
   int64_t shape[3];
   hipStream_t stream;
@@ -609,6 +671,7 @@ int dmx_plan_create(dmx_plan **plan, int nu2u, const dmx_array *coord,
                              fft_order != 0, coord->typecode == DMX_F64, opt);
     auto h = std::make_unique<dmx_plan>();
     h->stream = pick_stream(opt);
+    h->async = (opt.flags & DMX_ASYNC) != 0;
     h->nu2u = nu2u != 0;
     h->ndim = (int)ndim;
     for (size_t d = 0; d < 3; ++d) h->shape[d] = d < ndim ? grid_shape[d] : 1;
@@ -638,8 +701,8 @@ int dmx_plan_nu2u(dmx_plan *plan, int forward, size_t verbosity,
     DevIn pin(*points, s, false);
     DevOut uo(*out, true);
     plan->plan->nu2u(forward != 0, pin.p, f64, uo.p, f64, uo.st);
-    uo.finish(s);
-    plan->plan->flush_profile();
+    uo.finish(s, !(plan->async && points->device && out->device));
+    plan->plan->flush_profile(false);
   });
 }
 
@@ -661,8 +724,8 @@ int dmx_plan_u2nu(dmx_plan *plan, int forward, size_t verbosity,
     DevIn gin(*grid, s, true);
     DevOut po(*out, false);
     plan->plan->u2nu(forward != 0, gin.p, f64, gin.st, po.p, f64);
-    po.finish(s);
-    plan->plan->flush_profile();
+    po.finish(s, !(plan->async && grid->device && out->device));
+    plan->plan->flush_profile(false);
   });
 }
 
@@ -774,12 +837,16 @@ int dmx_profile_enable(int on) {
 }
 
 int dmx_profile_reset(void) {
-  return guard([&] { Profiler::get().reset(); });
+  return guard([&] {
+    reap(true);
+    Profiler::get().reset();
+  });
 }
 
 int dmx_profile_read(char *buf, size_t buflen) {
   return guard([&] {
     DMX_ASSERT(buf && buflen, "null argument");
+    reap(true);
     std::string r = Profiler::get().report();
     DMX_ASSERT(r.size() + 1 <= buflen, "profile buffer too small");
     std::memcpy(buf, r.c_str(), r.size() + 1);

@@ -121,12 +121,17 @@ class Plan {
   const uint32_t *idx() const { return idx_.as<uint32_t>(); }
   uint32_t *keys_copy_to(uint32_t *dev_keys_out);  // re-fold keys
   void info(dmx_plan_info *o) const;
+  // the caller has seen all work queued so far complete (the destructor then
+  // does not wait for the stream)
+  void mark_done() { done_ = true; }
   const Params &params() const { return p_; }
   hipStream_t stream() const { return s_; }
   size_t npoints() const { return M_; }
   Geo geo() const { return g_; }
   // resolves recorded phase events (after the stream has been synchronised)
-  void flush_profile();
+  // moves finished phase timings to the profiler; wait = false stops at
+  // the first phase still running instead of waiting for it
+  void flush_profile(bool wait = true);
 
   struct Phase {
     Plan *p;
@@ -142,6 +147,7 @@ class Plan {
     hipEvent_t a, b;
   };
   std::vector<Pending> pend_;
+  bool done_ = false;
   std::string tag_ = "plan";  // "index", "nu2u" or "u2nu": phase prefix
   void prepare_grid_side();
   void run_fft(bool forward, bool type1);

@@ -116,8 +116,14 @@ Plan::Phase::~Phase() {
   p->pend_.push_back({name, a, b});
 }
 
-void Plan::flush_profile() {
-  for (auto &q : pend_) {
+void Plan::flush_profile(bool wait) {
+  size_t n = 0;
+  for (; n < pend_.size(); ++n) {
+    auto &q = pend_[n];
+    if (!wait && hipEventQuery(q.b) != hipSuccess) {
+      (void)hipGetLastError();  // "not ready" is not an error here
+      break;
+    }
     float ms = 0;
     if (hipEventSynchronize(q.b) == hipSuccess &&
         hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess)
@@ -125,7 +131,7 @@ void Plan::flush_profile() {
     (void)hipEventDestroy(q.a);
     (void)hipEventDestroy(q.b);
   }
-  pend_.clear();
+  pend_.erase(pend_.begin(), pend_.begin() + n);
 }
 
 // -------------------------------------------------------- parameter choice
@@ -262,7 +268,7 @@ Plan::Plan(const Params &p, hipStream_t s) : p_(p), s_(s) {
 }
 
 Plan::~Plan() {
-  (void)hipStreamSynchronize(s_);
+  if (!done_) (void)hipStreamSynchronize(s_);
   flush_profile();
 }
 

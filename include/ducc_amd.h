@@ -51,14 +51,23 @@ typedef struct dmx_options {
   int32_t log2tile;  /* 0 automatic, else tile side 2^log2tile */
   int32_t kernel_idx;/* -1 automatic, else force this kernel table row */
   double fp64_below; /* auto policy switch, default 1e-6 */
-  void *stream;      /* hipStream_t to run on; NULL: library stream */
+  void *stream;      /* hipStream_t to run on; NULL: library stream (the
+                        default stream when DMX_STREAM_GIVEN is set) */
   double sigma;      /* >0: force this oversampling factor (with kernel_idx) */
   int32_t spot_check;/* learned choices: verify accuracy once per request
                         signature with an FP64 direct sum, fall back to the
                         analytic choice on failure (default 1; 0 off; 2
                         treats the check as failed, for tests) */
-  int32_t reserved[7];
+  int32_t flags;     /* DMX_STREAM_GIVEN | DMX_ASYNC */
+  int32_t reserved[6];
 } dmx_options;
+
+/* dmx_options.flags */
+#define DMX_STREAM_GIVEN 1 /* run on `stream` even when it is NULL */
+#define DMX_ASYNC 2        /* when every array is in device memory, return
+                              once the work is queued on the stream (results
+                              are ordered on that stream, as torch
+                              operations are); host arrays always wait */
 
 typedef struct dmx_plan dmx_plan;
 
