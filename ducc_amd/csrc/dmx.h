@@ -45,6 +45,20 @@ struct Sub {
 };
 
 constexpr int MAXSUB = 1024;
+// LDS one workgroup may allocate on gfx950 (160 KiB per CU, all of it
+// available to a single workgroup once the kernel opts in)
+constexpr size_t LDS_MAX_BYTES = 163840;
+
+#ifdef __HIPCC__
+// lets `kernel` take more than the default 64 KiB of dynamic LDS
+template <typename K>
+inline void allow_big_lds(K kernel, size_t bytes) {
+  if (bytes <= 65536) return;
+  DMX_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)LDS_MAX_BYTES));
+}
+#endif
 // points per LDS staging round of the spreader: a whole subproblem in 1-D
 // and 2-D (FP32), half of one in FP64, 256 in 3-D where the region itself
 // needs most of the LDS

@@ -15,6 +15,7 @@ void go(const SpreadArgs &a, void *out, int out_f64, hipStream_t s) {
     size_t cells = interp_inner_pad(a.lds_side);
     for (int d = 1; d < NDIM; ++d) cells *= a.lds_side;
     size_t sh = 2 * cells * sizeof(T);
+    allow_big_lds(k_interp<T, NDIM, W, true, MODE>, sh);
     hipLaunchKernelGGL((k_interp<T, NDIM, W, true, MODE>), dim3(nb), dim3(256),
                        sh, s, a, out, out_f64);
   } else {

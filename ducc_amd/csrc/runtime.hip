@@ -152,7 +152,7 @@ size_t lds_bytes(int ndim, int tile, int W, bool fp64) {
   }
   return std::max(spread_lds_bytes(c, ndim, fp64), ci * (fp64 ? 16 : 8));
 }
-constexpr size_t LDS_LIMIT = 65536;
+constexpr size_t LDS_LIMIT = LDS_MAX_BYTES;
 
 // reference limits: nufft.h:225-228, 400, 409 with its own tile sizes
 void reference_limits(const Params &p) {

@@ -20,6 +20,7 @@ void go(const SpreadArgs &a, hipStream_t s) {
     size_t cells = a.lds_side;
     for (int d = 1; d < NDIM; ++d) cells *= a.lds_side;
     size_t sh = spread_lds_bytes(cells, NDIM, sizeof(T) == 8);
+    allow_big_lds(k_spread<T, NDIM, W, true>, sh);
     hipLaunchKernelGGL((k_spread<T, NDIM, W, true>), dim3(nb), dim3(256), sh,
                        s, a);
   } else {
