@@ -130,8 +130,8 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh,
 //      sorted block at global base[digit] + (position - local start).
 __global__ __launch_bounds__(RT) void k_radix_scatter(
     const uint32_t *keys_in, const uint32_t *idx_in, size_t M, int shift,
-    const uint32_t *offs, int nblocks, uint32_t *keys_out,
-    uint32_t *idx_out) {
+    const uint32_t *offs, const uint32_t *offs_chunk, int nblocks,
+    uint32_t *keys_out, uint32_t *idx_out) {
   constexpr int NW = RT / 64;
   __shared__ uint32_t cnt[NW][RB];
   __shared__ uint32_t gbase[RB];
@@ -180,7 +180,9 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
     const uint32_t start = block_exclusive_scan(run, scan_sh, tot);
 #pragma unroll
     for (int w = 0; w < NW; ++w) cnt[w][tid] += start;
-    gbase[tid] = offs[(size_t)tid * nblocks + blockIdx.x] - start;
+    // offsets: scan inside chunks of STILE entries + scanned chunk sums
+    const size_t o = (size_t)tid * nblocks + blockIdx.x;
+    gbase[tid] = offs[o] + offs_chunk[o / STILE] - start;
   }
   __syncthreads();
   const int nvalid = (int)min((size_t)RTILE, M - blk0);
@@ -411,9 +413,17 @@ void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
     if (p > 0 || !hist0_ready)
       hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nblocks), dim3(RT), 0,
                          s, kin, M, 8 * p, hist, (int)nblocks);
-    exclusive_scan_u32(hist, offs, hist_n, nullptr, sw, s);
+    // digit-major offsets: per-chunk scans and the scan of the chunk sums;
+    // the scatter adds the two
+    const size_t nch = (hist_n + STILE - 1) / STILE;
+    uint32_t *bsum = (uint32_t *)sw;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nch), dim3(ST), 0, s,
+                       (const uint32_t *)hist, offs, hist_n, bsum);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(ST), 0, s,
+                       (const uint32_t *)bsum, bsum, nch, nullptr);
     hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nblocks), dim3(RT), 0,
                        s, kin, iin, M, 8 * p, (const uint32_t *)offs,
+                       (const uint32_t *)bsum,
                        (int)nblocks, kout, iout);
     DMX_HIP(hipGetLastError());
     kin = kout;
