@@ -232,7 +232,14 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
         }
         __syncthreads();
         const int run = ((lane & 15) << 2) | (lane >> 4);
-        for (int jr = wave; jr < K; jr += 4) {
+        // 3-D tiles of few points (runs shorter than the four waves): every
+        // wave takes every run and a quarter of each point's W planes, so
+        // all waves work; the planes shift all lanes alike, which keeps
+        // the lanes of a quarter-wave on distinct classes
+        const bool split = NDIM == 3 && K < 4;
+        const int jr0 = split ? 0 : wave, jstep = split ? 1 : 4;
+        const int q00 = split ? wave : 0, qstep = split ? 4 : 1;
+        for (int jr = jr0; jr < K; jr += jstep) {
           if (run * K + jr >= nc) continue;
           const int i = jr * 64 + run;
           const uint32_t o = st.o[i];
@@ -259,24 +266,31 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
                 atomicAdd(&sim[row + q1], (Acc)(fi * w[1][q1]));
               }
             }
-          } else if constexpr (W <= 8) {
-#pragma unroll
-            for (int q0 = 0; q0 < W; ++q0)
-#pragma unroll
-              for (int q1 = 0; q1 < W; ++q1) {
-                const T f01 = w[0][q0] * w[1][q1];
-                spread_row3<T, W>(sre, sim, cell0 + (q0 * S + q1) * S,
-                                  vr * f01, vi * f01, w[2]);
-              }
           } else {
-#pragma unroll 1
-            for (int q0 = 0; q0 < W; ++q0)
+            if (W <= 8 && qstep == 1) {
 #pragma unroll
-              for (int q1 = 0; q1 < W; ++q1) {
-                const T f01 = w[0][q0] * w[1][q1];
-                spread_row3<T, W>(sre, sim, cell0 + (q0 * S + q1) * S,
-                                  vr * f01, vi * f01, w[2]);
+              for (int q0 = 0; q0 < W; ++q0)
+#pragma unroll
+                for (int q1 = 0; q1 < W; ++q1) {
+                  const T f01 = w[0][q0] * w[1][q1];
+                  spread_row3<T, W>(sre, sim, cell0 + (q0 * S + q1) * S,
+                                    vr * f01, vi * f01, w[2]);
+                }
+            } else {
+#pragma unroll 1
+              for (int q0 = q00; q0 < W; q0 += qstep) {
+                T w0 = w[0][0];
+#pragma unroll
+                for (int k = 1; k < W; ++k)
+                  if (k == q0) w0 = w[0][k];
+#pragma unroll
+                for (int q1 = 0; q1 < W; ++q1) {
+                  const T f01 = w0 * w[1][q1];
+                  spread_row3<T, W>(sre, sim, cell0 + (q0 * S + q1) * S,
+                                    vr * f01, vi * f01, w[2]);
+                }
               }
+            }
           }
         }
       }
