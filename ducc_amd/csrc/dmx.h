@@ -226,9 +226,6 @@ void launch_tile_index(const uint32_t *skeys, size_t M, size_t ntiles,
 void launch_gather_coords(const void *coords, int coord_f32, int ndim,
                           const uint32_t *idx, size_t M, void *out,
                           hipStream_t s);
-// dst[p] = src[idx[p]] for elements of 8 or 16 bytes
-void launch_gather_elems(const void *src, int elem_bytes, const uint32_t *idx,
-                         size_t n, void *dst, hipStream_t s);
 void launch_zero(void *p, size_t bytes, hipStream_t s);
 
 // spreading / interpolation (spread.hip, interp.hip)

@@ -326,13 +326,6 @@ __global__ void k_gather_coords(const T *c, int ndim, const uint32_t *idx,
   for (int d = 0; d < ndim; ++d) out[j * ndim + d] = c[s * ndim + d];
 }
 
-template <typename E>
-__global__ void k_gather_elems(const E *src, const uint32_t *idx, size_t n,
-                               E *dst) {
-  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) dst[j] = src[idx[j]];
-}
-
 __global__ void k_zero16(uint4 *p, size_t n16) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -466,19 +459,6 @@ void launch_gather_coords(const void *coords, int coord_f32, int ndim,
     hipLaunchKernelGGL(k_gather_coords<double>, dim3(nblk(M, 256)), dim3(256),
                        0, s, (const double *)coords, ndim, idx, M,
                        (double *)out);
-  DMX_HIP(hipGetLastError());
-}
-
-void launch_gather_elems(const void *src, int elem_bytes, const uint32_t *idx,
-                         size_t n, void *dst, hipStream_t s) {
-  if (!n) return;
-  DMX_ASSERT(elem_bytes == 8 || elem_bytes == 16, "gather element size");
-  if (elem_bytes == 8)
-    hipLaunchKernelGGL(k_gather_elems<uint2>, dim3(nblk(n, 256)), dim3(256),
-                       0, s, (const uint2 *)src, idx, n, (uint2 *)dst);
-  else
-    hipLaunchKernelGGL(k_gather_elems<uint4>, dim3(nblk(n, 256)), dim3(256),
-                       0, s, (const uint4 *)src, idx, n, (uint4 *)dst);
   DMX_HIP(hipGetLastError());
 }
 
