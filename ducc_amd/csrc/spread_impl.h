@@ -13,6 +13,8 @@
 // Per point: FP64 fold (fold_axis), kernel argument x = W-1-2*frac cast to
 // the compute type, W weights per axis by even/odd Horner (kernel_weights).
 #pragma once
+#include <type_traits>
+
 #include "dmx.h"
 
 namespace dmx {
@@ -413,8 +415,23 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
       }
       __syncthreads();
     }
-    for (uint32_t p = sb.start + threadIdx.x; p < sb.end; p += blockDim.x) {
-      uint32_t j = a.idx[p];
+    // 3-D tiles of few points: tpp adjacent lanes share a point, each
+    // summing a 1/tpp share of its W planes (combined by lane shuffles), so
+    // that more of the workgroup works
+    const uint32_t nc = sb.end - sb.start;
+    auto point_loop = [&](auto tpp_c) {
+    constexpr int tpp = decltype(tpp_c)::value;
+    const uint32_t lp = threadIdx.x / tpp;
+    const int part = (int)(threadIdx.x % tpp);
+    const uint32_t pstep = blockDim.x / tpp;
+    for (uint32_t pb = 0; pb < nc; pb += pstep) {
+      const bool valid = tpp == 1 || pb + lp < nc;
+      if (tpp == 1 && pb + lp >= nc) break;
+      const uint32_t p = sb.start + pb + lp;
+      uint32_t j = 0;
+      T sr = 0, si = 0;  // MODE 0: this lane's share of the value
+      if (valid) {
+      j = a.idx[p];
       const size_t cj = a.coords_sorted ? (size_t)p : (size_t)j;
       const T *cf = (const T *)a.coef;
       int i0[3];
@@ -511,7 +528,7 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
           }
         } else {
 #pragma unroll 1
-          for (int c0 = 0; c0 < W; ++c0) {
+          for (int c0 = part; c0 < W; c0 += tpp) {
             T pr = 0, pi = 0;
 #pragma unroll
             for (int c1 = 0; c1 < W; ++c1) {
@@ -531,7 +548,8 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
             ai += pi * w0;
           }
         }
-        store_value<T>(out, j, out_f64, ar, ai);
+        sr = ar;
+        si = ai;
       } else {
         // gradient: sum over cells of g * dw_d * prod_{e != d} w_e, times
         // dx/dcoord_d = -2 * nover_d / period_d
@@ -572,6 +590,28 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
                          gi_[d] * s);
         }
       }
+      }  // valid
+      if constexpr (MODE == 0) {
+        // every lane takes part in the shuffles (tpp consecutive lanes)
+        for (int off = 1; off < tpp; off <<= 1) {
+          sr += __shfl_xor(sr, off);
+          si += __shfl_xor(si, off);
+        }
+        if (valid && part == 0) store_value<T>(out, j, out_f64, sr, si);
+      }
+    }
+    };
+    // (only for W >= 8: the extra loop variants cost registers, and smaller
+    // supports come with larger oversampling, i.e. more points per tile)
+    if constexpr (NDIM == 3 && MODE == 0 && W >= 8) {
+      if (nc <= 64)
+        point_loop(std::integral_constant<int, 4>{});
+      else if (nc <= 128)
+        point_loop(std::integral_constant<int, 2>{});
+      else
+        point_loop(std::integral_constant<int, 1>{});
+    } else {
+      point_loop(std::integral_constant<int, 1>{});
     }
     if (LDS) __syncthreads();
   }
