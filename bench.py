@@ -285,6 +285,18 @@ def main():
             peak=PEAK_LDS_F64_ATOMICS, unit="atomics/s",
             frac=round(ra / PEAK_LDS_F64_ATOMICS, 4))
 
+    # whole-step roofline: per call M*(d*s_c + s_v) + N*s_v +
+    # (2d+2)*N_over*s_v (coordinates and values once; the uniform array once;
+    # the oversampled grid written, read + written per FFT axis, read once)
+    def call_bytes(inf):
+        n_over = int(np.prod(inf["nover"]))
+        return M * (d * 8 + sv) + n * n * sv + (2 * d + 2) * n_over * sv
+    step_b = call_bytes(info1) + call_bytes(info2)
+    step_gbs = step_b * args.steps / t / 1e9
+    step_roofline = dict(alg_bytes_per_step=int(step_b),
+                         achieved=round(step_gbs, 1), unit="GB/s",
+                         frac=round(step_gbs / PEAK_HBM_GBS, 4))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(c_np, p_np, g_np, eps, M,
@@ -310,7 +322,8 @@ def main():
                               ofactor=info2.get("ofactor"),
                               log2tile=info2.get("log2tile"),
                               learned=info2.get("chooser_used"))),
-        roofline=roofline, cpu_baseline=cpu, fp64_plain=fp64,
+        roofline=roofline, step_roofline=step_roofline, cpu_baseline=cpu,
+        fp64_plain=fp64,
         phases_ms_per_step={k: round(v[0] / args.steps, 4)
                             for k, v in prof.items()})
     print(json.dumps(line))
