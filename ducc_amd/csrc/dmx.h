@@ -207,22 +207,27 @@ struct FftAxisPlan;  // defined in fft.hip
 void launch_fold_keys(const Geo &g, const void *coords, size_t M,
                       uint32_t *keys, uint32_t *hist0, hipStream_t s);
 // stable LSD sort of keys < 2^keybits; hist0_ready: the first pass's
-// histogram was already written by launch_fold_keys
+// histogram was already written by launch_fold_keys; keys_out may be NULL.
+// tb_start_c / tb_end (zeroed, one entry per key value, optional): the
+// final pass records each key's [start, end) in the sorted order, start
+// complemented (~start)
 void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
                       uint32_t *idx_out, uint32_t *keys_out, void *work,
                       size_t work_bytes, hipStream_t s,
-                      bool hist0_ready = false);
+                      bool hist0_ready = false, uint32_t *tb_start_c = nullptr,
+                      uint32_t *tb_end = nullptr);
 size_t radix_sort_workspace(size_t M);
 uint32_t *radix_sort_hist0(void *work, size_t M);
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n,
                         uint32_t *total, void *work, hipStream_t s);
 size_t scan_workspace(size_t n);
-// from the sorted tile keys: per-tile [start, end) and point counts, and
-// the subproblem list (runs of at most MAXSUB points of one tile)
-void launch_tile_index(const uint32_t *skeys, size_t M, size_t ntiles,
-                       uint32_t *counts, uint32_t *tstart, uint32_t *tend,
-                       uint32_t *nsub, uint32_t *sub_start, Sub *subs,
-                       uint32_t *nsub_total, void *scanwork, hipStream_t s);
+// from the tile ranges recorded by the sort (start complemented, end 0 for
+// empty tiles): per-tile start (in place) and point counts, and the
+// subproblem list (runs of at most MAXSUB points of one tile)
+void launch_tile_subs(size_t ntiles, uint32_t *counts, uint32_t *tstart,
+                      const uint32_t *tend, uint32_t *nsub,
+                      uint32_t *sub_start, Sub *subs, uint32_t *nsub_total,
+                      void *scanwork, hipStream_t s);
 void launch_gather_coords(const void *coords, int coord_f32, int ndim,
                           const uint32_t *idx, size_t M, void *out,
                           hipStream_t s);

@@ -168,7 +168,7 @@ class Plan {
   std::vector<std::shared_ptr<Buf>> tw_keep_;
   static std::shared_ptr<Setup> setup_for(const Params &p);
   Buf idx_, subs_, nsub_total_, sorted_coords_;
-  Buf tmp_keys_, tmp_skeys_, tmp_tiles_, tmp_work_;
+  Buf tmp_keys_, tmp_tiles_, tmp_work_;
   Buf grid_, scratch_;
   void *cur_ = nullptr;  // buffer holding the grid after the FFT passes
   const void *twiddle_for(int L);

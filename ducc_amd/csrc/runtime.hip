@@ -385,7 +385,6 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
   // stream (the pool may hand freed blocks to other streams)
   size_t ws = std::max(radix_sort_workspace(M), scan_workspace(ntiles_));
   tmp_keys_.reset(M * 4);
-  tmp_skeys_.reset(M * 4);
   tmp_tiles_.reset(ntiles_ * 4 * 5);
   tmp_work_.reset(ws);
   uint32_t *counts = tmp_tiles_.as<uint32_t>(), *tstart = counts + ntiles_,
@@ -393,17 +392,18 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
            *sstart = nsub + ntiles_;
   {
     Phase ph(this, "building index");
+    launch_zero(tstart, ntiles_ * 8, s_);  // tstart and tend, adjacent
     launch_fold_keys(g_, coords, M, tmp_keys_.as<uint32_t>(),
                      radix_sort_hist0(tmp_work_.p, M), s_);
     idx_.reset(M * 4);
     radix_sort_pairs(tmp_keys_.as<uint32_t>(), M, keybits, idx_.as<uint32_t>(),
-                     tmp_skeys_.as<uint32_t>(), tmp_work_.p, ws, s_, true);
+                     nullptr, tmp_work_.p, ws, s_, true, tstart, tend);
     nsub_max_ = std::min(ntiles_, M) + (M + MAXSUB - 1) / MAXSUB;
     subs_.reset(nsub_max_ * sizeof(Sub));
     nsub_total_.reset(4);
-    launch_tile_index(tmp_skeys_.as<uint32_t>(), M, ntiles_, counts, tstart,
-                      tend, nsub, sstart, subs_.as<Sub>(),
-                      nsub_total_.as<uint32_t>(), tmp_work_.p, s_);
+    launch_tile_subs(ntiles_, counts, tstart, tend, nsub, sstart,
+                     subs_.as<Sub>(), nsub_total_.as<uint32_t>(), tmp_work_.p,
+                     s_);
   }
   if (keep_sorted_coords) {
     // a random gather of 1e7 points costs about as much as the gathers it

@@ -131,7 +131,8 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh,
 __global__ __launch_bounds__(RT) void k_radix_scatter(
     const uint32_t *keys_in, const uint32_t *idx_in, size_t M, int shift,
     const uint32_t *offs, const uint32_t *offs_chunk, int nblocks,
-    uint32_t *keys_out, uint32_t *idx_out) {
+    uint32_t *keys_out, uint32_t *idx_out, uint32_t *tb_start_c,
+    uint32_t *tb_end) {
   constexpr int NW = RT / 64;
   __shared__ uint32_t cnt[NW][RB];
   __shared__ uint32_t gbase[RB];
@@ -204,6 +205,14 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
       const uint32_t k = stage[q];
       gpos[i] = gbase[(k >> shift) & 0xFFu] + (uint32_t)q;
       if (keys_out) keys_out[gpos[i]] = k;
+      if (tb_end) {
+        // final pass: equal keys are contiguous here and globally, so the
+        // block's first / last element of a key bound the tile's range
+        // (start kept complemented, so both arrays start at zero)
+        if (q == 0 || stage[q - 1] != k) atomicMax(&tb_start_c[k], ~gpos[i]);
+        if (q == nvalid - 1 || stage[q + 1] != k)
+          atomicMax(&tb_end[k], gpos[i] + 1);
+      }
     }
   }
   __syncthreads();
@@ -276,27 +285,19 @@ __global__ __launch_bounds__(ST) void k_scan_add(uint32_t *out, size_t n,
     if (base + i < n) out[base + i] += a;
 }
 
-// ------------------------------------------------------------ tile bounds
-// first and one-past-last sorted position of every non-empty tile, from the
-// sorted keys (no atomics: hot tiles of clustered points would serialise a
-// histogram); empty tiles keep start = end = 0.
-__global__ void k_tile_bounds(const uint32_t *skeys, size_t M,
-                              uint32_t *tstart, uint32_t *tend) {
-  size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M) return;
-  const uint32_t k = skeys[p];
-  if (p == 0 || skeys[p - 1] != k) tstart[k] = (uint32_t)p;
-  if (p == M - 1 || skeys[p + 1] != k) tend[k] = (uint32_t)(p + 1);
-}
-
 // ------------------------------------------------------------ subproblems
-// points per tile and subproblems per tile (runs of at most MAXSUB points)
-__global__ void k_counts_nsub(const uint32_t *tstart, const uint32_t *tend,
+// tile [start, end) from the sort's final pass (start stored complemented;
+// empty tiles have end 0), points per tile and subproblems per tile (runs
+// of at most MAXSUB points)
+__global__ void k_counts_nsub(uint32_t *tstart, const uint32_t *tend,
                               size_t ntiles, uint32_t *counts,
                               uint32_t *nsub) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
-  const uint32_t c = tend[t] - tstart[t];
+  const uint32_t e = tend[t];
+  const uint32_t st = e ? ~tstart[t] : 0u;
+  tstart[t] = st;
+  const uint32_t c = e - st;
   counts[t] = c;
   nsub[t] = (c + MAXSUB - 1) / MAXSUB;
 }
@@ -386,7 +387,8 @@ uint32_t *radix_sort_hist0(void *work, size_t M) {
 
 void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
                       uint32_t *idx_out, uint32_t *keys_out, void *work,
-                      size_t work_bytes, hipStream_t s, bool hist0_ready) {
+                      size_t work_bytes, hipStream_t s, bool hist0_ready,
+                      uint32_t *tb_start_c, uint32_t *tb_end) {
   if (!M) return;
   DMX_ASSERT(work_bytes >= radix_sort_workspace(M), "sort workspace too small");
   size_t nblocks = (M + RTILE - 1) / RTILE;
@@ -417,29 +419,20 @@ void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
     hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nblocks), dim3(RT), 0,
                        s, kin, iin, M, 8 * p, (const uint32_t *)offs,
                        (const uint32_t *)bsum,
-                       (int)nblocks, kout, iout);
+                       (int)nblocks, kout, iout, last ? tb_start_c : nullptr,
+                       last ? tb_end : nullptr);
     DMX_HIP(hipGetLastError());
     kin = kout;
     iin = iout;
   }
 }
 
-void launch_tile_index(const uint32_t *skeys, size_t M, size_t ntiles,
-                       uint32_t *counts, uint32_t *tstart, uint32_t *tend,
-                       uint32_t *nsub, uint32_t *sub_start, Sub *subs,
-                       uint32_t *nsub_total, void *scanwork, hipStream_t s) {
-  if (tend == tstart + ntiles) {
-    launch_zero(tstart, ntiles * 8, s);
-  } else {
-    launch_zero(tstart, ntiles * 4, s);
-    launch_zero(tend, ntiles * 4, s);
-  }
-  if (M)
-    hipLaunchKernelGGL(k_tile_bounds, dim3(nblk(M, 256)), dim3(256), 0, s,
-                       skeys, M, tstart, tend);
+void launch_tile_subs(size_t ntiles, uint32_t *counts, uint32_t *tstart,
+                      const uint32_t *tend, uint32_t *nsub,
+                      uint32_t *sub_start, Sub *subs, uint32_t *nsub_total,
+                      void *scanwork, hipStream_t s) {
   hipLaunchKernelGGL(k_counts_nsub, dim3(nblk(ntiles, 256)), dim3(256), 0, s,
-                     (const uint32_t *)tstart, (const uint32_t *)tend, ntiles,
-                     counts, nsub);
+                     tstart, tend, ntiles, counts, nsub);
   exclusive_scan_u32(nsub, sub_start, ntiles, nsub_total, scanwork, s);
   hipLaunchKernelGGL(k_fill_subs, dim3(nblk(ntiles, 256)), dim3(256), 0, s,
                      counts, tstart, (const uint32_t *)nsub,
