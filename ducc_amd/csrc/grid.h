@@ -13,15 +13,18 @@ struct CorfacArgs {
 };
 
 // uniform <-> oversampled grid copy with correction.  Axes padded to 3 with
-// n = N = 1; cf* point at n/2+1 correction factors per axis (a single 1.0
-// for padded axes); ust: strides of the uniform array in elements.
+// n = N = 1; cf[d] points at n/2+1 correction factors (a single 1.0 for
+// padded axes); ust: strides of the uniform array in elements.  full[d]
+// (pad only): cover all N[d] grid positions of axis d, writing zero where
+// the mode is not retained, instead of the n[d] retained ones.
 struct GridCopyArgs {
   void *grid;
   void *uni;
   int u_f64, fft_order;
   int n[3], N[3];
+  int full[3];
   int64_t ust[3];
-  const double *cf0, *cf1, *cf2;
+  const double *cf[3];
 };
 
 struct NudftArgs {

@@ -2,6 +2,9 @@
 // and the fused crop + deconvolve of type 1 (nufft.h:1299-1312, 1331-1344,
 // comp_indices at nufft.h:142-151), plus the direct FP64 NUDFT used as the
 // spot-check of the parameter chooser (the sum of test_nufft.py:32-43).
+#include <algorithm>
+#include <type_traits>
+
 #include "dmx.h"
 #include "grid.h"
 
@@ -20,60 +23,122 @@ __global__ void k_corfac(CorfacArgs a, double *out) {
   out[k] = 1.0 / s;
 }
 
-struct AxisMap {
-  int n, N, fft_order;
-};
-
-__device__ __forceinline__ void comp_idx(int idx, const AxisMap &m, int &icf,
-                                         int &iuser, int &igrid) {
-  int h = m.n / 2;
+// Position p along one axis of the copied region -> correction index icf,
+// index into the uniform array iu, grid index ig (comp_indices,
+// nufft.h:142-151: mode idx sits at grid index idx - n/2 mod N and at
+// uniform index idx, or idx + n - n/2 mod n in FFT order).  With `full`, p
+// is the grid index itself and the result says whether a mode sits there.
+__device__ __forceinline__ bool axis_pos(int p, int n, int N, int full,
+                                         int fft_order, int &icf, int &iu,
+                                         int &ig) {
+  const int h = n / 2;
+  int idx;
+  if (full) {
+    ig = p;
+    if (p < n - h) idx = p + h;
+    else if (p >= N - h) idx = p - (N - h);
+    else {
+      // no mode here: in-range indices all the same, as callers may still
+      // form their addresses
+      icf = 0;
+      iu = 0;
+      return false;
+    }
+  } else {
+    idx = p;
+    ig = idx + N - h;
+    if (ig >= N) ig -= N;
+  }
   icf = h - idx;
   if (icf < 0) icf = -icf;
-  int i1 = m.fft_order ? m.n - h + idx : idx;
-  if (i1 >= m.n) i1 -= m.n;
-  iuser = i1;
-  int i2 = m.N - h + idx;
-  if (i2 >= m.N) i2 -= m.N;
-  igrid = i2;
+  iu = fft_order ? idx + n - h : idx;
+  if (iu >= n) iu -= n;
+  return true;
 }
 
-// mode: 0 = crop (grid -> uniform), 1 = pad (uniform -> grid)
-template <typename T, int MODE>
-__global__ void k_grid_copy(GridCopyArgs a) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t n1 = a.n[1], n2 = a.n[2];
-  if (q >= (int64_t)a.n[0] * n1 * n2) return;
-  const int i = (int)(q / (n1 * n2));
-  const int j = (int)((q / n2) % n1);
-  const int k = (int)(q % n2);
-  int c[3], u[3], g[3];
-  AxisMap m0{a.n[0], a.N[0], a.fft_order}, m1{a.n[1], a.N[1], a.fft_order},
-      m2{a.n[2], a.N[2], a.fft_order};
-  comp_idx(i, m0, c[0], u[0], g[0]);
-  comp_idx(j, m1, c[1], u[1], g[1]);
-  comp_idx(k, m2, c[2], u[2], g[2]);
-  double f = a.cf0[c[0]] * a.cf1[c[1]] * a.cf2[c[2]];
-  const int64_t gi = ((int64_t)g[0] * a.N[1] + g[1]) * a.N[2] + g[2];
-  const int64_t ui = (int64_t)u[0] * a.ust[0] + (int64_t)u[1] * a.ust[1] +
-                     (int64_t)u[2] * a.ust[2];
-  T *grid = (T *)a.grid;
-  if (MODE == 0) {
-    T re = grid[2 * gi] * (T)f, im = grid[2 * gi + 1] * (T)f;
-    if (a.u_f64)
-      ((double2 *)a.uni)[ui] = make_double2((double)re, (double)im);
-    else
-      ((float2 *)a.uni)[ui] = make_float2((float)re, (float)im);
-  } else {
-    T re, im;
-    if (a.u_f64) {
-      double2 x = ((const double2 *)a.uni)[ui];
-      re = (T)x.x; im = (T)x.y;
-    } else {
-      float2 x = ((const float2 *)a.uni)[ui];
-      re = (T)x.x; im = (T)x.y;
+constexpr int GC_EL = 4;  // elements per lane, 256 apart
+
+// uniform element <-> compute-type complex
+template <typename T, typename U>
+__device__ __forceinline__ void load_uni(const void *uni, int64_t i, T &re,
+                                         T &im) {
+  const U x = ((const U *)uni)[i];
+  re = (T)x.x;
+  im = (T)x.y;
+}
+
+// mode: 0 = crop (grid -> uniform), 1 = pad (uniform -> grid).  U: the
+// uniform array's complex type.  One workgroup per (row of the first two
+// axes, run of 256 * GC_EL along the last axis) of the region: the first two
+// axes' index math is done once per row, lanes take consecutive elements
+// of the last axis, and each lane issues its GC_EL loads before its stores.
+// Axes are right-aligned (padded axes first).
+template <typename T, typename U, int MODE>
+__global__ __launch_bounds__(256) void k_grid_copy(GridCopyArgs a,
+                                                   uint32_t nblocks,
+                                                   uint32_t nchunk) {
+  using C = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+  const uint32_t R1 = (uint32_t)(a.full[1] ? a.N[1] : a.n[1]);
+  const int R2 = a.full[2] ? a.N[2] : a.n[2];
+  C *grid = (C *)a.grid;
+  for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const uint32_t row = b / nchunk;
+    const int base = (int)(b - row * nchunk) * (256 * GC_EL) + (int)threadIdx.x;
+    const uint32_t p0 = row / R1, p1 = row - p0 * R1;
+    int c0, u0, g0, c1, u1, g1;
+    bool keep01 =
+        axis_pos((int)p0, a.n[0], a.N[0], a.full[0], a.fft_order, c0, u0, g0);
+    keep01 &=
+        axis_pos((int)p1, a.n[1], a.N[1], a.full[1], a.fft_order, c1, u1, g1);
+    const double f01 = a.cf[0][c0] * a.cf[1][c1];  // c0, c1 always set
+    const int64_t growb = ((int64_t)g0 * a.N[1] + g1) * a.N[2];
+    const int64_t urow = (int64_t)u0 * a.ust[0] + (int64_t)u1 * a.ust[1];
+    T re[GC_EL], im[GC_EL];
+    int64_t gi[GC_EL];
+    bool ok[GC_EL], keep[GC_EL];
+#pragma unroll
+    for (int e = 0; e < GC_EL; ++e) {
+      const int p2 = base + e * 256;
+      ok[e] = p2 < R2;
+      keep[e] = false;
+      gi[e] = 0;
+      re[e] = im[e] = T(0);
+      if (!ok[e]) continue;
+      int c2, u2, g2;
+      // (evaluated even where keep01 is false: g2 is needed for the zero)
+      const bool k2 =
+          axis_pos(p2, a.n[2], a.N[2], a.full[2], a.fft_order, c2, u2, g2);
+      keep[e] = keep01 && k2;
+      gi[e] = growb + g2;
+      if (!keep[e]) continue;
+      const T f = (T)(f01 * a.cf[2][c2]);
+      const int64_t ui = urow + (int64_t)u2 * a.ust[2];
+      if (MODE == 0) {
+        const C g = grid[gi[e]];
+        re[e] = g.x * f;
+        im[e] = g.y * f;
+        gi[e] = ui;  // now the destination
+      } else {
+        load_uni<T, U>(a.uni, ui, re[e], im[e]);
+        re[e] *= f;
+        im[e] *= f;
+      }
     }
-    grid[2 * gi] = re * (T)f;
-    grid[2 * gi + 1] = im * (T)f;
+#pragma unroll
+    for (int e = 0; e < GC_EL; ++e) {
+      if (!ok[e]) continue;
+      if (MODE == 0) {
+        U o;
+        o.x = re[e];
+        o.y = im[e];
+        ((U *)a.uni)[gi[e]] = o;
+      } else {
+        C g;  // zero where no mode sits (re = im = 0)
+        g.x = re[e];
+        g.y = im[e];
+        grid[gi[e]] = g;
+      }
+    }
   }
 }
 
@@ -185,18 +250,40 @@ void launch_corfac(const CorfacArgs &a, double *out, hipStream_t s) {
   DMX_HIP(hipGetLastError());
 }
 
-void launch_grid_copy(bool fp64, bool pad, const GridCopyArgs &a,
+void launch_grid_copy(bool fp64, bool pad, const GridCopyArgs &a0,
                       hipStream_t s) {
-  int64_t n = (int64_t)a.n[0] * a.n[1] * a.n[2];
-  if (!n) return;
-  unsigned nb = (unsigned)((n + 255) / 256);
-  if (fp64) {
-    if (pad) hipLaunchKernelGGL((k_grid_copy<double, 1>), dim3(nb), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_grid_copy<double, 0>), dim3(nb), dim3(256), 0, s, a);
-  } else {
-    if (pad) hipLaunchKernelGGL((k_grid_copy<float, 1>), dim3(nb), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_grid_copy<float, 0>), dim3(nb), dim3(256), 0, s, a);
+  // right-align the axes: padded axes (n = N = 1) come last in the
+  // arguments, the kernel wants the contiguous one last
+  GridCopyArgs a = a0;
+  int shift = 0;
+  while (shift < 2 && a0.N[2 - shift] == 1 && a0.n[2 - shift] == 1) ++shift;
+  for (int d = 0; d < 3; ++d) {
+    const int e = d - shift;
+    a.n[d] = e >= 0 ? a0.n[e] : 1;
+    a.N[d] = e >= 0 ? a0.N[e] : 1;
+    a.full[d] = e >= 0 ? (pad ? a0.full[e] : 0) : 0;
+    a.ust[d] = e >= 0 ? a0.ust[e] : 0;
+    a.cf[d] = e >= 0 ? a0.cf[e] : a0.cf[2];  // a padded axis's single 1.0
   }
+  uint64_t R[3];
+  for (int d = 0; d < 3; ++d) R[d] = (uint64_t)(a.full[d] ? a.N[d] : a.n[d]);
+  if (!R[0] || !R[1] || !R[2]) return;
+  const uint64_t nchunk = (R[2] + 256 * GC_EL - 1) / (256 * GC_EL);
+  const uint64_t nb64 = R[0] * R[1] * nchunk;
+  DMX_ASSERT(nb64 < (1ull << 32), "grid too large for the copy kernel");
+  const uint32_t nblocks = (uint32_t)nb64;
+  const unsigned grid = std::min<uint32_t>(nblocks, 1u << 20);
+#define DMX_GC(T, U, M)                                                    \
+  hipLaunchKernelGGL((k_grid_copy<T, U, M>), dim3(grid), dim3(256), 0, s, a, \
+                     nblocks, (uint32_t)nchunk)
+  if (fp64) {
+    if (a.u_f64) { if (pad) DMX_GC(double, double2, 1); else DMX_GC(double, double2, 0); }
+    else { if (pad) DMX_GC(double, float2, 1); else DMX_GC(double, float2, 0); }
+  } else {
+    if (a.u_f64) { if (pad) DMX_GC(float, double2, 1); else DMX_GC(float, double2, 0); }
+    else { if (pad) DMX_GC(float, float2, 1); else DMX_GC(float, float2, 0); }
+  }
+#undef DMX_GC
   DMX_HIP(hipGetLastError());
 }
 

@@ -12,6 +12,7 @@
 #include "ducc_amd.h"
 #include "dmx.h"
 #include "fft.h"
+#include "grid.h"
 
 namespace dmx {
 
@@ -150,6 +151,14 @@ class Plan {
   bool done_ = false;
   std::string tag_ = "plan";  // "index", "nu2u" or "u2nu": phase prefix
   void prepare_grid_side();
+  // how the FFT along real axis d runs: in LDS, as the two-pass 2-D column
+  // FFT, or as global passes (lines longer than LDS)
+  enum class FftRoute { none, lds, columns, global };
+  FftRoute fft_route(int d) const;
+  GridCopyArgs copy_args(const void *uni, bool uni_f64,
+                         const int64_t *ust) const;
+  // zero-padded, corrected uniform array -> grid, type 2
+  void pad_grid(const void *uni, bool uni_f64, const int64_t *ust);
   void run_fft(bool forward, bool type1);
   void fill_spread_args(SpreadArgs &a) const;
 

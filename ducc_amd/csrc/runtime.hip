@@ -4,8 +4,9 @@
 // Type 1 (nu2u):  zero grid -> spread (LDS tiles, atomic flush) -> pruned
 //                 FFT (last axis over all lines, then each earlier axis over
 //                 the kept bands only) -> crop + deconvolve.
-// Type 2 (u2nu):  zero grid -> deconvolve + pad -> pruned FFT in reverse
-//                 order -> interpolate (LDS tiles).
+// Type 2 (u2nu):  deconvolve + pad (zeros only where an FFT reads whole
+//                 lines) -> pruned FFT in reverse order -> interpolate (LDS
+//                 tiles).
 // (nufft.h:1274-1362, 836-890, 1731-1860.)
 #include "runtime.h"
 
@@ -456,6 +457,53 @@ void Plan::prepare_grid_side() {
   cur_ = grid_.p;
 }
 
+GridCopyArgs Plan::copy_args(const void *uni, bool uni_f64,
+                             const int64_t *ust) const {
+  GridCopyArgs c{};
+  c.uni = const_cast<void *>(uni);
+  c.u_f64 = uni_f64 ? 1 : 0;
+  c.fft_order = p_.fft_order ? 1 : 0;
+  for (int d = 0; d < 3; ++d) {
+    c.n[d] = d < p_.ndim ? (int)p_.nuni[d] : 1;
+    c.N[d] = d < p_.ndim ? (int)p_.nover[d] : 1;
+    c.ust[d] = d < p_.ndim ? ust[d] : 0;
+    if (d >= p_.ndim) {
+      c.cf[d] = su_->one.as<double>();
+      continue;
+    }
+    int e = d;
+    while (!su_->corfac[e].p) --e;  // reused from the previous axis
+    c.cf[d] = su_->corfac[e].as<double>();
+  }
+  return c;
+}
+
+Plan::FftRoute Plan::fft_route(int d) const {
+  const int L = (int)p_.nover[d];
+  if (L <= 1) return FftRoute::none;
+  const int maxel = p_.fp64 ? FFT_MAXEL64 : FFT_MAXEL32;
+  // 2-D columns too long for >= 16 per workgroup: two coalesced passes
+  if (p_.ndim == 2 && d == 0 && maxel / L < 16 && fft_split(L) > 0 &&
+      scratch_.p)
+    return FftRoute::columns;
+  return L <= maxel ? FftRoute::lds : FftRoute::global;
+}
+
+void Plan::pad_grid(const void *uni, bool uni_f64, const int64_t *ust) {
+  GridCopyArgs c = copy_args(uni, uni_f64, ust);
+  c.grid = grid_.p;
+  // An FFT in LDS reads the retained modes of its axis only (the rest is
+  // known to be zero), so along such an axis the retained rows suffice.
+  // An FFT of whole lines reads every position: up to the last axis done
+  // that way, every grid position is written, zero outside the modes.
+  int last_whole = -1;
+  for (int d = 0; d < p_.ndim; ++d)
+    if (fft_route(d) != FftRoute::lds) last_whole = d;
+  for (int d = 0; d <= last_whole; ++d) c.full[d] = 1;
+  Phase ph(this, "grid correction");
+  launch_grid_copy(p_.fp64, true, c, s_);
+}
+
 void Plan::run_fft(bool forward, bool type1) {
   const int nd = p_.ndim, f = 3 - nd;
   int64_t E[3], st[3];
@@ -485,7 +533,6 @@ void Plan::run_fft(bool forward, bool type1) {
   if (type1) for (int a = 2; a >= f; --a) order.push_back(a);
   else for (int a = f; a <= 2; ++a) order.push_back(a);
   const int maxel = p_.fp64 ? FFT_MAXEL64 : FFT_MAXEL32;
-  bool cols_moved = false;  // the 2-D column pass left the grid in scratch
   for (int a : order) {
     int pax = a == 0 ? 1 : 0, qax = a == 2 ? 1 : 2;
     FftLaunch L{};
@@ -499,24 +546,22 @@ void Plan::run_fft(bool forward, bool type1) {
     range(qax, qax > a, L.nv, L.vn1, L.va1, L.va2);
     if (L.L <= 1) continue;
     Phase ph(this, std::string("FFT axis ") + std::to_string(a - f));
-    // 2-D columns too long for >= 16 per workgroup: two coalesced passes
-    // (64 columns per workgroup), the second writing natural order into
-    // the other buffer
-    const int L1 = fft_split(L.L);
-    if (nd == 2 && L.st_ax > 1 && maxel / L.L < 16 && L1 > 0 &&
-        scratch_.p) {
+    const FftRoute route = fft_route(a - f);
+    if (route == FftRoute::columns) {
+      // two coalesced passes (64 columns per workgroup), the second writing
+      // natural order into the other buffer
       void *other = cur_ == grid_.p ? scratch_.p : grid_.p;
       // rows longer than LDS cannot skip the unwritten columns: clear them
       if (!type1 && E[2] > maxel) launch_zero(other, grid_.bytes, s_);
+      const int L1 = fft_split(L.L);
       fft_columns_2pass(L, L1, other, p_.fp64, twiddle_for(L1),
                         twiddle_for(L.L / L1), twiddle_for(L.L), s_);
       cur_ = other;
-      cols_moved = true;
       continue;
     }
-    if (cols_moved && L.st_ax == 1) {
-      // rows after the column pass of type 2: only the kept columns were
-      // written; the others are zero by construction
+    if (!type1 && route == FftRoute::lds) {
+      // type 2: along this axis only the retained modes were written (by
+      // the pad or by the FFTs of the earlier axes); the rest is zero
       L.kprune = 1;
       L.kn1 = (n_act[a] + 1) / 2;
       L.ka2 = N_act[a] - n_act[a] / 2;
@@ -544,24 +589,8 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
     launch_spread(p_.fp64, a, s_);
   }
   run_fft(forward, true);
-  GridCopyArgs c{};
+  GridCopyArgs c = copy_args(uni, uni_f64, ust);
   c.grid = cur_;
-  c.uni = uni;
-  c.u_f64 = uni_f64 ? 1 : 0;
-  c.fft_order = p_.fft_order ? 1 : 0;
-  for (int d = 0; d < 3; ++d) {
-    c.n[d] = d < p_.ndim ? (int)p_.nuni[d] : 1;
-    c.N[d] = d < p_.ndim ? (int)p_.nover[d] : 1;
-    c.ust[d] = d < p_.ndim ? ust[d] : 0;
-  }
-  const double *cf[3];
-  for (int d = 0; d < 3; ++d) {
-    if (d >= p_.ndim) { cf[d] = su_->one.as<double>(); continue; }
-    int e = d;
-    while (!su_->corfac[e].p) --e;  // reused from the previous axis
-    cf[d] = su_->corfac[e].as<double>();
-  }
-  c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
   Phase ph(this, "grid correction");
   launch_grid_copy(p_.fp64, false, c, s_);
 }
@@ -570,32 +599,7 @@ void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
                 const int64_t *ust, void *out, bool out_f64) {
   tag_ = "u2nu";
   prepare_grid_side();
-  {
-    Phase ph(this, "zeroing grid");
-    launch_zero(grid_.p, grid_.bytes, s_);
-  }
-  GridCopyArgs c{};
-  c.grid = grid_.p;
-  c.uni = const_cast<void *>(uni);
-  c.u_f64 = uni_f64 ? 1 : 0;
-  c.fft_order = p_.fft_order ? 1 : 0;
-  for (int d = 0; d < 3; ++d) {
-    c.n[d] = d < p_.ndim ? (int)p_.nuni[d] : 1;
-    c.N[d] = d < p_.ndim ? (int)p_.nover[d] : 1;
-    c.ust[d] = d < p_.ndim ? ust[d] : 0;
-  }
-  const double *cf[3];
-  for (int d = 0; d < 3; ++d) {
-    if (d >= p_.ndim) { cf[d] = su_->one.as<double>(); continue; }
-    int e = d;
-    while (!su_->corfac[e].p) --e;
-    cf[d] = su_->corfac[e].as<double>();
-  }
-  c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
-  {
-    Phase ph(this, "grid correction");
-    launch_grid_copy(p_.fp64, true, c, s_);
-  }
+  pad_grid(uni, uni_f64, ust);
   run_fft(forward, false);
   if (M_ > 0) {
     SpreadArgs a{};
@@ -610,32 +614,7 @@ void Plan::u2nu_grad(bool forward, const void *uni, bool uni_f64,
                      const int64_t *ust, void *out, bool out_f64) {
   tag_ = "u2nu_grad";
   prepare_grid_side();
-  {
-    Phase ph(this, "zeroing grid");
-    launch_zero(grid_.p, grid_.bytes, s_);
-  }
-  GridCopyArgs c{};
-  c.grid = grid_.p;
-  c.uni = const_cast<void *>(uni);
-  c.u_f64 = uni_f64 ? 1 : 0;
-  c.fft_order = p_.fft_order ? 1 : 0;
-  for (int d = 0; d < 3; ++d) {
-    c.n[d] = d < p_.ndim ? (int)p_.nuni[d] : 1;
-    c.N[d] = d < p_.ndim ? (int)p_.nover[d] : 1;
-    c.ust[d] = d < p_.ndim ? ust[d] : 0;
-  }
-  const double *cf[3];
-  for (int d = 0; d < 3; ++d) {
-    if (d >= p_.ndim) { cf[d] = su_->one.as<double>(); continue; }
-    int e = d;
-    while (!su_->corfac[e].p) --e;
-    cf[d] = su_->corfac[e].as<double>();
-  }
-  c.cf0 = cf[0]; c.cf1 = cf[1]; c.cf2 = cf[2];
-  {
-    Phase ph(this, "grid correction");
-    launch_grid_copy(p_.fp64, true, c, s_);
-  }
+  pad_grid(uni, uni_f64, ust);
   run_fft(forward, false);
   if (M_ > 0) {
     SpreadArgs a{};

@@ -133,6 +133,34 @@ def test_plan_matches_unplanned(gpu, shape):
     assert O.l2error(m1, m2) < 1e-13
 
 
+@pmp("shape,single,fft_order", [
+    ((300,), False, False), ((9000,), False, True), ((9000,), True, False),
+    ((48, 40), False, True), ((250, 257), False, False),
+    ((250, 257), True, True), ((64, 5000), False, False),
+    ((16, 18, 20), False, False), ((16, 18, 20), True, True)])
+def test_u2nu_ignores_stale_grid(gpu, shape, single, fft_order):
+    """Type 2 writes only the grid region its FFTs read (retained modes
+    along axes whose FFT runs in LDS, zeros elsewhere): a grid buffer that
+    still holds a type-1 result must not change the output."""
+    from ducc_amd import nufft
+    rng = np.random.default_rng(11)
+    c, ms, g = data(rng, shape, 4000, 2 * np.pi, single)
+    if single:
+        c = c.astype(np.float32)
+    eps = 1e-5 if single else 1e-10
+    p = nufft.plan(nu2u=False, coord=c, grid_shape=shape, epsilon=eps,
+                   fft_order=fft_order)
+    m1 = p.u2nu(grid=g, forward=True)
+    p.nu2u(points=ms, forward=False)
+    m2 = p.u2nu(grid=g, forward=True)
+    assert np.array_equal(m1, m2)
+    gg = np.fft.fftshift(g) if fft_order else g
+    pidx = rng.choice(4000, 300, replace=False)
+    ref = O.nudft_type2(c.astype(np.float64), gg.astype(np.complex128), True,
+                        pidx=pidx)
+    assert O.l2error(m2[pidx], ref) <= TOLMUL[len(shape)] * eps
+
+
 def test_coord_and_value_dtype_matrix(gpu):
     """coords {f64,f32} x values {c128,c64} (nufft_pymod.cc:70-94)"""
     from ducc_amd import nufft
