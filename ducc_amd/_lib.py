@@ -98,6 +98,8 @@ _SIG = {
     "dmx_profile_enable": (ct.c_int, [ct.c_int]),
     "dmx_profile_reset": (ct.c_int, []),
     "dmx_profile_read": (ct.c_int, [ct.c_char_p, ct.c_size_t]),
+    "dmx_c2c": (ct.c_int, [_AP, _AP, _P, ct.c_size_t, ct.c_int, ct.c_double,
+                           ct.c_size_t, ct.POINTER(DmxOptions)]),
     "dmx_device_count": (ct.c_int, []),
 }
 

@@ -17,6 +17,7 @@
 
 #include "chooser.h"
 #include "grid.h"
+#include "c2c.h"
 #include "mlp.h"
 #include "params.h"
 #include "runtime.h"
@@ -233,6 +234,7 @@ hipStream_t pick_stream(const dmx_options &o) {
 struct Retired {
   hipEvent_t ev;
   std::unique_ptr<Plan> plan;
+  std::shared_ptr<void> keep;  // buffers of a call without a plan
 };
 std::mutex g_retire_mu;
 // never destroyed: plans must not be torn down after the HIP runtime at exit
@@ -252,24 +254,26 @@ void reap(bool wait_all) {
         (void)hipGetLastError();  // "not ready" is not an error here
         break;
       }
-      r.plan->mark_done();
+      if (r.plan) r.plan->mark_done();
       done.push_back(std::move(r));
       g_retired.pop_front();
     }
   }
   for (auto &r : done) {
     r.plan.reset();
+    r.keep.reset();
     (void)hipEventDestroy(r.ev);
   }
 }
 
-void retire(std::unique_ptr<Plan> plan, hipStream_t s) {
+void retire(std::unique_ptr<Plan> plan, hipStream_t s,
+            std::shared_ptr<void> keep = nullptr) {
   hipEvent_t ev;
   DMX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   DMX_HIP(hipEventRecord(ev, s));
   {
     std::lock_guard<std::mutex> lk(g_retire_mu);
-    g_retired.push_back({ev, std::move(plan)});
+    g_retired.push_back({ev, std::move(plan), std::move(keep)});
   }
   reap(false);
 }
@@ -850,6 +854,44 @@ int dmx_profile_read(char *buf, size_t buflen) {
     std::string r = Profiler::get().report();
     DMX_ASSERT(r.size() + 1 <= buflen, "profile buffer too small");
     std::memcpy(buf, r.c_str(), r.size() + 1);
+  });
+}
+
+int dmx_c2c(const dmx_array *in, const dmx_array *out, const int64_t *axes,
+            size_t naxes, int forward, double fct, size_t nthreads,
+            const dmx_options *opt_in) {
+  (void)nthreads;
+  return guard([&] {
+    DMX_ASSERT(in && out && (axes || !naxes), "null argument");
+    reap(false);
+    dmx_options opt = opt_in ? *opt_in : defaults();
+    const int ndim = (int)in->ndim;
+    DMX_ASSERT(ndim >= 1 && ndim <= 10, "bad array dimensionality");
+    DMX_ASSERT(out->ndim == ndim, "dimension mismatch");
+    for (int d = 0; d < ndim; ++d)
+      DMX_ASSERT(out->shape[d] == in->shape[d], "dimension mismatch");
+    const bool fp64 = out->typecode == DMX_C128;
+    DMX_ASSERT(fp64 || out->typecode == DMX_C64, "incorrect data type");
+    DMX_ASSERT(fp64 ? (in->typecode == DMX_C128 || in->typecode == DMX_F64)
+                    : (in->typecode == DMX_C64 || in->typecode == DMX_F32),
+               "incorrect data type");
+    // makeaxes (python/fft_pymod.cc:73-94)
+    DMX_ASSERT(naxes >= 1 && naxes <= (size_t)ndim, "bad axes argument");
+    int ax[10];
+    for (size_t q = 0; q < naxes; ++q) {
+      int64_t a = axes[q];
+      if (a < 0) a += ndim;
+      DMX_ASSERT(a >= 0 && a < ndim, "axes exceeds dimensionality of output");
+      ax[q] = (int)a;
+    }
+    hipStream_t s = pick_stream(opt);
+    const bool async = (opt.flags & DMX_ASYNC) && in->device && out->device;
+    DevIn din(*in, s, true);
+    DevOut dout(*out, true);
+    auto job = c2c_exec(din.p, in->typecode, din.st, dout.p, dout.st, ndim,
+                        in->shape, ax, (int)naxes, forward != 0, fct, s);
+    dout.finish(s, !async);
+    if (async) retire(nullptr, s, job);
   });
 }
 

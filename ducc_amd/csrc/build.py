@@ -17,7 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 GEN = os.path.join(HERE, "_gen")
 
 SOURCES = ["params.cpp", "runtime.hip", "api.hip", "sort.hip", "spread.hip",
-           "interp.hip", "fft.hip", "grid.hip", "chooser.hip", "mlp.hip"]
+           "interp.hip", "fft.hip", "grid.hip", "chooser.hip", "mlp.hip",
+           "c2c.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DMX_ARCH", "gfx950")

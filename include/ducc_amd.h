@@ -151,6 +151,22 @@ int dmx_sort_points(const dmx_array *coord, const int64_t *nover,
                     const double *periodicity, dmx_array *keys,
                     dmx_array *idx);
 
+/* Complex FFT along chosen axes: replaces ducc0::c2c(cfmav in, vfmav out,
+   shape_t axes, bool forward, T fct, size_t nthreads)
+   (src/ducc0/fft/fft.h:848-850), as bound by ducc0.fft.c2c
+   (python/fft_pymod.cc:122-172, 164-172).  out = fct * DFT(in) over
+   axes[0..naxes) (negative axes count from the end; an axis named twice is
+   transformed twice), with exponent sign -1 if forward.  in: complex64 /
+   complex128, or float32 / float64 (read as complex with zero imaginary
+   part); out: complex of the same precision and shape, any strides, host or
+   device; out may be in itself (identical strides).  Any length: 2,3,5,7,
+   11-smooth lengths run as Stockham passes, others by Bluestein's algorithm.
+   nthreads is accepted and ignored.  opt: stream and DMX_ASYNC as for the
+   transforms (NULL for defaults). */
+int dmx_c2c(const dmx_array *in, const dmx_array *out, const int64_t *axes,
+            size_t naxes, int forward, double fct, size_t nthreads,
+            const dmx_options *opt);
+
 /* ---- learned parameter chooser (no ducc0 counterpart; its fallback and
    label judge is ducc0's findNufftParameters, nufft.h:155-208) ----
    Network: 6 features -> 64 -> 64 -> 64 (ReLU) -> 3 outputs (oversampling
