@@ -12,7 +12,8 @@ is timed on the same geometry for the speed-up, and ducc0 itself (built from
 source into oracle/_ref) is timed on the host cores as the CPU baseline.
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K]
-[--warmup W]
+[--warmup W]; `--workload c2c` measures the ducc0.fft.c2c drop-in instead
+(see bench_c2c).
 """
 import argparse
 import json
@@ -130,9 +131,123 @@ def cpu_baseline(coords, pts, grid, eps, M_total, fast=None):
                         f"nu2u {t1 - t0:.3f} s, u2nu {t2 - t1:.3f} s"))
 
 
+def c2c_cpu(R, a, nthreads, reps):
+    """ducc0::c2c (oracle/_ref) on the host: mean seconds of `reps` calls"""
+    R.c2c(a, nthreads=nthreads)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = R.c2c(a, nthreads=nthreads)
+    return (time.perf_counter() - t0) / reps, out
+
+
+def bench_c2c(args):
+    """ducc0.fft.c2c drop-in: in-place forward c2c of a 4096 x 4096
+    complex64 array (the headline of this mode, HBM-bound: each transformed
+    axis reads and writes the array once, 2 * N * 8 bytes per axis), plus the
+    reference's own FFT benchmark shapes (python/demos/fft_bench.py:127-170:
+    random good_size extents in [nmax/3, nmax], nmax 8192 / 2048 / 256 for
+    1-D / 2-D / 3-D, complex128 and complex64, in place), each timed on the
+    GPU and with ducc0 itself on one host thread (as fft_bench.py runs it)."""
+    import torch
+    from ducc_amd import _lib, fft
+    try:
+        from oracle import ref as R
+        have_ref = R.available() and not args.no_cpu
+    except Exception:
+        have_ref = False
+    dev = torch.device("cuda:0")
+    sync = torch.cuda.synchronize
+    shp = (4096, 4096)
+    N = shp[0] * shp[1]
+    g = torch.Generator(device="cuda").manual_seed(21)
+    x = torch.randn(shp, generator=g, dtype=torch.complex64, device=dev)
+    work = x.clone()
+
+    def step():
+        fft.c2c(work, out=work, forward=True)
+    t = time_steps(step, args.steps, args.warmup, sync)
+    ms = 1e3 * t / args.steps
+    _lib.profile_enable(True)
+    _lib.profile_reset()
+    for _ in range(args.steps):
+        step()
+    sync()
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    per = {k: v[0] / v[1] for k, v in prof.items()}
+    alg = 2 * N * 8  # per axis: read + write of the array
+    axis_ms = {k: v for k, v in per.items() if "FFT axis" in k}
+    dom = max(axis_ms, key=axis_ms.get)
+    ach = alg / (axis_ms[dom] * 1e-3) / 1e9
+    value = 2 * alg / (ms * 1e-3) / 1e9
+    # accuracy against numpy's FP64 FFT of the same input
+    xin = x.cpu().numpy()
+    work.copy_(x)
+    step()
+    got = work.cpu().numpy()
+    ref64 = np.fft.fft2(xin.astype(np.complex128))
+    err = l2rel(got, ref64)
+    cpu = None
+    if have_ref:
+        nth = os.cpu_count() or 1
+        tc, rc = c2c_cpu(R, xin, nth, 3)
+        cpu = dict(value=round(2 * alg / tc / 1e9, 2), unit="GB/s",
+                   cores=nth, kind="reference",
+                   err_gpu_vs_ducc0=l2rel(got, rc),
+                   sample=f"ducc0 c2c of the same 4096x4096 complex64 array, "
+                          f"{nth} threads, mean of 3 calls after a warm-up: "
+                          f"{1e3 * tc:.2f} ms")
+    # the reference's benchmark shapes
+    rng = np.random.default_rng(17)
+    rows = []
+    for ndim, nmax in ((1, 8192), (2, 2048), (3, 256)):
+        for tp in (np.complex128, np.complex64):
+            for _ in range(3):
+                sh = tuple(fft.good_size(int(v)) for v in
+                           rng.integers(nmax // 3, nmax + 1, ndim))
+                a = (rng.random(sh) - 0.5 + 1j * (rng.random(sh) - 0.5)) \
+                    .astype(tp)
+                at = torch.from_numpy(a).to(dev)
+                w = at.clone()
+
+                def st():
+                    fft.c2c(w, out=w, forward=True)
+                tg = time_steps(st, 10, 2, sync) / 10
+                w.copy_(at)
+                st()
+                row = dict(shape=list(sh), dtype=np.dtype(tp).name,
+                           gpu_ms=round(1e3 * tg, 4))
+                if have_ref:
+                    tcpu, rr = c2c_cpu(R, a, 1, 3)
+                    row.update(ducc0_1thread_ms=round(1e3 * tcpu, 4),
+                               speedup=round(tcpu / tg, 1),
+                               l2err_vs_ducc0=l2rel(w.cpu().numpy(), rr))
+                rows.append(row)
+    line = dict(
+        metric="c2c GB/s (algorithmic: read + write of the array per "
+               "transformed axis), 4096x4096 complex64, forward, in place",
+        value=round(value, 1), unit="GB/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=round(ms, 4), higher_is_better=True,
+        scaling="weak", vs_baseline=None, dtype="fp32", data="synthetic",
+        config=dict(workload="ducc0.fft.c2c, 2-D 4096x4096 complex64, "
+                             "forward, in place on the device"),
+        roofline=dict(bound="hbm", kernel=f"c2c/{dom}",
+                      achieved=round(ach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                      frac=round(ach / PEAK_HBM_GBS, 4), traffic=None,
+                      alg_bytes_per_launch=alg,
+                      avg_launch_ms=round(axis_ms[dom], 4)),
+        cpu_baseline=cpu, err_vs_numpy_fp64=err,
+        phases_ms={k: round(v, 4) for k, v in per.items()},
+        fft_bench_shapes=rows)
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="nufft", choices=["nufft", "c2c"],
+                    help="nufft: the 2-D type-1 + type-2 workload (default); "
+                         "c2c: the ducc0.fft.c2c drop-in")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--M", type=int, default=10_000_000)
@@ -152,6 +267,9 @@ def main():
               "--gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c2c":
+        bench_c2c(args)
+        return
     import torch
     from ducc_amd import _lib, nufft
     from ducc_amd import chooser as learned

@@ -332,9 +332,67 @@ void run_lines(void *data, int64_t outer, int L, int64_t inner, bool fp64,
   }
 }
 
+// the two-pass column FFT of (outer, L, inner) along its middle axis:
+// pass A data -> scratch, pass B scratch -> data (same layout)
+void run_columns(void *data, int64_t outer, int L, int L1, int64_t inner,
+                 bool fp64, int sign, const void *tw1, const void *tw2,
+                 const void *tw, void *scratch, hipStream_t s) {
+  const size_t esz = fp64 ? 16 : 8;
+  DMX_ASSERT(inner <= 0x7fffffff, "FFT line set too large");
+  const int64_t chunk = 65535;
+  for (int64_t u0 = 0; u0 < outer; u0 += chunk) {
+    const int64_t off = u0 * (int64_t)L * inner;
+    FftLaunch p{};
+    p.L = L;
+    p.sign = sign;
+    p.data = (char *)data + off * esz;
+    p.st_ax = inner;
+    p.nu = (int)std::min<int64_t>(chunk, outer - u0);
+    p.un1 = p.nu;
+    p.st_u = (int64_t)L * inner;
+    p.nv = (int)inner;
+    p.vn1 = p.nv;
+    p.st_v = 1;
+    const bool ok = fft_columns_2pass(p, L1, p.data, fp64, tw1, tw2, tw, s,
+                                      (char *)scratch + off * esz);
+    DMX_ASSERT(ok, "internal: column FFT split rejected");
+  }
+}
+
 struct Job {
   std::vector<std::shared_ptr<Buf>> keep;
   Buf work, scratch, blue, blue_scratch;
+};
+
+// Phase timers (profiling only): HIP events on the call's stream, read back
+// at the end of the call.
+struct Timers {
+  bool on;
+  hipStream_t s;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  explicit Timers(hipStream_t st) : on(Profiler::get().enabled()), s(st) {}
+  void begin(const std::string &name) {
+    if (!on) return;
+    hipEvent_t a, b;
+    DMX_HIP(hipEventCreate(&a));
+    DMX_HIP(hipEventCreate(&b));
+    DMX_HIP(hipEventRecord(a, s));
+    ev.push_back({name, {a, b}});
+  }
+  void end() {
+    if (on) DMX_HIP(hipEventRecord(ev.back().second.second, s));
+  }
+  ~Timers() {
+    for (auto &e : ev) {
+      float ms = 0;
+      if (hipEventSynchronize(e.second.second) == hipSuccess &&
+          hipEventElapsedTime(&ms, e.second.first, e.second.second) ==
+              hipSuccess)
+        Profiler::get().add("c2c/" + e.first, ms);
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+  }
 };
 
 template <typename T>
@@ -425,10 +483,19 @@ std::shared_ptr<void> c2c_exec(const void *in, int in_tc, const int64_t *in_st,
     job->work.reset((size_t)n * esz);
     work = job->work.p;
   }
-  if (fp64)
-    copy_nd<double>(in, in_tc, in_st, work, cst, ndim, shape, fct, s);
-  else
-    copy_nd<float>(in, in_tc, in_st, work, cst, ndim, shape, fct, s);
+  Timers tm(s);
+  // in place on a contiguous array without scaling: nothing to copy
+  bool in_contig = in == work && in_tc == (fp64 ? DMX_C128 : DMX_C64);
+  for (int d = 0; d < ndim; ++d)
+    if (shape[d] != 1 && in_st[d] != cst[d]) in_contig = false;
+  if (!(in_contig && fct == 1.0)) {
+    tm.begin("copy in");
+    if (fp64)
+      copy_nd<double>(in, in_tc, in_st, work, cst, ndim, shape, fct, s);
+    else
+      copy_nd<float>(in, in_tc, in_st, work, cst, ndim, shape, fct, s);
+    tm.end();
+  }
   const int sign = forward ? -1 : 1;
   const int maxel = fp64 ? FFT_MAXEL64 : FFT_MAXEL32;
   for (int q = 0; q < naxes; ++q) {
@@ -441,10 +508,26 @@ std::shared_ptr<void> c2c_exec(const void *in, int in_tc, const int64_t *in_st,
     for (int d = 0; d < ax; ++d) outer *= shape[d];
     for (int d = ax + 1; d < ndim; ++d) inner *= shape[d];
     if (is_smooth(L)) {
-      if (L > maxel && !job->scratch.p) job->scratch.reset((size_t)n * esz);
       auto tw = twiddles(L, fp64);
       job->keep.push_back(tw);
-      run_lines(work, outer, L, inner, fp64, sign, tw->p, job->scratch.p, s);
+      tm.begin("FFT axis " + std::to_string(ax));
+      const int L1 = fft_split(L);
+      if (inner > 1 && maxel / L < 16 && L1 > 0) {
+        // strided lines too long for >= 16 per workgroup: the two coalesced
+        // column passes (64 columns per workgroup), through the scratch
+        // array and back
+        if (!job->scratch.p) job->scratch.reset((size_t)n * esz);
+        auto t1 = twiddles(L1, fp64), t2 = twiddles(L / L1, fp64);
+        job->keep.push_back(t1);
+        job->keep.push_back(t2);
+        run_columns(work, outer, L, L1, inner, fp64, sign, t1->p, t2->p,
+                    tw->p, job->scratch.p, s);
+      } else {
+        if (L > maxel && !job->scratch.p) job->scratch.reset((size_t)n * esz);
+        run_lines(work, outer, L, inner, fp64, sign, tw->p, job->scratch.p,
+                  s);
+      }
+      tm.end();
       continue;
     }
     const int n2 = (int)smooth_length(2 * L64 - 1);
@@ -466,6 +549,7 @@ std::shared_ptr<void> c2c_exec(const void *in, int in_tc, const int64_t *in_st,
     b.n = nb;
     b.small = nb < (1ll << 32) ? 1 : 0;
     void *A = job->blue.p;
+    tm.begin("Bluestein axis " + std::to_string(ax));
     if (fp64) launch_blue<double, 0>(work, A, bt.w->p, b, s);
     else launch_blue<float, 0>(work, A, bt.w->p, b, s);
     run_lines(A, outer, n2, inner, fp64, -1, tw2->p, job->blue_scratch.p, s);
@@ -475,12 +559,15 @@ std::shared_ptr<void> c2c_exec(const void *in, int in_tc, const int64_t *in_st,
     b.n = outer * (int64_t)L * inner;
     if (fp64) launch_blue<double, 2>(work, A, bt.w->p, b, s);
     else launch_blue<float, 2>(work, A, bt.w->p, b, s);
+    tm.end();
   }
   if (!out_contig) {
+    tm.begin("copy out");
     if (fp64)
       copy_nd<double>(work, DMX_C128, cst, out, out_st, ndim, shape, 1.0, s);
     else
       copy_nd<float>(work, DMX_C64, cst, out, out_st, ndim, shape, 1.0, s);
+    tm.end();
   }
   return job;
 }

@@ -58,13 +58,16 @@ struct FftCols {
 };
 
 // FFT of length L = L1*L2 along a strided axis, as two coalesced passes:
-// A (in place): length-L1 FFTs over rows n1*L2+n2, times w_L^{n2 k1};
-// B (src -> dst): length-L2 FFTs over rows k1*L2+n2, written to row
-// k1 + L1*k2, i.e. in natural order.  `twiddle(len)` returns device
-// twiddles.  Returns false when L has no split worth using.
+// A: length-L1 FFTs over rows n1*L2+n2, times w_L^{n2 k1}, in place (or,
+// with `a_dst`, from p.data into a_dst at the same positions);
+// B: length-L2 FFTs over rows k1*L2+n2 of A's output, written to row
+// k1 + L1*k2 of dst, i.e. in natural order (dst may be p.data when a_dst is
+// given: the result then replaces the input).  Returns false when L has no
+// split worth using.
 bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
                        const void *tw_L1, const void *tw_L2,
-                       const void *tw_L, hipStream_t s);
+                       const void *tw_L, hipStream_t s,
+                       void *a_dst = nullptr);
 // a divisor of L near sqrt(L) with both factors <= 128 (0: none)
 int fft_split(int L);
 

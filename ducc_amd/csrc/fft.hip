@@ -2,7 +2,8 @@
 // grid (role of c2c / general_nd / cfftp passes, fft.h:848-850,
 // fftnd_impl.h:593-, fft1d_impl.h:194-825).  Unnormalised; forward means
 // exponent -1.  Grid sizes are 2*(2,3,5,7,11-smooth), so the radices are
-// 4, 2, 3, 5, 7, 11 (4 first, as the reference factorises).
+// 8, 4, 2, 3, 5, 7, 11 (radix 8, then 4, first, as the reference factorises
+// complex lengths, fft.h:195-217).
 //
 // A "line set" is every line along one axis of a row-major 3-D view
 // (1-D and 2-D grids get leading unit axes); the two other axes index the
@@ -98,6 +99,28 @@ __device__ __forceinline__ void dft(C *v, int sign) {
     v[2] = csub(a0, a2);
     v[1] = cadd(a1, a3);
     v[3] = csub(a1, a3);
+  } else if constexpr (R == 8) {
+    // two length-4 DFTs (even / odd inputs), odd outputs times W^k with
+    // W = exp(sign 2 pi i / 8), then the length-2 combination
+    C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft<T, 4>(e, sign);
+    dft<T, 4>(o, sign);
+    const T h = (T)0.70710678118654752440;
+    const T sg = (T)sign;
+    C w1, w3;
+    w1.x = (o[1].x - sg * o[1].y) * h;  // (1 + sign i) / sqrt 2
+    w1.y = (o[1].y + sg * o[1].x) * h;
+    w3.x = (-o[3].x - sg * o[3].y) * h;  // (-1 + sign i) / sqrt 2
+    w3.y = (-o[3].y + sg * o[3].x) * h;
+    const C w2 = cmuli(o[2], sign);
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], w1);
+    v[5] = csub(e[1], w1);
+    v[2] = cadd(e[2], w2);
+    v[6] = csub(e[2], w2);
+    v[3] = cadd(e[3], w3);
+    v[7] = csub(e[3], w3);
   } else {
     // odd R: pair r with R-r
     constexpr int H = (R - 1) / 2;
@@ -210,6 +233,7 @@ __device__ __forceinline__ void lds_passes(typename V2<T>::t *buf, int LP,
   for (int s = 0; s < npass; ++s) {
     const FftPass ps = pass[s];
     switch (ps.radix) {
+      case 8: lds_pass<T, 8, FFT_KM_EL(MAXEL, 8), SIGN>(buf, LP, Bl, ps, tw); break;
       case 4: lds_pass<T, 4, FFT_KM_EL(MAXEL, 4), SIGN>(buf, LP, Bl, ps, tw); break;
       case 2: lds_pass<T, 2, FFT_KM_EL(MAXEL, 2), SIGN>(buf, LP, Bl, ps, tw); break;
       case 3: lds_pass<T, 3, FFT_KM_EL(MAXEL, 3), SIGN>(buf, LP, Bl, ps, tw); break;
@@ -391,7 +415,7 @@ int fft_plan_passes(int L, FftPass *pass, int maxpass) {
 namespace {
 std::vector<int> factorize(int L) {
   std::vector<int> f;
-  for (int r : {4, 2, 3, 5, 7, 11})
+  for (int r : {8, 4, 2, 3, 5, 7, 11})
     while (L % r == 0 && L > 1) {
       f.push_back(r);
       L /= r;
@@ -438,7 +462,7 @@ int fft_split(int L) {
 
 bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
                        const void *tw_L1, const void *tw_L2,
-                       const void *tw_L, hipStream_t s) {
+                       const void *tw_L, hipStream_t s, void *a_dst) {
   const int L = p.L, L2 = L / L1;
   if (L1 < 2 || L2 < 2 || L1 * L2 != L || p.st_v != 1) return false;
   DMX_ASSERT(p.nu <= 65535 && L1 <= 65535 && L2 <= 65535,
@@ -483,16 +507,17 @@ bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
 #undef DMX_COLS
     DMX_HIP(hipGetLastError());
   };
-  // pass A: in place, length L1 over rows n1*L2+n2, twiddled
+  // pass A: length L1 over rows n1*L2+n2, twiddled (in place unless a_dst)
+  void *amid = a_dst ? a_dst : p.data;
   c.src = p.data;
-  c.dst = p.data;
+  c.dst = amid;
   c.tw_sub = tw_L1;
   c.twiddle = 1;
   c.ia = 1; c.ib = L2; c.oa = 1; c.ob = L2;
   c.nlines = L2;
   launch(L1);
   // pass B: length L2 over rows k1*L2+n2, to row k1 + L1*k2 of dst
-  c.src = p.data;
+  c.src = amid;
   c.dst = dst;
   c.tw_sub = tw_L2;
   c.twiddle = 0;
@@ -548,6 +573,7 @@ void fft_lineset(const FftLaunch &base, bool fp64, void *scratch,
       switch (R) {
         case 1: DMX_GP(double, 1); break; case 2: DMX_GP(double, 2); break;
         case 3: DMX_GP(double, 3); break; case 4: DMX_GP(double, 4); break;
+        case 8: DMX_GP(double, 8); break;
         case 5: DMX_GP(double, 5); break; case 7: DMX_GP(double, 7); break;
         case 11: DMX_GP(double, 11); break;
       }
@@ -555,6 +581,7 @@ void fft_lineset(const FftLaunch &base, bool fp64, void *scratch,
       switch (R) {
         case 1: DMX_GP(float, 1); break; case 2: DMX_GP(float, 2); break;
         case 3: DMX_GP(float, 3); break; case 4: DMX_GP(float, 4); break;
+        case 8: DMX_GP(float, 8); break;
         case 5: DMX_GP(float, 5); break; case 7: DMX_GP(float, 7); break;
         case 11: DMX_GP(float, 11); break;
       }

@@ -13,11 +13,31 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["c2c"]
+__all__ = ["c2c", "good_size"]
 
 
 def _is_torch(x):
     return type(x).__module__.startswith("torch")
+
+
+def good_size(n, real=False):
+    """Smallest length >= n that is 2,3,5,7,11-smooth (2,3,5-smooth if
+    `real`): ducc0.fft.good_size (src/ducc0/fft/fft.h:93-119, the complex
+    and real variants; argument checks of python/fft_pymod.cc:427-449)."""
+    n = int(n)
+    if n < 0:
+        raise ValueError("Target length must be positive")
+    if n <= 6:
+        return n
+    primes = (2, 3, 5) if real else (2, 3, 5, 7, 11)
+    while True:
+        m = n
+        for p in primes:
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            return n
+        n += 1
 
 
 def _makeaxes(ndim, axes):

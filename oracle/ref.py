@@ -35,6 +35,7 @@ def lib():
         L.ref_sort_index.argtypes = [I, S, P, P, P, D, D, D, I, I, S, P, P, P]
         L.ref_nu2u.argtypes = [I, S, P, I, P, I, I, D, S, P, P, D, D, P, I]
         L.ref_u2nu.argtypes = L.ref_nu2u.argtypes
+        L.ref_c2c.argtypes = [I, I, P, P, P, P, I, I, D, S]
         _lib = L
     return _lib
 
@@ -136,4 +137,20 @@ def u2nu(grid, coord, forward, epsilon, nthreads=1, sigma_min=1.2,
                           int(forward), epsilon, nthreads,
                           _p(np.asarray(v.shape, np.uintp)), _p(out),
                           sigma_min, sigma_max, _p(per), int(fft_order)))
+    return out
+
+
+def c2c(a, axes=None, forward=True, fct=1.0, nthreads=1):
+    """ducc0::c2c (src/ducc0/fft/fft.h:848-850) on a complex array"""
+    a = np.ascontiguousarray(a)
+    single = a.dtype == np.complex64
+    if not single:
+        a = a.astype(np.complex128, copy=False)
+    ax = np.asarray(range(a.ndim) if axes is None else axes, dtype=np.int64)
+    ax = np.where(ax < 0, ax + a.ndim, ax)
+    shp = np.asarray(a.shape, dtype=np.int64)
+    out = np.empty_like(a)
+    _check(lib().ref_c2c(int(single), a.ndim, _p(shp), _p(a), _p(out),
+                         _p(ax), len(ax), int(bool(forward)), float(fct),
+                         int(nthreads)))
     return out

@@ -10,6 +10,7 @@
 //   selectKernel / Coeff / corfunc  src/ducc0/math/gridding_kernel.{h,cc}
 //   bestEpsilon                     src/ducc0/math/gridding_kernel.cc:1562
 //   build_index (coord_idx order)   src/ducc0/nufft/nufft.h:892-907,1364,1862
+//   c2c                             src/ducc0/fft/fft.h:848-850
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -206,6 +207,24 @@ int ref_u2nu(int ndim, size_t M, const void *coords, int coord_single,
       if (coord_single) REF_CALL(double, float); else REF_CALL(double, double);
     }
 #undef REF_CALL
+  });
+}
+
+// contiguous complex arrays of `shape`; single!=0 -> complex64
+int ref_c2c(int single, int ndim, const int64_t *shape, const void *in,
+            void *out, const int64_t *axes, int naxes, int forward,
+            double fct, size_t nthreads) {
+  return guard([&] {
+    std::vector<size_t> shp(shape, shape + ndim), ax(axes, axes + naxes);
+    if (single) {
+      cfmav<complex<float>> a((const complex<float> *)in, shp);
+      vfmav<complex<float>> o((complex<float> *)out, shp);
+      c2c(a, o, ax, forward != 0, (float)fct, nthreads);
+    } else {
+      cfmav<complex<double>> a((const complex<double> *)in, shp);
+      vfmav<complex<double>> o((complex<double> *)out, shp);
+      c2c(a, o, ax, forward != 0, fct, nthreads);
+    }
   });
 }
 

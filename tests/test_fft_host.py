@@ -60,3 +60,36 @@ def test_norm_fct():
     assert abs(fft._norm_fct(1, shp, [2, 2]) - 1 / 5) < 1e-16
     with pytest.raises(ValueError, match="invalid value for inorm"):
         fft._norm_fct(3, shp, [0])
+
+
+def test_reference_c2c_matches_numpy():
+    """ducc0::c2c (oracle/_ref) agrees with the numpy checker"""
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("reference library not built (needs the ducc0 sources)")
+    rng = np.random.default_rng(3)
+    for shp, ax in (((1999,), None), ((24, 35), (0,)), ((6, 7, 11), None)):
+        a = rng.random(shp) - 0.5 + 1j * (rng.random(shp) - 0.5)
+        assert l2error(ref.c2c(a, axes=ax), np.fft.fftn(a, axes=ax)) < 2e-15
+        assert l2error(ref.c2c(a, axes=ax, forward=False),
+                       np.fft.ifftn(a, axes=ax) *
+                       np.prod([a.shape[i] for i in
+                                (ax if ax else range(a.ndim))])) < 2e-15
+
+
+def test_good_size():
+    """smallest 11-smooth (5-smooth for real) length >= n, by brute force"""
+    def smooth(m, ps):
+        for p in ps:
+            while m % p == 0:
+                m //= p
+        return m == 1
+    for n in list(range(0, 300)) + [1021, 4097, 65537, 1000003]:
+        for real, ps in ((False, (2, 3, 5, 7, 11)), (True, (2, 3, 5))):
+            g = fft.good_size(n, real)
+            assert g >= n
+            if n > 0:
+                assert smooth(g, ps)
+                assert all(not smooth(m, ps) for m in range(n, g))
+    with pytest.raises(ValueError):
+        fft.good_size(-1)

@@ -1,8 +1,10 @@
 """ducc_amd.fft.c2c against the reference's own c2c tests
 (python/test/test_fft.py:25-37, 98-160: round trips within 6e-7 / 2e-15 in
 l2 error over lengths 1-255, 1700-2047 and 137^2, n-d shapes, axis
-subsets) and against numpy.fft (a C port of pocketfft, the algorithm family
-of ducc0's FFT), itself pinned here by the direct DFT sum on small sizes."""
+subsets), against ducc0::c2c itself (compiled from the reference sources
+into oracle/_ref, where built) and against numpy.fft (a C port of pocketfft,
+the algorithm family of ducc0's FFT), itself pinned by the direct DFT sum
+on small sizes in test_fft_host.py."""
 import numpy as np
 import pytest
 
@@ -180,3 +182,23 @@ def test_torch_device(gpu, dt):
     assert r is y
     ref = torch.fft.fft(x.to(cd), dim=0).cpu().numpy()
     assert l2error(y.cpu().numpy(), ref) < tl
+
+
+@pmp("shape,axes,dt", [
+    ((8192,), None, np.complex128), ((6561,), None, np.complex64),
+    ((137 * 137,), None, np.complex128), ((1999,), None, np.complex64),
+    ((480, 1155), None, np.complex128), ((256, 243), (1,), np.complex64),
+    ((97, 64, 50), None, np.complex128), ((60, 61, 62), (0, 2), np.complex64)])
+@pmp("forward", [True, False])
+def test_vs_reference_c2c(gpu, shape, axes, dt, forward):
+    """same transform as ducc0::c2c compiled from the reference sources"""
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("reference library not built (needs the ducc0 sources)")
+    rng = np.random.default_rng(9)
+    a = (rng.random(shape) - 0.5 + 1j * (rng.random(shape) - 0.5)).astype(dt)
+    r = ref.c2c(a, axes=axes, forward=forward, fct=0.5)
+    from ducc_amd import fft
+    g = fft.c2c(a, axes=axes, forward=forward)
+    t = 2 * tol[np.float32 if dt == np.complex64 else np.float64]
+    assert l2error(0.5 * g, r) < t
