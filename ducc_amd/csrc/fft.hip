@@ -268,25 +268,29 @@ k_fft_lds(FftLaunch p) {
   const int nlines = min(Bl, p.nv - v0);
   const int nel = nlines * L;
   const float inv_L = 1.0f / (float)L, inv_nl = 1.0f / (float)nlines;
-  // load
+  // load (unrolled: several loads of a lane in flight before their LDS
+  // writes)
+#pragma unroll 4
   for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     int b, k;
     if (p.st_ax == 1) b = fdivq(e, L, inv_L, k);
     else k = fdivq(e, nlines, inv_nl, b);
+    C x;
     if (p.kprune && k >= p.kn1 && k < p.ka2) {
-      C z;
-      z.x = T(0);
-      z.y = T(0);
-      buf[b * LP + k] = z;
-      continue;
+      x.x = T(0);
+      x.y = T(0);
+    } else {
+      const int64_t addr = ubase +
+                           map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v +
+                           k * p.st_ax;
+      x = data[addr];
     }
-    const int64_t addr =
-        ubase + map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v + k * p.st_ax;
-    buf[b * LP + k] = data[addr];
+    buf[b * LP + k] = x;
   }
   __syncthreads();
   lds_passes<T, MAXEL, SIGN>(buf, LP, nlines, p.npass, p.pass, tw);
   // store
+#pragma unroll 4
   for (int e = threadIdx.x; e < nel; e += blockDim.x) {
     int b, k;
     if (p.st_ax == 1) b = fdivq(e, L, inv_L, k);
@@ -323,12 +327,27 @@ k_fft_cols(FftCols p) {
       map_range(blockIdx.z, p.on1, p.oa1, p.oa2) * p.st_o;
   const int nel = nb * Ls;
   const float inv_nb = 1.0f / (float)nb, inv_L = 1.0f / (float)p.L;
-  for (int i = threadIdx.x; i < nel; i += blockDim.x) {
-    int b;
-    const int e = fdivq(i, nb, inv_nb, b);
-    const int64_t row = (int64_t)u * p.ia + (int64_t)e * p.ib;
-    buf[b * LP + e] =
-        src[obase + row * p.rs + map_range(v0 + b, p.vn1, p.va1, p.va2)];
+  // a lane's global loads are all issued before its LDS writes (nel < MAXEL)
+  constexpr int NPER = (MAXEL + FFT_THREADS - 1) / FFT_THREADS;
+  {
+    C tmp[NPER];
+    int slot[NPER];
+#pragma unroll
+    for (int j = 0; j < NPER; ++j) {
+      const int i = (int)threadIdx.x + j * FFT_THREADS;
+      slot[j] = -1;
+      if (i < nel) {
+        int b;
+        const int e = fdivq(i, nb, inv_nb, b);
+        const int64_t row = (int64_t)u * p.ia + (int64_t)e * p.ib;
+        slot[j] = b * LP + e;
+        tmp[j] = src[obase + row * p.rs +
+                     map_range(v0 + b, p.vn1, p.va1, p.va2)];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NPER; ++j)
+      if (slot[j] >= 0) buf[slot[j]] = tmp[j];
   }
   __syncthreads();
   lds_passes<T, MAXEL, SIGN>(buf, LP, nb, p.npass, p.pass, tws);
