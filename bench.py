@@ -180,6 +180,24 @@ def bench_c2c(args):
     dom = max(axis_ms, key=axis_ms.get)
     ach = alg / (axis_ms[dom] * 1e-3) / 1e9
     value = 2 * alg / (ms * 1e-3) / 1e9
+    # HBM bytes of the phase from the profiler's FETCH_SIZE + WRITE_SIZE of
+    # this transform (calibrated by a plain copy)
+    traffic = None
+    tp = os.path.join(HERE, "profiles", "c2c_traffic.json")
+    if os.path.exists(tp):
+        try:
+            traffic = json.load(open(tp)).get(dom)
+        except Exception:
+            traffic = None
+    roofline = dict(bound="hbm", kernel=dom, achieved=round(ach, 1),
+                    peak=PEAK_HBM_GBS, unit="GB/s",
+                    frac=round(ach / PEAK_HBM_GBS, 4), traffic=traffic,
+                    alg_bytes_per_launch=alg,
+                    avg_launch_ms=round(axis_ms[dom], 4))
+    if traffic:
+        hb = traffic / (axis_ms[dom] * 1e-3) / 1e9
+        roofline["traffic_GBps"] = round(hb, 1)
+        roofline["traffic_frac"] = round(hb / PEAK_HBM_GBS, 4)
     # accuracy against numpy's FP64 FFT of the same input
     xin = x.cpu().numpy()
     work.copy_(x)
@@ -231,11 +249,7 @@ def bench_c2c(args):
         scaling="weak", vs_baseline=None, dtype="fp32", data="synthetic",
         config=dict(workload="ducc0.fft.c2c, 2-D 4096x4096 complex64, "
                              "forward, in place on the device"),
-        roofline=dict(bound="hbm", kernel=f"c2c/{dom}",
-                      achieved=round(ach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                      frac=round(ach / PEAK_HBM_GBS, 4), traffic=None,
-                      alg_bytes_per_launch=alg,
-                      avg_launch_ms=round(axis_ms[dom], 4)),
+        roofline=roofline,
         cpu_baseline=cpu, err_vs_numpy_fp64=err,
         phases_ms={k: round(v, 4) for k, v in per.items()},
         fft_bench_shapes=rows)
@@ -267,13 +281,7 @@ def main():
               "--gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "c2c":
-        bench_c2c(args)
-        return
     import torch
-    from ducc_amd import _lib, nufft
-    from ducc_amd import chooser as learned
-
     dev = torch.device("cuda:0")
     if args.calib:
         # a plain copy of known size: 256 MiB read + 256 MiB written
@@ -281,6 +289,11 @@ def main():
         y = x.clone()
         torch.cuda.synchronize()
         del x, y
+    if args.workload == "c2c":
+        bench_c2c(args)
+        return
+    from ducc_amd import _lib, nufft
+    from ducc_amd import chooser as learned
     rng = np.random.default_rng(102)
     M, n, eps = args.M, args.n, args.eps
     c_np = clustered_points(rng, M)
