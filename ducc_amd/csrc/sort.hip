@@ -23,7 +23,10 @@ constexpr int RTILE = RT * RITEMS;
 
 // Tile key per point (nufft.h:1364-1381 role); one 4096-point block per
 // workgroup, so that the first sort pass's per-block digit histogram comes
-// out of the same sweep.
+// out of the same sweep.  Each lane loads a batch of FB points before it
+// folds them, so the loads of a batch are in flight together.
+constexpr int FB = 8;
+template <int NDIM>
 __global__ __launch_bounds__(RT) void k_fold_keys(Geo g, const void *coords,
                                                   size_t M, uint32_t *keys,
                                                   uint32_t *hist0,
@@ -33,28 +36,30 @@ __global__ __launch_bounds__(RT) void k_fold_keys(Geo g, const void *coords,
   if (hist0) h[tid] = 0;
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * RTILE;
-#pragma unroll 4
-  for (int i = 0; i < RITEMS; ++i) {
-    const size_t j = base + (size_t)i * RT + tid;
-    if (j >= M) break;
-    uint32_t t[3] = {0, 0, 0};
-    double x[3];
-    load_point(g, coords, j, g.ndim, x);
-    for (int d = 0; d < g.ndim; ++d) {
-      double fr;
-      int i0 = fold_axis(g, d, x[d], fr);
-      t[d] = (uint32_t)((i0 + g.nsafe) >> g.log2tile);
+  for (int i0 = 0; i0 < RITEMS; i0 += FB) {
+    double x[FB][NDIM];
+#pragma unroll
+    for (int q = 0; q < FB; ++q) {
+      const size_t j = base + (size_t)(i0 + q) * RT + tid;
+      if (j < M) load_point(g, coords, j, NDIM, x[q]);
     }
-    uint32_t key;
-    if (g.ndim == 1)
-      key = t[0];
-    else if (g.ndim == 2)
-      key = t[0] * (uint32_t)g.ntiles[1] + t[1];
-    else
-      key = (t[0] * (uint32_t)g.ntiles[1] + t[1]) * (uint32_t)g.ntiles[2] +
-            t[2];
-    keys[j] = key;
-    if (hist0) atomicAdd(&h[key & 0xFFu], 1u);
+#pragma unroll
+    for (int q = 0; q < FB; ++q) {
+      const size_t j = base + (size_t)(i0 + q) * RT + tid;
+      if (j >= M) continue;
+      uint32_t t[NDIM];
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) {
+        double fr;
+        const int c0 = fold_axis(g, d, x[q][d], fr);
+        t[d] = (uint32_t)((c0 + g.nsafe) >> g.log2tile);
+      }
+      uint32_t key = t[0];
+#pragma unroll
+      for (int d = 1; d < NDIM; ++d) key = key * (uint32_t)g.ntiles[d] + t[d];
+      keys[j] = key;
+      if (hist0) atomicAdd(&h[key & 0xFFu], 1u);
+    }
   }
   if (hist0) {
     __syncthreads();
@@ -344,8 +349,16 @@ void launch_fold_keys(const Geo &g, const void *coords, size_t M,
                       uint32_t *keys, uint32_t *hist0, hipStream_t s) {
   if (!M) return;
   const size_t nblocks = (M + RTILE - 1) / RTILE;
-  hipLaunchKernelGGL(k_fold_keys, dim3((unsigned)nblocks), dim3(RT), 0, s, g,
-                     coords, M, keys, hist0, (int)nblocks);
+  const dim3 grid((unsigned)nblocks), block(RT);
+  if (g.ndim == 1)
+    hipLaunchKernelGGL(k_fold_keys<1>, grid, block, 0, s, g, coords, M, keys,
+                       hist0, (int)nblocks);
+  else if (g.ndim == 2)
+    hipLaunchKernelGGL(k_fold_keys<2>, grid, block, 0, s, g, coords, M, keys,
+                       hist0, (int)nblocks);
+  else
+    hipLaunchKernelGGL(k_fold_keys<3>, grid, block, 0, s, g, coords, M, keys,
+                       hist0, (int)nblocks);
   DMX_HIP(hipGetLastError());
 }
 
@@ -358,7 +371,9 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n,
                         uint32_t *total, void *work, hipStream_t s) {
   if (!n) return;
   size_t nb = (n + STILE - 1) / STILE;
-  if (nb <= 32) {
+  // one workgroup walks the chunks one after another (a load latency each):
+  // only for one or two chunks; longer arrays take the three-launch scan
+  if (nb <= 2) {
     hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(ST), 0, s, in, out, n,
                        total);
     DMX_HIP(hipGetLastError());
