@@ -8,6 +8,7 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -188,6 +189,27 @@ hipStream_t library_stream() {
     dev = cur;
   }
   return s;
+}
+
+// A second stream per thread and device (non-blocking, default priority)
+// for the part of a call that can run beside the main stream, with the
+// events that fork work to it and join it back.  Re-recording an event is
+// safe: a stream wait uses the record that precedes it.
+struct Aux {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+Aux &aux() {
+  thread_local std::map<int, Aux> per_dev;
+  int cur = 0;
+  DMX_HIP(hipGetDevice(&cur));
+  Aux &a = per_dev[cur];
+  if (!a.s) {
+    DMX_HIP(hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking));
+    DMX_HIP(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming));
+    DMX_HIP(hipEventCreateWithFlags(&a.join, hipEventDisableTiming));
+  }
+  return a;
 }
 
 dmx_options defaults() {
@@ -404,21 +426,35 @@ int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
                        uni->device && pts->device &&
                        (!grad || gradout->device);
     auto plan = std::make_unique<Plan>(p, s);
+    std::unique_ptr<DevIn> ui;  // type 2: the uniform array, on the device
+    if (!type1) ui = std::make_unique<DevIn>(*uni, s, true);
+    // The grid side (type 1: zeroing; type 2: pad and FFT) does not depend
+    // on the points: it runs on the second stream while the points are
+    // sorted on s, and s waits for it before the points are processed.
+    Aux &ax = aux();
+    DMX_HIP(hipEventRecord(ax.fork, s));
+    DMX_HIP(hipStreamWaitEvent(ax.s, ax.fork, 0));
+    plan->set_stream(ax.s);
+    if (type1)
+      plan->nu2u_grid();
+    else
+      plan->u2nu_grid(forward != 0, ui->p, vf64, ui->st, grad);
+    DMX_HIP(hipEventRecord(ax.join, ax.s));
+    plan->set_stream(s);
     plan->build_index(cin.p, cf32, M, false);
+    DMX_HIP(hipStreamWaitEvent(s, ax.join, 0));
     if (type1) {
       DevIn pin(*pts, s, false);
       DevOut uo(*uni, true);
       plan->nu2u(forward != 0, pin.p, vf64, uo.p, vf64, uo.st);
       uo.finish(s, !async);
     } else if (!grad) {
-      DevIn ui(*uni, s, true);
       DevOut po(*pts, false);
-      plan->u2nu(forward != 0, ui.p, vf64, ui.st, po.p, vf64);
+      plan->u2nu(forward != 0, ui->p, vf64, ui->st, po.p, vf64);
       po.finish(s, !async);
     } else {
-      DevIn ui(*uni, s, true);
       DevOut go(*gradout, false);
-      plan->u2nu_grad(forward != 0, ui.p, vf64, ui.st, go.p, vf64);
+      plan->u2nu_grad(forward != 0, ui->p, vf64, ui->st, go.p, vf64);
       go.finish(s, !async);
     }
     if (info) plan->info(info);

@@ -109,6 +109,15 @@ class Plan {
   // several times: one random gather instead of one per execution).
   void build_index(const void *coords, bool coord_f32, size_t M,
                    bool keep_sorted_coords);
+  // The grid-side part of a transform that does not depend on the points
+  // (type 1: the zeroed grid; type 2: pad and FFT), enqueued ahead of the
+  // rest, e.g. on another stream while build_index runs.  The next nu2u /
+  // u2nu / u2nu_grad then does only the remainder (same arguments).
+  void nu2u_grid();
+  void u2nu_grid(bool forward, const void *uni, bool uni_f64,
+                 const int64_t *ust, bool grad = false);
+  // the stream the plan enqueues on from now on
+  void set_stream(hipStream_t s) { s_ = s; }
   // type 1: points (device, M complex, c64/c128) -> uniform (device, strided)
   void nu2u(bool forward, const void *points, bool pts_f64, void *uni,
             bool uni_f64, const int64_t *ust);
@@ -149,6 +158,7 @@ class Plan {
   };
   std::vector<Pending> pend_;
   bool done_ = false;
+  bool grid_ready_ = false;  // a *_grid call enqueued the grid side
   std::string tag_ = "plan";  // "index", "nu2u" or "u2nu": phase prefix
   void prepare_grid_side();
   // how the FFT along real axis d runs: in LDS, as the two-pass 2-D column

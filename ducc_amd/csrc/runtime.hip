@@ -571,14 +571,19 @@ void Plan::run_fft(bool forward, bool type1) {
   }
 }
 
-void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
-                bool uni_f64, const int64_t *ust) {
+void Plan::nu2u_grid() {
   tag_ = "nu2u";
   prepare_grid_side();
-  {
-    Phase ph(this, "zeroing grid");
-    launch_zero(grid_.p, grid_.bytes, s_);
-  }
+  Phase ph(this, "zeroing grid");
+  launch_zero(grid_.p, grid_.bytes, s_);
+  grid_ready_ = true;
+}
+
+void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
+                bool uni_f64, const int64_t *ust) {
+  if (!grid_ready_) nu2u_grid();
+  grid_ready_ = false;
+  tag_ = "nu2u";
   if (M_ > 0) {
     Phase ph(this, "spreading");
     SpreadArgs a{};
@@ -595,12 +600,20 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
   launch_grid_copy(p_.fp64, false, c, s_);
 }
 
-void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
-                const int64_t *ust, void *out, bool out_f64) {
-  tag_ = "u2nu";
+void Plan::u2nu_grid(bool forward, const void *uni, bool uni_f64,
+                     const int64_t *ust, bool grad) {
+  tag_ = grad ? "u2nu_grad" : "u2nu";
   prepare_grid_side();
   pad_grid(uni, uni_f64, ust);
   run_fft(forward, false);
+  grid_ready_ = true;
+}
+
+void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
+                const int64_t *ust, void *out, bool out_f64) {
+  if (!grid_ready_) u2nu_grid(forward, uni, uni_f64, ust);
+  grid_ready_ = false;
+  tag_ = "u2nu";
   if (M_ > 0) {
     SpreadArgs a{};
     fill_spread_args(a);
@@ -612,10 +625,9 @@ void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
 
 void Plan::u2nu_grad(bool forward, const void *uni, bool uni_f64,
                      const int64_t *ust, void *out, bool out_f64) {
+  if (!grid_ready_) u2nu_grid(forward, uni, uni_f64, ust, true);
+  grid_ready_ = false;
   tag_ = "u2nu_grad";
-  prepare_grid_side();
-  pad_grid(uni, uni_f64, ust);
-  run_fft(forward, false);
   if (M_ > 0) {
     SpreadArgs a{};
     fill_spread_args(a);
