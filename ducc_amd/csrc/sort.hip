@@ -75,10 +75,17 @@ __global__ __launch_bounds__(RT) void k_radix_hist(const uint32_t *keys,
   h[threadIdx.x] = 0;
   __syncthreads();
   size_t base = (size_t)blockIdx.x * RTILE;
-#pragma unroll 4
+  // all of a lane's key loads in flight before its LDS atomics
+  uint32_t kv[RITEMS];
+#pragma unroll
   for (int i = 0; i < RITEMS; ++i) {
     size_t j = base + (size_t)i * RT + threadIdx.x;
-    if (j < M) atomicAdd(&h[(keys[j] >> shift) & 0xFFu], 1u);
+    kv[i] = j < M ? keys[j] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < RITEMS; ++i) {
+    size_t j = base + (size_t)i * RT + threadIdx.x;
+    if (j < M) atomicAdd(&h[(kv[i] >> shift) & 0xFFu], 1u);
   }
   __syncthreads();
   hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];

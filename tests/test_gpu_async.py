@@ -75,3 +75,40 @@ def test_plan_on_device_is_stream_ordered(gpu):
     ref = nufft.u2nu(grid=g, coord=c, forward=True, epsilon=1e-9)
     for k, o in enumerate(outs):
         assert O.l2error(o.cpu().numpy(), (k + 1) * ref) < 1e-8
+
+
+@pytest.mark.parametrize("kind", ["u2nu", "nu2u"])
+def test_inputs_written_just_before_the_call(gpu, kind):
+    """The grid side of an unplanned call runs on a second stream beside the
+    index build: it must still see inputs written on the caller's stream
+    right before the call (large inputs, so that a missed wait would read
+    the old contents)."""
+    import torch
+    from ducc_amd import nufft
+    c, p, g = _inputs(51, M=200000, shape=(600, 500))
+    if kind == "u2nu":
+        ref = nufft.u2nu(grid=g, coord=c, forward=True, epsilon=1e-9)
+    else:
+        ref = nufft.nu2u(points=p, coord=c, forward=True, epsilon=1e-9,
+                         out=np.empty(g.shape, np.complex128))
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        ct = torch.from_numpy(c).cuda()
+        gsrc = torch.from_numpy(g).cuda()
+        psrc = torch.from_numpy(p).cuda()
+        gt = torch.zeros_like(gsrc)
+        pt = torch.zeros_like(psrc)
+        for _ in range(3):  # several kernels queued ahead of the call
+            gt.copy_(gsrc)
+            pt.copy_(psrc)
+        if kind == "u2nu":
+            out = torch.empty(len(p), dtype=torch.complex128, device="cuda")
+            nufft.u2nu(grid=gt, coord=ct, forward=True, epsilon=1e-9,
+                       out=out)
+        else:
+            out = torch.empty(g.shape, dtype=torch.complex128, device="cuda")
+            nufft.nu2u(points=pt, coord=ct, forward=True, epsilon=1e-9,
+                       out=out)
+        res = out.clone()
+    side.synchronize()
+    assert O.l2error(res.cpu().numpy(), ref) < 1e-13
