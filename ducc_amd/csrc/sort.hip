@@ -156,11 +156,18 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
   for (int w = 0; w < NW; ++w) cnt[w][tid] = 0;
   const size_t blk0 = (size_t)blockIdx.x * RTILE;
   const size_t c0 = blk0 + (size_t)wave * (RTILE / NW);
-  uint32_t key[RITEMS], rank[RITEMS];
+  uint32_t key[RITEMS], rank[RITEMS], ival[RITEMS];
 #pragma unroll
   for (int r = 0; r < RITEMS; ++r) {
     const size_t j = c0 + (size_t)r * 64 + lane;
     key[r] = j < M ? keys_in[j] : 0xFFFFFFFFu;
+  }
+  // the indices travel with the keys: loaded now, so that their latency
+  // overlaps the ranking
+#pragma unroll
+  for (int r = 0; r < RITEMS; ++r) {
+    const size_t j = c0 + (size_t)r * 64 + lane;
+    ival[r] = idx_in ? (j < M ? idx_in[j] : 0u) : (uint32_t)j;
   }
   __syncthreads();
   const unsigned long long below_mask =
@@ -231,7 +238,7 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
 #pragma unroll
   for (int r = 0; r < RITEMS; ++r) {
     const size_t j = c0 + (size_t)r * 64 + lane;
-    if (j < M) stage[lpos[r]] = idx_in ? idx_in[j] : (uint32_t)j;
+    if (j < M) stage[lpos[r]] = ival[r];
   }
   __syncthreads();
 #pragma unroll
