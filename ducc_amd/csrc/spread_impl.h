@@ -424,6 +424,28 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
     const uint32_t lp = threadIdx.x / tpp;
     const int part = (int)(threadIdx.x % tpp);
     const uint32_t pstep = blockDim.x / tpp;
+    // one point per lane and round (tpp == 1): the indices and coordinates
+    // of all of a lane's rounds (at most MAXSUB / 256) are loaded up front,
+    // so that their latencies overlap
+    constexpr int PF = tpp == 1 ? MAXSUB / 256 : 1;
+    uint32_t jpf[PF];
+    double xpf[PF][NDIM];
+    if constexpr (tpp == 1) {
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+        const uint32_t q = lp + (uint32_t)k * pstep;
+        jpf[k] = q < nc ? a.idx[sb.start + q] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+        const uint32_t q = lp + (uint32_t)k * pstep;
+        if (q < nc)
+          load_point(a.g, a.coords,
+                     a.coords_sorted ? (size_t)(sb.start + q)
+                                     : (size_t)jpf[k],
+                     NDIM, xpf[k]);
+      }
+    }
     for (uint32_t pb = 0; pb < nc; pb += pstep) {
       const bool valid = tpp == 1 || pb + lp < nc;
       if (tpp == 1 && pb + lp >= nc) break;
@@ -431,14 +453,27 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
       uint32_t j = 0;
       T sr = 0, si = 0;  // MODE 0: this lane's share of the value
       if (valid) {
-      j = a.idx[p];
-      const size_t cj = a.coords_sorted ? (size_t)p : (size_t)j;
       const T *cf = (const T *)a.coef;
       int i0[3];
       T w[NDIM][W];
       T dw[MODE ? NDIM : 1][W];
       double x_[NDIM];
-      load_point(a.g, a.coords, cj, NDIM, x_);
+      if constexpr (tpp == 1) {
+        // round pb / pstep (< PF: nc <= MAXSUB, pstep = 256), selected
+        // with constant indices so the arrays stay in registers
+        const int kk = (int)(pb / pstep);
+#pragma unroll
+        for (int k = 0; k < PF; ++k)
+          if (k == kk) {
+            j = jpf[k];
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) x_[d] = xpf[k][d];
+          }
+      } else {
+        j = a.idx[p];
+        load_point(a.g, a.coords, a.coords_sorted ? (size_t)p : (size_t)j,
+                   NDIM, x_);
+      }
 #pragma unroll
       for (int d = 0; d < NDIM; ++d) {
         double frac;
