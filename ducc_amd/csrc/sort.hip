@@ -25,7 +25,7 @@ constexpr int RTILE = RT * RITEMS;
 // workgroup, so that the first sort pass's per-block digit histogram comes
 // out of the same sweep.  Each lane loads a batch of FB points before it
 // folds them, so the loads of a batch are in flight together.
-constexpr int FB = 8;
+constexpr int FB = 16;
 template <int NDIM>
 __global__ __launch_bounds__(RT) void k_fold_keys(Geo g, const void *coords,
                                                   size_t M, uint32_t *keys,
@@ -162,8 +162,11 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
     const size_t j = c0 + (size_t)r * 64 + lane;
     key[r] = j < M ? keys_in[j] : 0xFFFFFFFFu;
   }
-  // the indices travel with the keys: loaded now, so that their latency
-  // overlaps the ranking
+  // this block's global base of digit tid (chunk scan + scanned chunk
+  // sums), and the indices that travel with the keys: loaded now, so that
+  // their latency overlaps the ranking
+  const size_t og = (size_t)tid * nblocks + blockIdx.x;
+  const uint32_t gofs = offs[og] + offs_chunk[og / STILE];
 #pragma unroll
   for (int r = 0; r < RITEMS; ++r) {
     const size_t j = c0 + (size_t)r * 64 + lane;
@@ -200,9 +203,7 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
     const uint32_t start = block_exclusive_scan(run, scan_sh, tot);
 #pragma unroll
     for (int w = 0; w < NW; ++w) cnt[w][tid] += start;
-    // offsets: scan inside chunks of STILE entries + scanned chunk sums
-    const size_t o = (size_t)tid * nblocks + blockIdx.x;
-    gbase[tid] = offs[o] + offs_chunk[o / STILE] - start;
+    gbase[tid] = gofs - start;
   }
   __syncthreads();
   const int nvalid = (int)min((size_t)RTILE, M - blk0);
