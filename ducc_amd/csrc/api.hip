@@ -429,20 +429,26 @@ int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
     std::unique_ptr<DevIn> ui;  // type 2: the uniform array, on the device
     if (!type1) ui = std::make_unique<DevIn>(*uni, s, true);
     // The grid side (type 1: zeroing; type 2: pad and FFT) does not depend
-    // on the points: it runs on the second stream while the points are
-    // sorted on s, and s waits for it before the points are processed.
-    Aux &ax = aux();
-    DMX_HIP(hipEventRecord(ax.fork, s));
-    DMX_HIP(hipStreamWaitEvent(ax.s, ax.fork, 0));
-    plan->set_stream(ax.s);
-    if (type1)
-      plan->nu2u_grid();
-    else
-      plan->u2nu_grid(forward != 0, ui->p, vf64, ui->st, grad);
-    DMX_HIP(hipEventRecord(ax.join, ax.s));
-    plan->set_stream(s);
+    // on the points: for large requests it runs on the second stream while
+    // the points are sorted on s, and s waits for it before the points are
+    // processed (small ones would only pay the cross-stream latency).
+    size_t nover_total = 1;
+    for (int d = 0; d < ndim; ++d) nover_total *= p.nover[d];
+    const bool fork = M >= ((size_t)1 << 20) && nover_total >= ((size_t)1 << 20);
+    if (fork) {
+      Aux &ax = aux();
+      DMX_HIP(hipEventRecord(ax.fork, s));
+      DMX_HIP(hipStreamWaitEvent(ax.s, ax.fork, 0));
+      plan->set_stream(ax.s);
+      if (type1)
+        plan->nu2u_grid();
+      else
+        plan->u2nu_grid(forward != 0, ui->p, vf64, ui->st, grad);
+      DMX_HIP(hipEventRecord(ax.join, ax.s));
+      plan->set_stream(s);
+    }
     plan->build_index(cin.p, cf32, M, false);
-    DMX_HIP(hipStreamWaitEvent(s, ax.join, 0));
+    if (fork) DMX_HIP(hipStreamWaitEvent(s, aux().join, 0));
     if (type1) {
       DevIn pin(*pts, s, false);
       DevOut uo(*uni, true);

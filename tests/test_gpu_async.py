@@ -85,7 +85,9 @@ def test_inputs_written_just_before_the_call(gpu, kind):
     the old contents)."""
     import torch
     from ducc_amd import nufft
-    c, p, g = _inputs(51, M=200000, shape=(600, 500))
+    # large enough for the second stream (M >= 2^20, oversampled grid of
+    # at least 2^20 cells)
+    c, p, g = _inputs(51, M=1_100_000, shape=(600, 500))
     if kind == "u2nu":
         ref = nufft.u2nu(grid=g, coord=c, forward=True, epsilon=1e-9)
     else:
