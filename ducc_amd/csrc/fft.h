@@ -36,9 +36,12 @@ struct FftLaunch {
   int64_t st_ax, st_u, st_v;
   int nu, nv;
   int un1, ua1, ua2, vn1, va1, va2;
-  // contiguous lines only: elements kn1 <= k < ka2 are read as zero (the
-  // input is known to vanish there)
+  // elements kn1 <= k < ka2 are read as zero (the input is known to vanish
+  // there)
   int kprune, kn1, ka2;
+  // elements sn1 <= k < sa2 of the result are not stored (no later pass
+  // reads them)
+  int sprune, sn1, sa2;
 };
 
 // One half of the two-pass FFT along a strided axis (see k_fft_cols).
@@ -55,6 +58,7 @@ struct FftCols {
   int nlines;
   int nouter, on1, oa1, oa2;
   int64_t st_o;
+  int sprune, sn1, sa2;  // output rows sn1 <= row < sa2 are not stored
 };
 
 // FFT of length L = L1*L2 along a strided axis, as two coalesced passes:

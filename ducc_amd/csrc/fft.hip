@@ -295,6 +295,7 @@ k_fft_lds(FftLaunch p) {
     int b, k;
     if (p.st_ax == 1) b = fdivq(e, L, inv_L, k);
     else k = fdivq(e, nlines, inv_nl, b);
+    if (p.sprune && k >= p.sn1 && k < p.sa2) continue;
     const int64_t addr =
         ubase + map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v + k * p.st_ax;
     data[addr] = buf[b * LP + k];
@@ -364,6 +365,7 @@ k_fft_cols(FftCols p) {
       x = cmul(x, w);
     }
     const int64_t row = (int64_t)u * p.oa + (int64_t)k * p.ob;
+    if (p.sprune && row >= p.sn1 && row < p.sa2) continue;
     dst[obase + row * p.rs + map_range(v0 + b, p.vn1, p.va1, p.va2)] = x;
   }
 }
@@ -542,6 +544,9 @@ bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
   c.twiddle = 0;
   c.ia = L2; c.ib = 1; c.oa = 1; c.ob = L1;
   c.nlines = L1;
+  c.sprune = p.sprune;  // pass B writes natural order: prune as asked
+  c.sn1 = p.sn1;
+  c.sa2 = p.sa2;
   launch(L2);
   return true;
 }

@@ -545,6 +545,13 @@ void Plan::run_fft(bool forward, bool type1) {
     range(pax, pax > a, L.nu, L.un1, L.ua1, L.ua2);
     range(qax, qax > a, L.nv, L.vn1, L.va1, L.va2);
     if (L.L <= 1) continue;
+    if (type1) {
+      // type 1: later passes and the crop read only the retained modes of
+      // this axis
+      L.sprune = 1;
+      L.sn1 = (n_act[a] + 1) / 2;
+      L.sa2 = N_act[a] - n_act[a] / 2;
+    }
     Phase ph(this, std::string("FFT axis ") + std::to_string(a - f));
     const FftRoute route = fft_route(a - f);
     if (route == FftRoute::columns) {
