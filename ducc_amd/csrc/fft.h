@@ -59,6 +59,7 @@ struct FftCols {
   int nouter, on1, oa1, oa2;
   int64_t st_o;
   int sprune, sn1, sa2;  // output rows sn1 <= row < sa2 are not stored
+  int kprune, kn1, ka2;  // input rows kn1 <= row < ka2 are read as zero
 };
 
 // FFT of length L = L1*L2 along a strided axis, as two coalesced passes:

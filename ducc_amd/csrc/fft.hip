@@ -342,8 +342,13 @@ k_fft_cols(FftCols p) {
         const int e = fdivq(i, nb, inv_nb, b);
         const int64_t row = (int64_t)u * p.ia + (int64_t)e * p.ib;
         slot[j] = b * LP + e;
-        tmp[j] = src[obase + row * p.rs +
-                     map_range(v0 + b, p.vn1, p.va1, p.va2)];
+        if (p.kprune && row >= p.kn1 && row < p.ka2) {
+          tmp[j].x = T(0);
+          tmp[j].y = T(0);
+        } else {
+          tmp[j] = src[obase + row * p.rs +
+                       map_range(v0 + b, p.vn1, p.va1, p.va2)];
+        }
       }
     }
 #pragma unroll
@@ -534,6 +539,10 @@ bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
   c.dst = amid;
   c.tw_sub = tw_L1;
   c.twiddle = 1;
+  // pass A reads natural order: known-zero input rows as asked
+  c.kprune = p.kprune;
+  c.kn1 = p.kn1;
+  c.ka2 = p.ka2;
   c.ia = 1; c.ib = L2; c.oa = 1; c.ob = L2;
   c.nlines = L2;
   launch(L1);
@@ -544,6 +553,7 @@ bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
   c.twiddle = 0;
   c.ia = L2; c.ib = 1; c.oa = 1; c.ob = L1;
   c.nlines = L1;
+  c.kprune = 0;
   c.sprune = p.sprune;  // pass B writes natural order: prune as asked
   c.sn1 = p.sn1;
   c.sa2 = p.sa2;

@@ -492,13 +492,14 @@ Plan::FftRoute Plan::fft_route(int d) const {
 void Plan::pad_grid(const void *uni, bool uni_f64, const int64_t *ust) {
   GridCopyArgs c = copy_args(uni, uni_f64, ust);
   c.grid = grid_.p;
-  // An FFT in LDS reads the retained modes of its axis only (the rest is
-  // known to be zero), so along such an axis the retained rows suffice.
-  // An FFT of whole lines reads every position: up to the last axis done
-  // that way, every grid position is written, zero outside the modes.
+  // An FFT in LDS (rows, or the two-pass columns) reads the retained modes
+  // of its axis only (the rest is known to be zero), so along such an axis
+  // the retained rows suffice.  An FFT of whole lines (global passes) reads
+  // every position: up to the last axis done that way, every grid position
+  // is written, zero outside the modes.
   int last_whole = -1;
   for (int d = 0; d < p_.ndim; ++d)
-    if (fft_route(d) != FftRoute::lds) last_whole = d;
+    if (fft_route(d) == FftRoute::global) last_whole = d;
   for (int d = 0; d <= last_whole; ++d) c.full[d] = 1;
   Phase ph(this, "grid correction");
   launch_grid_copy(p_.fp64, true, c, s_);
@@ -554,6 +555,13 @@ void Plan::run_fft(bool forward, bool type1) {
     }
     Phase ph(this, std::string("FFT axis ") + std::to_string(a - f));
     const FftRoute route = fft_route(a - f);
+    if (!type1 && route != FftRoute::global) {
+      // type 2: along this axis only the retained modes were written (by
+      // the pad or by the FFTs of the earlier axes); the rest is zero
+      L.kprune = 1;
+      L.kn1 = (n_act[a] + 1) / 2;
+      L.ka2 = N_act[a] - n_act[a] / 2;
+    }
     if (route == FftRoute::columns) {
       // two coalesced passes (64 columns per workgroup), the second writing
       // natural order into the other buffer
@@ -565,13 +573,6 @@ void Plan::run_fft(bool forward, bool type1) {
                         twiddle_for(L.L / L1), twiddle_for(L.L), s_);
       cur_ = other;
       continue;
-    }
-    if (!type1 && route == FftRoute::lds) {
-      // type 2: along this axis only the retained modes were written (by
-      // the pad or by the FFTs of the earlier axes); the rest is zero
-      L.kprune = 1;
-      L.kn1 = (n_act[a] + 1) / 2;
-      L.ka2 = N_act[a] - n_act[a] / 2;
     }
     L.tw = twiddle_for(L.L);
     fft_lineset(L, p_.fp64, cur_ == grid_.p ? scratch_.p : grid_.p, s_);
