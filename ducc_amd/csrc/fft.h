@@ -44,6 +44,19 @@ struct FftLaunch {
   int sprune, sn1, sa2;
 };
 
+// Crop + deconvolution of type 1 fused into the last column pass of a 2-D
+// transform: the retained mode at grid (row, column) goes to the uniform
+// array (strides ust0, ust1 in elements; complex128 if u_f64), times the
+// FP64 correction factors cf0[|n0/2 - i0|] * cf1[|n1/2 - i1|]
+// (comp_indices, nufft.h:142-151).
+struct FftCrop {
+  void *uni;
+  const double *cf0, *cf1;
+  int64_t ust0, ust1;
+  int n0, N0, n1, N1;
+  int fft_order, u_f64;
+};
+
 // One half of the two-pass FFT along a strided axis (see k_fft_cols).
 struct FftCols {
   const void *src;
@@ -60,6 +73,8 @@ struct FftCols {
   int64_t st_o;
   int sprune, sn1, sa2;  // output rows sn1 <= row < sa2 are not stored
   int kprune, kn1, ka2;  // input rows kn1 <= row < ka2 are read as zero
+  int crop;              // store through `cr` instead of dst (pass B)
+  FftCrop cr;
 };
 
 // FFT of length L = L1*L2 along a strided axis, as two coalesced passes:
@@ -67,12 +82,13 @@ struct FftCols {
 // with `a_dst`, from p.data into a_dst at the same positions);
 // B: length-L2 FFTs over rows k1*L2+n2 of A's output, written to row
 // k1 + L1*k2 of dst, i.e. in natural order (dst may be p.data when a_dst is
-// given: the result then replaces the input).  Returns false when L has no
-// split worth using.
+// given: the result then replaces the input; with `crop`, pass B writes the
+// retained modes to the uniform array instead, see FftCrop).  Returns false
+// when L has no split worth using.
 bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
                        const void *tw_L1, const void *tw_L2,
                        const void *tw_L, hipStream_t s,
-                       void *a_dst = nullptr);
+                       void *a_dst = nullptr, const FftCrop *crop = nullptr);
 // a divisor of L near sqrt(L) with both factors <= 128 (0: none)
 int fft_split(int L);
 

@@ -371,7 +371,31 @@ k_fft_cols(FftCols p) {
     }
     const int64_t row = (int64_t)u * p.oa + (int64_t)k * p.ob;
     if (p.sprune && row >= p.sn1 && row < p.sa2) continue;
-    dst[obase + row * p.rs + map_range(v0 + b, p.vn1, p.va1, p.va2)] = x;
+    const int col = (int)map_range(v0 + b, p.vn1, p.va1, p.va2);
+    if (p.crop) {
+      // grid index -> mode index (retained rows and columns only reach
+      // here), correction index and index into the uniform array
+      const FftCrop &q = p.cr;
+      const int g0 = (int)row, h0 = q.n0 / 2, h1 = q.n1 / 2;
+      const int i0 = g0 < q.n0 - h0 ? g0 + h0 : g0 - (q.N0 - h0);
+      const int i1 = col < q.n1 - h1 ? col + h1 : col - (q.N1 - h1);
+      const int c0 = h0 > i0 ? h0 - i0 : i0 - h0;
+      const int c1 = h1 > i1 ? h1 - i1 : i1 - h1;
+      int u0 = q.fft_order ? i0 + q.n0 - h0 : i0;
+      if (u0 >= q.n0) u0 -= q.n0;
+      int u1 = q.fft_order ? i1 + q.n1 - h1 : i1;
+      if (u1 >= q.n1) u1 -= q.n1;
+      const T fct = (T)(q.cf0[c0] * q.cf1[c1]);
+      const int64_t ui = (int64_t)u0 * q.ust0 + (int64_t)u1 * q.ust1;
+      if (q.u_f64)
+        ((double2 *)q.uni)[ui] =
+            make_double2((double)(x.x * fct), (double)(x.y * fct));
+      else
+        ((float2 *)q.uni)[ui] =
+            make_float2((float)(x.x * fct), (float)(x.y * fct));
+      continue;
+    }
+    dst[obase + row * p.rs + col] = x;
   }
 }
 
@@ -488,7 +512,8 @@ int fft_split(int L) {
 
 bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
                        const void *tw_L1, const void *tw_L2,
-                       const void *tw_L, hipStream_t s, void *a_dst) {
+                       const void *tw_L, hipStream_t s, void *a_dst,
+                       const FftCrop *crop) {
   const int L = p.L, L2 = L / L1;
   if (L1 < 2 || L2 < 2 || L1 * L2 != L || p.st_v != 1) return false;
   DMX_ASSERT(p.nu <= 65535 && L1 <= 65535 && L2 <= 65535,
@@ -554,6 +579,10 @@ bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
   c.ia = L2; c.ib = 1; c.oa = 1; c.ob = L1;
   c.nlines = L1;
   c.kprune = 0;
+  if (crop) {
+    c.crop = 1;
+    c.cr = *crop;
+  }
   c.sprune = p.sprune;  // pass B writes natural order: prune as asked
   c.sn1 = p.sn1;
   c.sa2 = p.sa2;

@@ -169,7 +169,10 @@ class Plan {
                          const int64_t *ust) const;
   // zero-padded, corrected uniform array -> grid, type 2
   void pad_grid(const void *uni, bool uni_f64, const int64_t *ust);
-  void run_fft(bool forward, bool type1);
+  // FFT passes of the oversampled grid; with `crop` (type 1), the last pass
+  // may also write the corrected retained modes to the uniform array:
+  // returns true when it did
+  bool run_fft(bool forward, bool type1, const GridCopyArgs *crop = nullptr);
   void fill_spread_args(SpreadArgs &a) const;
 
   Params p_;

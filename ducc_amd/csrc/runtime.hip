@@ -505,7 +505,8 @@ void Plan::pad_grid(const void *uni, bool uni_f64, const int64_t *ust) {
   launch_grid_copy(p_.fp64, true, c, s_);
 }
 
-void Plan::run_fft(bool forward, bool type1) {
+bool Plan::run_fft(bool forward, bool type1, const GridCopyArgs *crop) {
+  bool cropped = false;
   const int nd = p_.ndim, f = 3 - nd;
   int64_t E[3], st[3];
   int n_act[3], N_act[3];
@@ -569,14 +570,34 @@ void Plan::run_fft(bool forward, bool type1) {
       // rows longer than LDS cannot skip the unwritten columns: clear them
       if (!type1 && E[2] > maxel) launch_zero(other, grid_.bytes, s_);
       const int L1 = fft_split(L.L);
+      // type 1, 2-D: the column pass is the last one; it writes the
+      // retained modes to the uniform array with their correction factors
+      FftCrop fc{};
+      const bool fuse = type1 && crop && nd == 2 && a == f;
+      if (fuse) {
+        fc.uni = crop->uni;
+        fc.cf0 = crop->cf[0];
+        fc.cf1 = crop->cf[1];
+        fc.ust0 = crop->ust[0];
+        fc.ust1 = crop->ust[1];
+        fc.n0 = crop->n[0];
+        fc.N0 = crop->N[0];
+        fc.n1 = crop->n[1];
+        fc.N1 = crop->N[1];
+        fc.fft_order = crop->fft_order;
+        fc.u_f64 = crop->u_f64;
+      }
       fft_columns_2pass(L, L1, other, p_.fp64, twiddle_for(L1),
-                        twiddle_for(L.L / L1), twiddle_for(L.L), s_);
+                        twiddle_for(L.L / L1), twiddle_for(L.L), s_, nullptr,
+                        fuse ? &fc : nullptr);
       cur_ = other;
+      cropped = cropped || fuse;
       continue;
     }
     L.tw = twiddle_for(L.L);
     fft_lineset(L, p_.fp64, cur_ == grid_.p ? scratch_.p : grid_.p, s_);
   }
+  return cropped;
 }
 
 void Plan::nu2u_grid() {
@@ -601,8 +622,8 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
     a.grid = grid_.p;
     launch_spread(p_.fp64, a, s_);
   }
-  run_fft(forward, true);
   GridCopyArgs c = copy_args(uni, uni_f64, ust);
+  if (run_fft(forward, true, &c)) return;  // the last FFT pass cropped
   c.grid = cur_;
   Phase ph(this, "grid correction");
   launch_grid_copy(p_.fp64, false, c, s_);
