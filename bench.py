@@ -276,9 +276,9 @@ def main():
                          "byte counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU "
-              "path (see DESIGN.md for how the NUFFT would shard); run with "
-              "--gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(the NUFFT would shard points by tile ranges; not built); run "
+              "with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch

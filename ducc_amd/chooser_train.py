@@ -347,8 +347,8 @@ def main():
     ap.add_argument("--eval", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=105)
     ap.add_argument("--out", default=chooser.WEIGHTS)
-    ap.add_argument("--report", default=os.path.join(ROOT, "gpurun_out",
-                                                     "chooser_eval.json"))
+    ap.add_argument("--report", default="chooser_eval.json",
+                    help="evaluation report (JSON) path")
     ap.add_argument("--label-only", default=None, metavar="NPZ",
                     help="label a batch, write it to NPZ (+ .json timings)")
     ap.add_argument("--labels", default=None,

@@ -168,7 +168,8 @@ int dmx_c2c(const dmx_array *in, const dmx_array *out, const int64_t *axes,
             const dmx_options *opt);
 
 /* ---- learned parameter chooser (no ducc0 counterpart; its fallback and
-   label judge is ducc0's findNufftParameters, nufft.h:155-208) ----
+   the reference for its labels is ducc0's findNufftParameters,
+   nufft.h:155-208) ----
    Network: 6 features -> 64 -> 64 -> 64 (ReLU) -> 3 outputs (oversampling
    factor, support W, log2 tile), float32, MLP_NPARAM = 8963 parameters
    (W1 64x6, b1, W2 64x64, b2, W3 64x64, b3, W4 3x64, b4).  All arrays host

@@ -1,4 +1,4 @@
-"""Pin the network oracle (oracle/mlp.py) before it judges the HIP
+"""Pin the network oracle (oracle/mlp.py) before it checks the HIP
 kernels: analytic gradient vs central differences in float64."""
 import numpy as np
 
