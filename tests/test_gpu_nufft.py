@@ -219,3 +219,33 @@ def test_errors(gpu):
     with pytest.raises(RuntimeError, match="dimensionality mismatch"):
         nufft.u2nu(grid=np.zeros((4, 4, 4), np.complex128), coord=c,
                    forward=True, epsilon=1e-5)
+
+
+@pmp("fft_order", [False, True])
+@pmp("precision,vdt", [("fp64", np.complex128), ("fp32", np.complex64),
+                       ("fp32", np.complex128)])
+def test_fused_crop_strided_device_output(gpu, fft_order, precision, vdt):
+    """2-D type 1 on a grid whose column FFT runs as two passes (the second
+    one writes the uniform array itself): strided device output, both mode
+    orders, compute precision independent of the output type."""
+    import torch
+    from ducc_amd import nufft
+    rng = np.random.default_rng(21)
+    shape = (400, 257)
+    c, ms, _ = data(rng, shape, 1000, 2 * np.pi, False)
+    eps = 1e-9 if precision == "fp64" else 1e-5
+    info = {}
+    big = torch.zeros((shape[0], 2 * shape[1] + 3),
+                      dtype=torch.complex128 if vdt == np.complex128
+                      else torch.complex64, device="cuda")
+    out = big[:, 1:2 * shape[1] + 1:2]  # column stride 2
+    nufft.nu2u(points=torch.from_numpy(ms.astype(vdt)).cuda(),
+               coord=torch.from_numpy(c).cuda(), forward=True, epsilon=eps,
+               out=out, fft_order=fft_order, precision=precision, info=info)
+    # columns longer than 512 (FP32) / 256 (FP64) elements: the two-pass
+    # column route
+    assert info["nover"][0] > 512
+    ref = O.nudft_type1(c, ms, shape, True, fft_order=fft_order)
+    assert O.l2error(out.cpu().numpy(), ref) <= 2 * eps
+    # untouched in between the strided columns
+    assert torch.count_nonzero(big[:, 0::2]).item() == 0
