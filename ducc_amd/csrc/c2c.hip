@@ -13,7 +13,7 @@
 //    (outer, n2, inner) buffer and a product with the precomputed FFT of the
 //    chirp.
 // The chirp and its transform are computed once per (device, L, precision)
-// in FP64 and kept for the life of the process.
+// in FP64 and cached (a bounded cache).
 #include "c2c.h"
 
 #include <cmath>
@@ -181,9 +181,17 @@ bool is_smooth(int64_t n) {
 }
 
 // ------------------------------------------------------------ table cache
+// Twiddles and Bluestein tables per (device, length, precision).  Bounded:
+// past TABLES_MAX entries the cache starts over; calls in flight keep their
+// tables through their own references (Job::keep).
+constexpr size_t TABLES_MAX = 512;
 struct Tables {
   std::mutex mu;
   std::map<std::string, std::shared_ptr<Buf>> m;
+  void put(const std::string &k, std::shared_ptr<Buf> b) {
+    if (m.size() >= TABLES_MAX) m.clear();
+    m[k] = std::move(b);
+  }
 };
 Tables &tables() {
   static Tables *t = new Tables();  // never destroyed (buffers outlive exit)
@@ -206,7 +214,7 @@ std::shared_ptr<Buf> twiddles(int L, bool fp64) {
   fft_twiddles(L, fp64, h);
   auto buf = std::make_shared<Buf>(h.size());
   DMX_HIP(hipMemcpy(buf->p, h.data(), h.size(), hipMemcpyHostToDevice));
-  T.m[k] = buf;
+  T.put(k, buf);
   return buf;
 }
 
@@ -287,8 +295,8 @@ Blue bluestein_tables(int L, int n2, bool fp64, int sign, hipStream_t s) {
   // tables are shared across streams: complete before publishing
   DMX_HIP(hipStreamSynchronize(s));
   std::lock_guard<std::mutex> lk(T.mu);
-  T.m[kw] = wb;
-  T.m[kb] = Bb;
+  T.put(kw, wb);
+  T.put(kb, Bb);
   return {wb, Bb};
 }
 
