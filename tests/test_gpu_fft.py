@@ -182,6 +182,12 @@ def test_torch_device(gpu, dt):
     assert r is y
     ref = torch.fft.fft(x.to(cd), dim=0).cpu().numpy()
     assert l2error(y.cpu().numpy(), ref) < tl
+    # in place with a normalisation factor (the scaled copy reads and writes
+    # the same elements)
+    z = x.to(cd).clone()
+    fft.c2c(z, out=z, forward=False, inorm=2)
+    ref = torch.fft.ifftn(x.to(cd)).cpu().numpy()
+    assert l2error(z.cpu().numpy(), ref) < tl
 
 
 @pmp("shape,axes,dt", [
@@ -202,3 +208,23 @@ def test_vs_reference_c2c(gpu, shape, axes, dt, forward):
     g = fft.c2c(a, axes=axes, forward=forward)
     t = 2 * tol[np.float32 if dt == np.complex64 else np.float64]
     assert l2error(0.5 * g, r) < t
+
+
+def test_back_to_back_device_calls(gpu):
+    """calls on device tensors return once queued: their scratch and
+    Bluestein buffers must outlive the kernels (checked after all calls)"""
+    import torch
+    from ducc_amd import fft
+    g = torch.Generator(device="cuda").manual_seed(10)
+    shapes = [(257, 130), (64, 1999), (3, 4096, 5), (6001,), (97, 101),
+              (128, 128)]
+    ins, outs = [], []
+    for shp in shapes:
+        x = torch.randn(shp, generator=g, device="cuda",
+                        dtype=torch.complex128)
+        ins.append(x)
+        outs.append(fft.c2c(x, inorm=1))
+    torch.cuda.synchronize()
+    for x, y in zip(ins, outs):
+        ref = torch.fft.fftn(x, norm="ortho").cpu().numpy()
+        assert l2error(y.cpu().numpy(), ref) < 1e-14
