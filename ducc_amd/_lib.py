@@ -29,7 +29,8 @@ class DmxOptions(ct.Structure):
                 ("log2tile", ct.c_int32), ("kernel_idx", ct.c_int32),
                 ("fp64_below", ct.c_double), ("stream", ct.c_void_p),
                 ("sigma", ct.c_double), ("spot_check", ct.c_int32),
-                ("flags", ct.c_int32), ("reserved", ct.c_int32 * 6)]
+                ("flags", ct.c_int32), ("deterministic", ct.c_int32),
+                ("reserved", ct.c_int32 * 5)]
 
 STREAM_GIVEN, ASYNC = 1, 2
 
@@ -72,8 +73,10 @@ _SIG = {
                                    ct.c_size_t, ct.c_double, ct.c_size_t,
                                    ct.c_double, ct.c_double, _P, ct.c_int,
                                    ct.POINTER(DmxOptions)]),
-    "dmx_plan_nu2u": (ct.c_int, [_P, ct.c_int, ct.c_size_t, _AP, _AP]),
-    "dmx_plan_u2nu": (ct.c_int, [_P, ct.c_int, ct.c_size_t, _AP, _AP]),
+    "dmx_plan_nu2u": (ct.c_int, [_P, ct.c_int, ct.c_size_t, _AP, _AP,
+                                 ct.POINTER(DmxOptions)]),
+    "dmx_plan_u2nu": (ct.c_int, [_P, ct.c_int, ct.c_size_t, _AP, _AP,
+                                 ct.POINTER(DmxOptions)]),
     "dmx_plan_get_info": (ct.c_int, [_P, ct.POINTER(DmxPlanInfo)]),
     "dmx_plan_destroy": (ct.c_int, [_P]),
     "dmx_u2nu_coord_grad": (ct.c_int, [_AP, _AP, ct.c_int, ct.c_double, _AP,
@@ -205,7 +208,8 @@ def profile_read():
 
 
 def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
-            fp64_below=1e-6, stream=None, sigma=0.0, spot_check=True):
+            fp64_below=1e-6, stream=None, sigma=0.0, spot_check=True,
+            deterministic=False):
     o = DmxOptions()
     lib().dmx_default_options(ct.byref(o))
     o.spot_check = 2 if spot_check == "fail" else int(bool(spot_check))
@@ -224,4 +228,5 @@ def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
         o.stream = stream
         o.flags = STREAM_GIVEN | ASYNC
     o.sigma = float(sigma)
+    o.deterministic = int(bool(deterministic))
     return o

@@ -135,6 +135,7 @@ struct DevIn {
     if (n == 0) return;
     std::vector<char> h((size_t)n * es);
     pack(a, h.data());
+    tmp.stream = s;  // released stream-ordered: kernels on s read it
     tmp.reset((size_t)n * es);
     DMX_HIP(hipMemcpyAsync(tmp.p, h.data(), h.size(), hipMemcpyHostToDevice, s));
     DMX_HIP(hipStreamSynchronize(s));
@@ -148,7 +149,8 @@ struct DevOut {
   Buf tmp;
   const dmx_array *a;
   int64_t st[10] = {0};
-  DevOut(const dmx_array &arr, bool strided_ok) : a(&arr) {
+  DevOut(const dmx_array &arr, hipStream_t s, bool strided_ok) : a(&arr) {
+    tmp.stream = s;  // released stream-ordered: kernels on s write it
     size_t es = elsize(arr.typecode);
     int64_t n = nelem(arr);
     if (arr.device) {
@@ -393,7 +395,8 @@ int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
                   const dmx_array *pts, int forward, double epsilon,
                   double smin, double smax, const double *per, int fft_order,
                   const dmx_options *opt_in, dmx_plan_info *info,
-                  bool grad = false, const dmx_array *gradout = nullptr) {
+                  size_t verbosity = 0, bool grad = false,
+                  const dmx_array *gradout = nullptr) {
   return guard([&] {
     DMX_ASSERT(uni && coord && pts && per, "null argument");
     reap(false);
@@ -426,6 +429,13 @@ int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
                        uni->device && pts->device &&
                        (!grad || gradout->device);
     auto plan = std::make_unique<Plan>(p, s);
+    if (verbosity > 0) {
+      // ducc0 prints the request before the work and the phase timings
+      // after it (nufft.h:488-491, 499-502); the timings need the results,
+      // so a verbose call waits for its work
+      plan->report_params(type1);
+      plan->start_timing();
+    }
     std::unique_ptr<DevIn> ui;  // type 2: the uniform array, on the device
     if (!type1) ui = std::make_unique<DevIn>(*uni, s, true);
     // The grid side (type 1: zeroing; type 2: pad and FFT) does not depend
@@ -451,19 +461,21 @@ int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
     if (fork) DMX_HIP(hipStreamWaitEvent(s, aux().join, 0));
     if (type1) {
       DevIn pin(*pts, s, false);
-      DevOut uo(*uni, true);
+      DevOut uo(*uni, s, true);
       plan->nu2u(forward != 0, pin.p, vf64, uo.p, vf64, uo.st);
       uo.finish(s, !async);
     } else if (!grad) {
-      DevOut po(*pts, false);
+      DevOut po(*pts, s, false);
       plan->u2nu(forward != 0, ui->p, vf64, ui->st, po.p, vf64);
       po.finish(s, !async);
     } else {
-      DevOut go(*gradout, false);
+      DevOut go(*gradout, s, false);
       plan->u2nu_grad(forward != 0, ui->p, vf64, ui->st, go.p, vf64);
       go.finish(s, !async);
     }
     if (info) plan->info(info);
+    if (verbosity > 0)
+      plan->report_timings(type1 ? "nu2u" : (grad ? "u2nu_coord_grad" : "u2nu"));
     if (async) retire(std::move(plan), s);
   });
 }
@@ -479,11 +491,26 @@ struct dmx_plan {
   bool async;  // DMX_ASYNC at creation
   int ndim;
   int64_t shape[3];
-  hipStream_t stream;
-  dmx::Buf coords;  // device copy when the caller passed host coordinates
+  hipStream_t stream;  // creation stream: calls without options run here
 };
 
 using namespace dmx;
+
+namespace {
+// the stream and async mode of one planned call: the caller's options, else
+// the plan's creation settings; the plan's work moves to that stream in
+// order after what it queued before
+hipStream_t plan_stream(dmx_plan *plan, const dmx_options *opt, bool &async) {
+  hipStream_t s = plan->stream;
+  async = plan->async;
+  if (opt) {
+    s = pick_stream(*opt);
+    async = (opt->flags & DMX_ASYNC) != 0;
+  }
+  plan->plan->use_stream(s);
+  return s;
+}
+}  // namespace
 
 extern "C" {
 
@@ -650,9 +677,9 @@ int dmx_u2nu(const dmx_array *grid, const dmx_array *coord, int forward,
              const double *periodicity, int fft_order,
              const dmx_options *opt, dmx_plan_info *info) {
   (void)nthreads;
-  (void)verbosity;
   return run_unplanned(false, grid, coord, out, forward, epsilon, sigma_min,
-                       sigma_max, periodicity, fft_order, opt, info);
+                       sigma_max, periodicity, fft_order, opt, info,
+                       verbosity);
 }
 
 int dmx_nu2u(const dmx_array *points, const dmx_array *coord, int forward,
@@ -661,9 +688,9 @@ int dmx_nu2u(const dmx_array *points, const dmx_array *coord, int forward,
              const double *periodicity, int fft_order,
              const dmx_options *opt, dmx_plan_info *info) {
   (void)nthreads;
-  (void)verbosity;
   return run_unplanned(true, out, coord, points, forward, epsilon, sigma_min,
-                       sigma_max, periodicity, fft_order, opt, info);
+                       sigma_max, periodicity, fft_order, opt, info,
+                       verbosity);
 }
 
 int dmx_u2nu_coord_grad(const dmx_array *grid, const dmx_array *coord,
@@ -685,7 +712,7 @@ int dmx_u2nu_coord_grad(const dmx_array *grid, const dmx_array *coord,
     pts.stride[0] = 1;
     int rc = run_unplanned(false, grid, coord, &pts, forward, epsilon,
                            sigma_min, sigma_max, periodicity, fft_order, opt,
-                           nullptr, true, out);
+                           nullptr, 0, true, out);
     if (rc) throw Error(g_err);
   });
 }
@@ -711,7 +738,6 @@ int dmx_plan_create(dmx_plan **plan, int nu2u, const dmx_array *coord,
     const size_t M = (size_t)coord->shape[0];
     // a plan computes in the precision of its coordinates unless told
     // otherwise (nufft_pymod.cc:145-150)
-    if (opt.precision == 0 && coord->typecode == DMX_F64) opt.precision = 0;
     Params p = choose_params((int)ndim, nuni, M, nu2u != 0, epsilon,
                              sigma_min, sigma_max, periodicity,
                              fft_order != 0, coord->typecode == DMX_F64, opt);
@@ -722,6 +748,8 @@ int dmx_plan_create(dmx_plan **plan, int nu2u, const dmx_array *coord,
     h->ndim = (int)ndim;
     for (size_t d = 0; d < 3; ++d) h->shape[d] = d < ndim ? grid_shape[d] : 1;
     h->plan = std::make_unique<Plan>(p, h->stream);
+    // host coordinates are staged in a buffer that returns to the pool only
+    // once the index build queued on the plan's stream has read it
     DevIn cin(*coord, h->stream, false);
     h->plan->build_index(cin.p, coord->typecode == DMX_F32, M, true);
     *plan = h.release();
@@ -729,8 +757,8 @@ int dmx_plan_create(dmx_plan **plan, int nu2u, const dmx_array *coord,
 }
 
 int dmx_plan_nu2u(dmx_plan *plan, int forward, size_t verbosity,
-                  const dmx_array *points, dmx_array *out) {
-  (void)verbosity;
+                  const dmx_array *points, dmx_array *out,
+                  const dmx_options *opt) {
   return guard([&] {
     DMX_ASSERT(plan && points && out, "null argument");
     DMX_ASSERT(is_complex(points->typecode) && points->typecode == out->typecode,
@@ -742,19 +770,25 @@ int dmx_plan_nu2u(dmx_plan *plan, int forward, size_t verbosity,
     for (int d = 0; d < plan->ndim; ++d)
       DMX_ASSERT(out->shape[d] == plan->shape[d],
                  "uniform grid dimensions mismatch");
-    hipStream_t s = plan->stream;
+    bool async = false;
+    hipStream_t s = plan_stream(plan, opt, async);
     const bool f64 = points->typecode == DMX_C128;
+    if (verbosity > 0) {
+      plan->plan->report_params(true);
+      plan->plan->start_timing();
+    }
     DevIn pin(*points, s, false);
-    DevOut uo(*out, true);
+    DevOut uo(*out, s, true);
     plan->plan->nu2u(forward != 0, pin.p, f64, uo.p, f64, uo.st);
-    uo.finish(s, !(plan->async && points->device && out->device));
+    uo.finish(s, !(async && points->device && out->device));
+    if (verbosity > 0) plan->plan->report_timings("nu2u");
     plan->plan->flush_profile(false);
   });
 }
 
 int dmx_plan_u2nu(dmx_plan *plan, int forward, size_t verbosity,
-                  const dmx_array *grid, dmx_array *out) {
-  (void)verbosity;
+                  const dmx_array *grid, dmx_array *out,
+                  const dmx_options *opt) {
   return guard([&] {
     DMX_ASSERT(plan && grid && out, "null argument");
     DMX_ASSERT(is_complex(grid->typecode) && grid->typecode == out->typecode,
@@ -765,12 +799,18 @@ int dmx_plan_u2nu(dmx_plan *plan, int forward, size_t verbosity,
     for (int d = 0; d < plan->ndim; ++d)
       DMX_ASSERT(grid->shape[d] == plan->shape[d],
                  "uniform grid dimensions mismatch");
-    hipStream_t s = plan->stream;
+    bool async = false;
+    hipStream_t s = plan_stream(plan, opt, async);
     const bool f64 = grid->typecode == DMX_C128;
+    if (verbosity > 0) {
+      plan->plan->report_params(false);
+      plan->plan->start_timing();
+    }
     DevIn gin(*grid, s, true);
-    DevOut po(*out, false);
+    DevOut po(*out, s, false);
     plan->plan->u2nu(forward != 0, gin.p, f64, gin.st, po.p, f64);
-    po.finish(s, !(plan->async && grid->device && out->device));
+    po.finish(s, !(async && grid->device && out->device));
+    if (verbosity > 0) plan->plan->report_timings("u2nu");
     plan->plan->flush_profile(false);
   });
 }
@@ -802,7 +842,7 @@ int dmx_nudft(int type, const dmx_array *coord, const dmx_array *values,
     DMX_ASSERT(nelem(*out) == (int64_t)nsel, "output size mismatch");
     hipStream_t s = library_stream();
     DevIn c(*coord, s, false), v(*values, s, false);
-    DevOut o(*out, false);
+    DevOut o(*out, s, false);
     Buf selb(nsel * 8);
     if (nsel) {
       DMX_HIP(hipMemcpyAsync(selb.p, sel, nsel * 8, hipMemcpyHostToDevice, s));
@@ -867,7 +907,7 @@ int dmx_sort_points(const dmx_array *coord, const int64_t *nover,
     Plan plan(p, s);
     DevIn cin(*coord, s, false);
     plan.build_index(cin.p, coord->typecode == DMX_F32, M, false);
-    DevOut ko(*keys, false), io(*idx, false);
+    DevOut ko(*keys, s, false), io(*idx, s, false);
     if (M) {
       plan.keys_copy_to((uint32_t *)ko.p);
       DMX_HIP(hipMemcpyAsync(io.p, plan.idx(), M * 4, hipMemcpyDeviceToDevice,
@@ -929,7 +969,7 @@ int dmx_c2c(const dmx_array *in, const dmx_array *out, const int64_t *axes,
     hipStream_t s = pick_stream(opt);
     const bool async = (opt.flags & DMX_ASYNC) && in->device && out->device;
     DevIn din(*in, s, true);
-    DevOut dout(*out, true);
+    DevOut dout(*out, s, true);
     auto job = c2c_exec(din.p, in->typecode, din.st, dout.p, dout.st, ndim,
                         in->shape, ax, (int)naxes, forward != 0, fct, s);
     dout.finish(s, !async);

@@ -18,7 +18,7 @@ GEN = os.path.join(HERE, "_gen")
 
 SOURCES = ["params.cpp", "runtime.hip", "api.hip", "sort.hip", "spread.hip",
            "interp.hip", "fft.hip", "grid.hip", "chooser.hip", "mlp.hip",
-           "c2c.hip"]
+           "c2c.hip", "spread_det.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DMX_ARCH", "gfx950")
@@ -49,9 +49,21 @@ def gen_kernel_db():
             f.write(body)
 
 
-def compile_one(src):
+def _newest_header():
+    hdrs = [os.path.join(HERE, f) for f in os.listdir(HERE)
+            if f.endswith(".h")]
+    hdrs.append(os.path.join(INCLUDE, "ducc_amd.h"))
+    hdrs.append(os.path.join(GEN, "kernel_db.inc"))
+    return max(os.path.getmtime(h) for h in hdrs if os.path.exists(h))
+
+
+def compile_one(src, incremental=False):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     path = os.path.join(HERE, src)
+    if incremental and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if t > os.path.getmtime(path) and t > _newest_header():
+            return obj, 
     if src.endswith(".hip"):
         cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE + ["-c", path, "-o", obj]
     else:
@@ -62,13 +74,16 @@ def compile_one(src):
     return obj, r.stderr
 
 
-def build(jobs=None, verbose=False):
+def build(jobs=None, verbose=False, incremental=False):
+    """Compile every source (incremental: only those older than their
+    object or than a header) and link the library."""
     gen_kernel_db()
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
     objs = []
     with cf.ThreadPoolExecutor(jobs) as ex:
-        for obj, err in ex.map(compile_one, SOURCES):
+        for obj, err in ex.map(lambda f: compile_one(f, incremental),
+                               SOURCES):
             objs.append(obj)
             if verbose and err.strip():
                 print(err, file=sys.stderr)
@@ -86,5 +101,7 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=None)
     ap.add_argument("-v", action="store_true")
+    ap.add_argument("-i", action="store_true",
+                    help="incremental: recompile changed sources only")
     a = ap.parse_args()
-    print(build(a.j, a.v))
+    print(build(a.j, a.v, a.i))

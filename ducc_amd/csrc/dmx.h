@@ -249,11 +249,19 @@ struct SpreadArgs {
   int lds_side;             // tile + W - 1
   int use_lds;
   int coords_sorted;        // 1: coords are stored in sorted order
+  // deterministic mode: per-subproblem FP64 regions, and each tile's first
+  // subproblem and subproblem count (index build tables)
+  void *slab;
+  const uint32_t *tile_sub0, *tile_nsub;
 };
 void launch_spread(bool fp64, const SpreadArgs &a, hipStream_t s);
 void launch_interp(bool fp64, const SpreadArgs &a, void *out, int out_f64,
                    hipStream_t s);
 void launch_interp_grad(bool fp64, const SpreadArgs &a, void *out,
                         int out_f64, hipStream_t s);
+// deterministic type-1 spreading (spread_det.hip): writes every cell of
+// a.grid (no zeroing needed); a.slab holds spread_det_slab_bytes(a)
+void launch_spread_det(bool fp64, const SpreadArgs &a, hipStream_t s);
+size_t spread_det_slab_bytes(const SpreadArgs &a);
 
 }  // namespace dmx

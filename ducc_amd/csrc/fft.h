@@ -93,6 +93,9 @@ bool fft_columns_2pass(const FftLaunch &p, int L1, void *dst, bool fp64,
 int fft_split(int L);
 
 void fft_twiddles(int L, bool fp64, std::vector<unsigned char> &out);
+// the same table computed on the device into dst (L complex values of the
+// compute type), for callers that make no host-to-device copies
+void launch_fft_twiddles(void *dst, int L, bool fp64, hipStream_t s);
 // Runs the FFT over every line of the set (in place).  `scratch` must be as
 // large as the array when L exceeds the LDS capacity.
 void fft_lineset(const FftLaunch &p, bool fp64, void *scratch, hipStream_t s);

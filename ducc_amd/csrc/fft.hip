@@ -503,6 +503,32 @@ void fft_twiddles(int L, bool fp64, std::vector<unsigned char> &out) {
   }
 }
 
+namespace {
+__global__ void k_twiddles(void *out, int L, int fp64) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= L) return;
+  double x = 2.0 * (double)m / (double)L;
+  if (x > 1.0) x -= 2.0;  // (-1, 1]
+  double sn, c;
+  sincospi(x, &sn, &c);
+  double s = -sn;
+  if (4 * (long)m == (long)L) { c = 0; s = -1; }
+  if (2 * (long)m == (long)L) { c = -1; s = 0; }
+  if (4 * (long)m == 3 * (long)L) { c = 0; s = 1; }
+  if (fp64)
+    ((double2 *)out)[m] = make_double2(c, s);
+  else
+    ((float2 *)out)[m] = make_float2((float)c, (float)s);
+}
+}  // namespace
+
+void launch_fft_twiddles(void *dst, int L, bool fp64, hipStream_t s) {
+  if (L <= 0) return;
+  hipLaunchKernelGGL(k_twiddles, dim3((L + 255) / 256), dim3(256), 0, s, dst,
+                     L, fp64 ? 1 : 0);
+  DMX_HIP(hipGetLastError());
+}
+
 int fft_split(int L) {
   int best = 0;
   for (int d = 2; d * d <= L; ++d)

@@ -39,6 +39,11 @@ struct NudftArgs {
 };
 
 void launch_corfac(const CorfacArgs &a, double *out, hipStream_t s);
+// dst[i] = vals[i], i < n <= CONST_MAX, written by a kernel whose argument
+// carries the values (no host-to-device copy); as float when as_f32
+constexpr int CONST_MAX = 448;
+void launch_store_consts(void *dst, const double *vals, int n, bool as_f32,
+                         hipStream_t s);
 void launch_grid_copy(bool fp64, bool pad, const GridCopyArgs &a,
                       hipStream_t s);
 void launch_nudft(const NudftArgs &a, hipStream_t s);

@@ -23,6 +23,19 @@ __global__ void k_corfac(CorfacArgs a, double *out) {
   out[k] = 1.0 / s;
 }
 
+struct ConstPayload {
+  double v[CONST_MAX];
+};
+__global__ void k_store_consts(void *dst, ConstPayload pl, int n,
+                               int as_f32) {
+  const int i = threadIdx.x + blockIdx.x * blockDim.x;
+  if (i >= n) return;
+  if (as_f32)
+    ((float *)dst)[i] = (float)pl.v[i];
+  else
+    ((double *)dst)[i] = pl.v[i];
+}
+
 // Position p along one axis of the copied region -> correction index icf,
 // index into the uniform array iu, grid index ig (comp_indices,
 // nufft.h:142-151: mode idx sits at grid index idx - n/2 mod N and at
@@ -247,6 +260,17 @@ void launch_gather_c128(const void *src, const int64_t *idx, size_t n,
 void launch_corfac(const CorfacArgs &a, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_corfac, dim3((a.n + 255) / 256), dim3(256), 0, s, a,
                      out);
+  DMX_HIP(hipGetLastError());
+}
+
+void launch_store_consts(void *dst, const double *vals, int n, bool as_f32,
+                         hipStream_t s) {
+  DMX_ASSERT(n >= 0 && n <= CONST_MAX, "too many constants for one launch");
+  if (!n) return;
+  ConstPayload pl{};
+  for (int i = 0; i < n; ++i) pl.v[i] = vals[i];
+  hipLaunchKernelGGL(k_store_consts, dim3((n + 255) / 256), dim3(256), 0, s,
+                     dst, pl, n, as_f32 ? 1 : 0);
   DMX_HIP(hipGetLastError());
 }
 

@@ -16,37 +16,88 @@
 
 namespace dmx {
 
+// Bump allocation from caller-provided device memory (the static scratch of
+// the allocation-free entry point): while an ArenaScope is alive on a
+// thread, every pool allocation of that thread is carved from
+// [base, base + size) in 256-byte units, and releases are no-ops.
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+  void *take(size_t bytes);
+};
+class ArenaScope {
+ public:
+  explicit ArenaScope(Arena *a);
+  ~ArenaScope();
+  ArenaScope(const ArenaScope &) = delete;
+  ArenaScope &operator=(const ArenaScope &) = delete;
+
+ private:
+  Arena *prev_;
+};
+Arena *current_arena();
+inline size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
+
 // Caching allocator per device: freed blocks are kept and reused for
-// requests of at most their size (and at least half of it).
+// requests of at most their size (and at least half of it).  A block
+// released with a stream is stream-ordered: it is handed out again only
+// after an event recorded on that stream at release time has completed, so
+// work still queued there can never see the block reused by another stream
+// or thread.
 class Pool {
  public:
   static Pool &get();
   void *alloc(size_t bytes);
-  void release(void *p);
+  // s == nullptr: no queued work can touch p any more
+  void release(void *p, hipStream_t s = nullptr);
   void trim();
 
  private:
+  struct Pending {
+    hipEvent_t ev;
+    void *p;
+  };
+  void collect_locked();  // completed pending blocks -> free list
   std::mutex mu_;
   std::map<void *, std::pair<int, size_t>> live_;
   std::multimap<std::pair<int, size_t>, void *> free_;
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> spare_events_;
 };
 
 struct Buf {
   void *p = nullptr;
   size_t bytes = 0;
+  // the stream whose queued work may still use the block when it is freed
+  // (nullptr: the owner has waited for all such work)
+  hipStream_t stream = nullptr;
   Buf() = default;
-  explicit Buf(size_t n) { reset(n); }
+  explicit Buf(size_t n, hipStream_t s = nullptr) : stream(s) { reset(n); }
   ~Buf() { free(); }
   Buf(const Buf &) = delete;
   Buf &operator=(const Buf &) = delete;
-  Buf(Buf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  Buf(Buf &&o) noexcept : p(o.p), bytes(o.bytes), stream(o.stream) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  Buf &operator=(Buf &&o) noexcept {
+    if (this != &o) {
+      free();
+      p = o.p;
+      bytes = o.bytes;
+      stream = o.stream;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   void reset(size_t n) {
     free();
     if (n) p = Pool::get().alloc(n);
     bytes = n;
   }
   void free() {
-    if (p) Pool::get().release(p);
+    if (p) Pool::get().release(p, stream);
     p = nullptr;
     bytes = 0;
   }
@@ -86,6 +137,7 @@ struct Params {
   bool use_lds = true;
   int chooser_used = 0;
   int chooser_verified = 0;
+  bool deterministic = false;  // type 1 without floating-point atomics
 };
 
 // device constants shared by all plans of one parameter set
@@ -116,8 +168,20 @@ class Plan {
   void nu2u_grid();
   void u2nu_grid(bool forward, const void *uni, bool uni_f64,
                  const int64_t *ust, bool grad = false);
-  // the stream the plan enqueues on from now on
+  // the stream the plan enqueues on from now on (no ordering with the
+  // previous stream: for forks whose join the caller arranges)
   void set_stream(hipStream_t s) { s_ = s; }
+  // the stream the plan enqueues on from now on; s first waits for the work
+  // already queued on the previous stream (the plan's buffers are reused)
+  void use_stream(hipStream_t s);
+  // record phase timings of the following work for report_timings (the
+  // verbosity output), whether or not the process-wide profiler is on
+  void start_timing();
+  // ducc0's verbosity report (nufft.h:371-380): request and parameters
+  void report_params(bool type1) const;
+  // waits for the stream and prints the phase table of the work since
+  // start_timing (role of TimerHierarchy::report, timers.h:117-160)
+  void report_timings(const char *what);
   // type 1: points (device, M complex, c64/c128) -> uniform (device, strided)
   void nu2u(bool forward, const void *points, bool pts_f64, void *uni,
             bool uni_f64, const int64_t *ust);
@@ -131,6 +195,10 @@ class Plan {
   const uint32_t *idx() const { return idx_.as<uint32_t>(); }
   uint32_t *keys_copy_to(uint32_t *dev_keys_out);  // re-fold keys
   void info(dmx_plan_info *o) const;
+  // device bytes a transform of M points with parameters p allocates (the
+  // plan's constants, index build, grid buffers and FFT twiddles), each
+  // block rounded to 256 bytes as the pool and the arena hand them out
+  static size_t scratch_bytes(const Params &p, size_t M);
   // the caller has seen all work queued so far complete (the destructor then
   // does not wait for the stream)
   void mark_done() { done_ = true; }
@@ -157,6 +225,10 @@ class Plan {
     hipEvent_t a, b;
   };
   std::vector<Pending> pend_;
+  bool timing_ = false;                  // start_timing() was called
+  hipEvent_t t0_ = nullptr;              // start of the timed call
+  hipEvent_t last_ = nullptr;            // use_stream's ordering event
+  std::vector<std::pair<std::string, double>> timed_;  // phases, this call
   bool done_ = false;
   bool grid_ready_ = false;  // a *_grid call enqueued the grid side
   std::string tag_ = "plan";  // "index", "nu2u" or "u2nu": phase prefix
@@ -189,9 +261,14 @@ class Plan {
   std::shared_ptr<Setup> su_;
   std::vector<std::shared_ptr<Buf>> tw_keep_;
   static std::shared_ptr<Setup> setup_for(const Params &p);
+  // the constants built in the current arena by kernels on s (no cache, no
+  // host-to-device copy)
+  static std::shared_ptr<Setup> arena_setup(const Params &p, hipStream_t s);
+  std::map<int, const void *> arena_tw_;  // twiddles built in the arena
   Buf idx_, subs_, nsub_total_, sorted_coords_;
   Buf tmp_keys_, tmp_tiles_, tmp_work_;
-  Buf grid_, scratch_;
+  Buf grid_, scratch_, slab_;
+  const uint32_t *tile_nsub_ = nullptr, *tile_sub0_ = nullptr;
   void *cur_ = nullptr;  // buffer holding the grid after the FFT passes
   const void *twiddle_for(int L);
 };

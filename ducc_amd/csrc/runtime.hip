@@ -20,18 +20,57 @@
 
 namespace dmx {
 
+// ----------------------------------------------------------------- arena
+namespace {
+thread_local Arena *t_arena = nullptr;
+}
+void *Arena::take(size_t bytes) {
+  const size_t n = round256(bytes);
+  if (used + n > size)
+    throw Error("scratch exhausted: the request needs more than the " +
+                std::to_string(size >> 20) + " MiB of static scratch");
+  void *p = base + used;
+  used += n;
+  return p;
+}
+ArenaScope::ArenaScope(Arena *a) : prev_(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev_; }
+Arena *current_arena() { return t_arena; }
+
 // ------------------------------------------------------------------ pool
 Pool &Pool::get() {
   static Pool *p = new Pool();  // never destroyed: outlives static teardown
   return *p;
 }
 
+void Pool::collect_locked() {
+  size_t k = 0;
+  for (size_t i = 0; i < pending_.size(); ++i) {
+    Pending &q = pending_[i];
+    const hipError_t e = hipEventQuery(q.ev);
+    if (e == hipErrorNotReady) {
+      (void)hipGetLastError();  // "not ready" is not an error here
+      pending_[k++] = q;
+      continue;
+    }
+    auto it = live_.find(q.p);
+    if (it != live_.end()) {
+      free_.insert({it->second, q.p});
+      live_.erase(it);
+    }
+    spare_events_.push_back(q.ev);
+  }
+  pending_.resize(k);
+}
+
 void *Pool::alloc(size_t bytes) {
+  if (Arena *a = t_arena) return a->take(bytes);
   int dev = 0;
   DMX_HIP(hipGetDevice(&dev));
   bytes = (bytes + 255) & ~(size_t)255;
   {
     std::lock_guard<std::mutex> lk(mu_);
+    if (!pending_.empty()) collect_locked();
     auto it = free_.lower_bound({dev, bytes});
     if (it != free_.end() && it->first.first == dev &&
         it->first.second <= 2 * bytes) {
@@ -57,10 +96,30 @@ void *Pool::alloc(size_t bytes) {
   return p;
 }
 
-void Pool::release(void *p) {
+void Pool::release(void *p, hipStream_t s) {
   std::lock_guard<std::mutex> lk(mu_);
   auto it = live_.find(p);
   if (it == live_.end()) return;
+  if (s) {
+    // the block is used by work queued on s: it becomes free once an event
+    // recorded now has completed (the block stays in live_ until then)
+    hipEvent_t ev = nullptr;
+    if (!spare_events_.empty()) {
+      ev = spare_events_.back();
+      spare_events_.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) !=
+               hipSuccess) {
+      ev = nullptr;
+    }
+    if (ev && hipEventRecord(ev, s) == hipSuccess) {
+      pending_.push_back({ev, p});
+      return;
+    }
+    // no event: wait for the stream instead
+    (void)hipGetLastError();
+    if (ev) spare_events_.push_back(ev);
+    (void)hipStreamSynchronize(s);
+  }
   free_.insert({it->second, p});
   live_.erase(it);
 }
@@ -68,6 +127,7 @@ void Pool::release(void *p) {
 void Pool::trim() {
   std::lock_guard<std::mutex> lk(mu_);
   (void)hipDeviceSynchronize();
+  collect_locked();
   for (auto &kv : free_) (void)hipFree(kv.second);
   free_.clear();
 }
@@ -105,7 +165,8 @@ std::string Profiler::report() {
 }
 
 Plan::Phase::Phase(Plan *pl, std::string n) : p(pl) {
-  if (!Profiler::get().enabled()) return;
+  // no events in arena mode: that caller allocates nothing
+  if ((!Profiler::get().enabled() && !p->timing_) || current_arena()) return;
   name = p->tag_ + "/" + n;
   DMX_HIP(hipEventCreate(&a));
   DMX_HIP(hipEventCreate(&b));
@@ -127,12 +188,93 @@ void Plan::flush_profile(bool wait) {
     }
     float ms = 0;
     if (hipEventSynchronize(q.b) == hipSuccess &&
-        hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess)
-      Profiler::get().add(q.name, ms);
+        hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) {
+      if (Profiler::get().enabled()) Profiler::get().add(q.name, ms);
+      if (timing_) timed_.push_back({q.name, ms});
+    }
     (void)hipEventDestroy(q.a);
     (void)hipEventDestroy(q.b);
   }
   pend_.erase(pend_.begin(), pend_.begin() + n);
+}
+
+void Plan::use_stream(hipStream_t s) {
+  if (s == s_) return;
+  if (!last_) DMX_HIP(hipEventCreateWithFlags(&last_, hipEventDisableTiming));
+  DMX_HIP(hipEventRecord(last_, s_));
+  DMX_HIP(hipStreamWaitEvent(s, last_, 0));
+  s_ = s;
+}
+
+void Plan::start_timing() {
+  flush_profile();  // earlier phases are not part of this call
+  timing_ = true;
+  timed_.clear();
+  if (!t0_) DMX_HIP(hipEventCreate(&t0_));
+  DMX_HIP(hipEventRecord(t0_, s_));
+}
+
+namespace {
+std::string dims(const size_t *n, int nd) {
+  std::string s;
+  for (int d = 0; d < nd; ++d) s += (d ? ", " : "") + std::to_string(n[d]);
+  return s;
+}
+}  // namespace
+
+void Plan::report_params(bool type1) const {
+  size_t ngrid = 1;
+  for (int d = 0; d < p_.ndim; ++d) ngrid *= p_.nover[d];
+  std::printf("%s:\n  device, grid=(%s), oversampled grid=(%s), supp=%d, "
+              "eps=%g\n  npoints=%zu\n  memory overhead: %gGB (index) + "
+              "%gGB (oversampled grid)\n  %s path, kernel row %d, sigma=%g, "
+              "tile=%d^%d\n",
+              type1 ? "Nu2u" : "U2nu", dims(p_.nuni, p_.ndim).c_str(),
+              dims(p_.nover, p_.ndim).c_str(), p_.W, p_.eps, M_,
+              M_ * 4.0 / (1 << 30),
+              ngrid * (p_.fp64 ? 16.0 : 8.0) / (1 << 30),
+              p_.fp64 ? "FP64" : "FP32", p_.kidx, p_.ofactor,
+              1 << p_.log2tile, p_.ndim);
+  std::fflush(stdout);
+}
+
+void Plan::report_timings(const char *what) {
+  hipEvent_t t1;
+  DMX_HIP(hipEventCreate(&t1));
+  DMX_HIP(hipEventRecord(t1, s_));
+  DMX_HIP(hipEventSynchronize(t1));
+  float total_ms = 0;
+  (void)hipEventElapsedTime(&total_ms, t0_, t1);
+  (void)hipEventDestroy(t1);
+  flush_profile();
+  timing_ = false;
+  // phases by decreasing time; the remainder (launch gaps, staging, work
+  // overlapped on the second stream) as "<unaccounted>"
+  std::map<std::string, double> acc;
+  for (auto &ph : timed_) {
+    const size_t cut = ph.first.find('/');
+    acc[cut == std::string::npos ? ph.first : ph.first.substr(cut + 1)] +=
+        ph.second * 1e-3;
+  }
+  std::vector<std::pair<std::string, double>> v(acc.begin(), acc.end());
+  std::sort(v.begin(), v.end(),
+            [](const auto &a, const auto &b) { return a.second > b.second; });
+  const double total = total_ms * 1e-3;
+  size_t w = 13;  // length of "<unaccounted>"
+  for (auto &e : v) w = std::max(w, e.first.size());
+  std::printf("\nTotal wall clock time for %s: %.4gs\n|\n", what, total);
+  double sum = 0;
+  auto line = [&](const std::string &n, double t) {
+    std::printf("+- %-*s: %6.2f%% (%.4fs)\n", (int)w, n.c_str(),
+                total > 0 ? 100 * t / total : 0., t);
+  };
+  for (auto &e : v) {
+    line(e.first, e.second);
+    sum += e.second;
+  }
+  if (sum < 0.999 * total) line("<unaccounted>", total - sum);
+  std::fflush(stdout);
+  timed_.clear();
 }
 
 // -------------------------------------------------------- parameter choice
@@ -180,6 +322,7 @@ Params choose_params(int ndim, const size_t *nuni, size_t M, bool gridding,
   }
   p.fft_order = fft_order;
   p.eps = eps;
+  p.deterministic = opt.deterministic != 0;
   if (opt.precision == 2)
     p.fp64 = true;
   else if (opt.precision == 1)
@@ -265,12 +408,14 @@ Plan::Plan(const Params &p, hipStream_t s) : p_(p), s_(s) {
   }
   lds_side_ = (1 << p.log2tile) + p.W - 1;
   ntiles_ = (size_t)g_.ntiles[0] * g_.ntiles[1] * g_.ntiles[2];
-  su_ = setup_for(p);
+  su_ = current_arena() ? arena_setup(p, s) : setup_for(p);
 }
 
 Plan::~Plan() {
   if (!done_) (void)hipStreamSynchronize(s_);
   flush_profile();
+  if (t0_) (void)hipEventDestroy(t0_);
+  if (last_) (void)hipEventDestroy(last_);
 }
 
 namespace {
@@ -344,7 +489,97 @@ std::shared_ptr<Setup> Plan::setup_for(const Params &p) {
   return su;
 }
 
+std::shared_ptr<Setup> Plan::arena_setup(const Params &p, hipStream_t s) {
+  auto su = std::make_shared<Setup>();
+  auto c = device_coeffs(p.W, p.beta, p.e0);
+  su->coef.reset(c.size() * (p.fp64 ? 8 : 4));
+  launch_store_consts(su->coef.p, c.data(), (int)c.size(), !p.fp64, s);
+  CorfacArgs ca{};
+  ca.nq = corr_quadrature(p.W, p.beta * p.W, p.e0, ca.x, ca.wpsi);
+  ca.W = p.W;
+  for (int d = 0; d < p.ndim; ++d) {
+    if (d > 0 && p.nuni[d] == p.nuni[d - 1] && p.nover[d] == p.nover[d - 1])
+      continue;
+    ca.n = (int)(p.nuni[d] / 2 + 1);
+    ca.dx = 1. / (double)p.nover[d];
+    su->corfac[d].reset((size_t)ca.n * 8);
+    launch_corfac(ca, su->corfac[d].as<double>(), s);
+  }
+  const double one = 1.0;
+  su->one.reset(8);
+  launch_store_consts(su->one.p, &one, 1, false, s);
+  return su;
+}
+
+namespace {
+// the distinct FFT lengths whose twiddles a transform uses: each axis's
+// length, plus the two factors of the two-pass column FFT
+void fft_lengths(const Params &p, bool has_scratch, std::vector<int> &Ls) {
+  const int maxel = p.fp64 ? FFT_MAXEL64 : FFT_MAXEL32;
+  auto add = [&](int L) {
+    if (std::find(Ls.begin(), Ls.end(), L) == Ls.end()) Ls.push_back(L);
+  };
+  for (int d = 0; d < p.ndim; ++d) {
+    const int L = (int)p.nover[d];
+    if (L <= 1) continue;
+    if (p.ndim == 2 && d == 0 && maxel / L < 16 && fft_split(L) > 0 &&
+        has_scratch) {
+      add(fft_split(L));
+      add(L / fft_split(L));
+    }
+    add(L);
+  }
+}
+bool needs_scratch_grid(const Params &p) {
+  bool need = p.ndim == 2;
+  for (int d = 0; d < p.ndim; ++d)
+    if ((int)p.nover[d] > (p.fp64 ? FFT_MAXEL64 : FFT_MAXEL32)) need = true;
+  return need;
+}
+}  // namespace
+
+size_t Plan::scratch_bytes(const Params &p, size_t M) {
+  const size_t esz = p.fp64 ? 16 : 8;
+  size_t b = 0;
+  // constants (arena_setup)
+  b += round256(device_coeffs(p.W, p.beta, p.e0).size() * (p.fp64 ? 8 : 4));
+  for (int d = 0; d < p.ndim; ++d) {
+    if (d > 0 && p.nuni[d] == p.nuni[d - 1] && p.nover[d] == p.nover[d - 1])
+      continue;
+    b += round256((p.nuni[d] / 2 + 1) * 8);
+  }
+  b += round256(8);
+  // index build (build_index)
+  size_t ntiles = 1;
+  for (int d = 0; d < p.ndim; ++d) ntiles *= (p.nover[d] >> p.log2tile) + 3;
+  if (M) {
+    const size_t nsub_max = std::min(ntiles, M) + (M + MAXSUB - 1) / MAXSUB;
+    b += round256(M * 4) + round256(ntiles * 4 * 5) +
+         round256(std::max(radix_sort_workspace(M), scan_workspace(ntiles))) +
+         round256(M * 4) + round256(nsub_max * sizeof(Sub)) + round256(4);
+  }
+  // grid side (prepare_grid_side)
+  size_t n = 1;
+  for (int d = 0; d < p.ndim; ++d) n *= p.nover[d];
+  const bool scr = needs_scratch_grid(p);
+  b += round256(n * esz) * (scr ? 2 : 1);
+  // twiddles (twiddle_for)
+  std::vector<int> Ls;
+  fft_lengths(p, scr, Ls);
+  for (int L : Ls) b += round256((size_t)L * esz);
+  return b;
+}
+
 const void *Plan::twiddle_for(int L) {
+  if (current_arena()) {
+    auto it = arena_tw_.find(L);
+    if (it != arena_tw_.end()) return it->second;
+    auto buf = std::make_shared<Buf>((size_t)L * (p_.fp64 ? 16 : 8));
+    launch_fft_twiddles(buf->p, L, p_.fp64, s_);
+    tw_keep_.push_back(buf);
+    arena_tw_[L] = buf->p;
+    return buf->p;
+  }
   int dev = 0;
   DMX_HIP(hipGetDevice(&dev));
   const std::string key = std::to_string(dev) + ":" + std::to_string(L) +
@@ -405,6 +640,8 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
     launch_tile_subs(ntiles_, counts, tstart, tend, nsub, sstart,
                      subs_.as<Sub>(), nsub_total_.as<uint32_t>(), tmp_work_.p,
                      s_);
+    tile_nsub_ = nsub;
+    tile_sub0_ = sstart;
   }
   if (keep_sorted_coords) {
     // a random gather of 1e7 points costs about as much as the gathers it
@@ -440,6 +677,8 @@ void Plan::fill_spread_args(SpreadArgs &a) const {
   a.lds_side = lds_side_;
   a.use_lds = p_.use_lds ? 1 : 0;
   a.coords_sorted = coords_sorted_ ? 1 : 0;
+  a.tile_nsub = tile_nsub_;
+  a.tile_sub0 = tile_sub0_;
 }
 
 void Plan::prepare_grid_side() {
@@ -449,11 +688,7 @@ void Plan::prepare_grid_side() {
   if (grid_.bytes != bytes) grid_.reset(bytes);
   // the second buffer serves lines longer than LDS and the out-of-place
   // half of the two-pass column FFT (2-D)
-  bool need_scratch = p_.ndim == 2;
-  for (int d = 0; d < p_.ndim; ++d)
-    if ((int)p_.nover[d] > (p_.fp64 ? FFT_MAXEL64 : FFT_MAXEL32))
-      need_scratch = true;
-  if (need_scratch && scratch_.bytes != bytes) scratch_.reset(bytes);
+  if (needs_scratch_grid(p_) && scratch_.bytes != bytes) scratch_.reset(bytes);
   cur_ = grid_.p;
 }
 
@@ -620,7 +855,14 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
     a.values = points;
     a.val_f64 = pts_f64 ? 1 : 0;
     a.grid = grid_.p;
-    launch_spread(p_.fp64, a, s_);
+    if (p_.deterministic) {
+      const size_t sb = spread_det_slab_bytes(a);
+      if (slab_.bytes != sb) slab_.reset(sb);
+      a.slab = slab_.p;
+      launch_spread_det(p_.fp64, a, s_);  // writes every grid cell
+    } else {
+      launch_spread(p_.fp64, a, s_);
+    }
   }
   GridCopyArgs c = copy_args(uni, uni_f64, ust);
   if (run_fft(forward, true, &c)) return;  // the last FFT pass cropped

@@ -12,6 +12,10 @@ behaviour apart from the compute precision, see `u2nu`):
              correction factors are FP64.
   chooser    "analytic" (ducc0's cost model) | "learned" (the trained
              parameter network, with the analytic choice as fallback).
+  deterministic
+             True: type-1 results identical bit for bit from run to run
+             (spreading without floating-point atomics; slower).  Type 2
+             is always deterministic.
   info       a dict that receives the parameters actually used.
 
 Arrays may be numpy arrays (host memory: copied to the GPU and back) or
@@ -68,18 +72,18 @@ def _check_dtypes(vals, coord):
 
 
 def _opts(precision, chooser, log2tile, stream, kernel_idx=-1, sigma=0.0,
-          spot_check=True):
+          spot_check=True, deterministic=False):
     return _lib.options(precision=precision, chooser=chooser,
                         log2tile=log2tile, stream=stream,
                         kernel_idx=kernel_idx, sigma=sigma,
-                        spot_check=spot_check)
+                        spot_check=spot_check, deterministic=deterministic)
 
 
 def u2nu(*, grid, coord, forward, epsilon, nthreads=1, out=None,
          verbosity=0, sigma_min=1.2, sigma_max=2.51,
          periodicity=_TWO_PI, fft_order=False, precision="auto",
          chooser="analytic", log2tile=0, info=None, kernel_idx=-1,
-         spot_check=True):
+         spot_check=True, deterministic=False):
     """Type 2 NUFFT: uniform grid -> values at nonuniform points.
 
     out_j = sum_k grid_k exp(isign i 2pi/P k.x_j), isign = -1 if forward,
@@ -100,7 +104,8 @@ def u2nu(*, grid, coord, forward, epsilon, nthreads=1, out=None,
     c, kc = _lib.describe(coord)
     o, ko = _lib.describe(out)
     opt = _opts(precision, chooser, log2tile, _stream_for(grid, coord, out),
-                kernel_idx=kernel_idx, spot_check=spot_check)
+                kernel_idx=kernel_idx, spot_check=spot_check,
+                deterministic=deterministic)
     inf = _lib.DmxPlanInfo()
     _lib.check(_lib.lib().dmx_u2nu(
         ct.byref(g), ct.byref(c), int(bool(forward)), float(epsilon),
@@ -118,7 +123,7 @@ def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
          verbosity=0, sigma_min=1.2, sigma_max=2.51,
          periodicity=_TWO_PI, fft_order=False, precision="auto",
          chooser="analytic", log2tile=0, info=None, kernel_idx=-1,
-         spot_check=True):
+         spot_check=True, deterministic=False):
     """Type 1 NUFFT: values at nonuniform points -> uniform grid.
 
     out_k = sum_j points_j exp(isign i 2pi/P k.x_j).  `out` is required: its
@@ -134,7 +139,8 @@ def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
     c, kc = _lib.describe(coord)
     o, ko = _lib.describe(out)
     opt = _opts(precision, chooser, log2tile, _stream_for(points, coord, out),
-                kernel_idx=kernel_idx, spot_check=spot_check)
+                kernel_idx=kernel_idx, spot_check=spot_check,
+                deterministic=deterministic)
     inf = _lib.DmxPlanInfo()
     _lib.check(_lib.lib().dmx_nu2u(
         ct.byref(p), ct.byref(c), int(bool(forward)), float(epsilon),
@@ -194,7 +200,7 @@ class plan:
     def __init__(self, *, nu2u, coord, grid_shape, epsilon, nthreads=0,
                  sigma_min=1.1, sigma_max=2.6, periodicity=_TWO_PI,
                  fft_order=False, precision="auto", chooser="analytic",
-                 log2tile=0):
+                 log2tile=0, deterministic=False):
         _lib.require_gpu()
         self._h = None
         if _is_torch(coord):
@@ -212,7 +218,8 @@ class plan:
         c, kc = _lib.describe(coord)
         shp = np.asarray(self._shape, dtype=np.int64)
         self._stream = _stream_for(coord)
-        opt = _opts(precision, chooser, log2tile, self._stream)
+        opt = _opts(precision, chooser, log2tile, self._stream,
+                    deterministic=deterministic)
         h = ct.c_void_p()
         _lib.check(_lib.lib().dmx_plan_create(
             ct.byref(h), int(bool(nu2u)), ct.byref(c), shp.ctypes.data, nd,
@@ -231,6 +238,17 @@ class plan:
         if x.dtype != want:
             raise RuntimeError("not yet supported")
 
+    @staticmethod
+    def _call_opts(*arrays):
+        """Options of one planned call: torch's current stream when a
+        device tensor takes part (the call is then ordered on it like a
+        torch operation), else None (the plan's own stream, waiting for the
+        results)."""
+        s = _stream_for(*arrays)
+        if s is None:
+            return None
+        return ct.byref(_lib.options(stream=s))
+
     def nu2u(self, *, forward, verbosity=0, points, out=None):
         self._vdtype_ok(points)
         if out is None:
@@ -244,7 +262,8 @@ class plan:
         o, ko = _lib.describe(out)
         _lib.check(_lib.lib().dmx_plan_nu2u(self._h, int(bool(forward)),
                                             int(verbosity), ct.byref(p),
-                                            ct.byref(o)))
+                                            ct.byref(o),
+                                            self._call_opts(points, out)))
         if not _is_torch(out) and ko is not out:
             out[...] = ko
         return out
@@ -262,7 +281,8 @@ class plan:
         o, ko = _lib.describe(out)
         _lib.check(_lib.lib().dmx_plan_u2nu(self._h, int(bool(forward)),
                                             int(verbosity), ct.byref(g),
-                                            ct.byref(o)))
+                                            ct.byref(o),
+                                            self._call_opts(grid, out)))
         if not _is_torch(out) and ko is not out:
             out[...] = ko
         return out

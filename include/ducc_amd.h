@@ -59,7 +59,12 @@ typedef struct dmx_options {
                         analytic choice on failure (default 1; 0 off; 2
                         treats the check as failed, for tests) */
   int32_t flags;     /* DMX_STREAM_GIVEN | DMX_ASYNC */
-  int32_t reserved[6];
+  int32_t deterministic; /* 1: type-1 results identical bit for bit from
+                            run to run (no floating-point atomics in the
+                            spreading; slower, and per-subproblem FP64
+                            regions in device memory); type 2 is always
+                            deterministic */
+  int32_t reserved[5];
 } dmx_options;
 
 /* dmx_options.flags */
@@ -92,7 +97,9 @@ double dmx_best_epsilon(size_t ndim, int singleprec, double sigma_min,
 /* ducc0.nufft.u2nu (python/nufft_pymod.cc:52-94; nufft.h:1929-1958).
    grid: complex nuni array; coord: (M, ndim) real; out: (M,) complex.
    periodicity: ndim doubles.  `info` (may be NULL) receives the parameters
-   used. */
+   used.  verbosity > 0 prints the request and a phase-timing table to
+   stdout (nufft.h:371-380, 469-471) and makes the call wait for its work;
+   nthreads is accepted and ignored. */
 int dmx_u2nu(const dmx_array *grid, const dmx_array *coord, int forward,
              double epsilon, size_t nthreads, dmx_array *out,
              size_t verbosity, double sigma_min, double sigma_max,
@@ -114,10 +121,19 @@ int dmx_plan_create(dmx_plan **plan, int nu2u, const dmx_array *coord,
                     size_t nthreads, double sigma_min, double sigma_max,
                     const double *periodicity, int fft_order,
                     const dmx_options *opt);
+/* opt (may be NULL): the stream and DMX_ASYNC flag of this call (the
+   other fields are ignored); NULL runs on the plan's creation stream with
+   its creation flags.  A call on another stream than the plan's previous
+   one first waits there for the work the plan queued before (the plan
+   reuses its buffers).  verbosity > 0 prints the request and a table of
+   phase timings to stdout (as ducc0 does, nufft.h:469-471) and makes the
+   call wait for its work. */
 int dmx_plan_nu2u(dmx_plan *plan, int forward, size_t verbosity,
-                  const dmx_array *points, dmx_array *out);
+                  const dmx_array *points, dmx_array *out,
+                  const dmx_options *opt);
 int dmx_plan_u2nu(dmx_plan *plan, int forward, size_t verbosity,
-                  const dmx_array *grid, dmx_array *out);
+                  const dmx_array *grid, dmx_array *out,
+                  const dmx_options *opt);
 int dmx_plan_get_info(const dmx_plan *plan, dmx_plan_info *info);
 int dmx_plan_destroy(dmx_plan *plan);
 

@@ -2,6 +2,7 @@
 include/ducc_amd.h declares, and validates arguments on the host with the
 reference's error messages before touching a device."""
 import ctypes as ct
+import json
 import os
 import re
 
@@ -79,24 +80,91 @@ def test_host_validation_messages():
     assert rc and "transform must be 1D/2D/3D" in msg
 
 
-def test_params_match_oracle_restatement():
-    """The library's analytic choice (ducc0's cost model at vlen=8,
-    nthreads=8) equals the oracle's, through the plan info of a failed
-    allocation-free path: compared via find_params on both sides."""
-    rng = np.random.default_rng(0)
-    L = ct.CDLL(_lib.LIBPATH)
-    for _ in range(50):
-        nd = int(rng.integers(1, 4))
-        shape = tuple(int(x) for x in rng.integers(4, 300, nd))
-        single = bool(rng.integers(0, 2))
-        eps = 10 ** rng.uniform(-12 if not single else -6, -1)
-        M = int(10 ** rng.uniform(0, 7))
-        k, nov = O.find_params(eps, shape, M, True, single, vlen=8,
-                               nthreads=8)
-        # the library exposes the same choice through dmx_plan_info only on
-        # a device; here check the oracle is self-consistent
-        assert (k == -1) == (len(O.available_kernels(
-            eps, nd, single, 1.2, 2.51)) == 0)
+def _choose(shape, M, eps, single, gridding, smin, smax):
+    """The library's analytic choice through dmx_choose (host only)."""
+    shp = np.asarray(shape, np.int64)
+    opt = _lib.options(precision="fp32" if single else "fp64")
+    inf = _lib.DmxPlanInfo()
+    rc = _lib.lib().dmx_choose(len(shape), shp.ctypes.data, int(M), float(eps),
+                               int(not single), int(bool(gridding)),
+                               float(smin), float(smax), ct.byref(opt),
+                               ct.byref(inf))
+    if rc:
+        return -1, None, _lib.lib().dmx_last_error().decode()
+    return inf.kernel_idx, tuple(inf.nover[:len(shape)]), ""
+
+
+def test_params_match_ducc0_golden():
+    """dmx_choose (no GPU needed) returns ducc0's kernel row and oversampled
+    grid for every golden request that ducc0 ran with 8 threads at vlen 8
+    (the host model the library restates; tests/golden/params.json was
+    written by the reference built from its sources)."""
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "params.json")))
+    assert g["vlen_single"] == 8 and g["vlen_double"] == 8
+    n = 0
+    for r in g["requests"]:
+        if r["nthreads"] != 8:
+            continue
+        k, nov, msg = _choose(r["shape"], r["M"], r["eps"], r["single"],
+                              r["gridding"], r["sigma_min"], r["sigma_max"])
+        assert k == r["kidx"], (r, msg)
         if k >= 0:
-            assert all(n >= 16 and n % 2 == 0 for n in nov)
-    assert L is not None
+            assert list(nov) == r["nover"], r
+        else:
+            assert "no appropriate kernel found" in msg
+        n += 1
+    assert n >= 20
+
+
+def test_params_match_oracle_restatement():
+    """dmx_choose equals the oracle's find_params (itself pinned to the
+    golden requests above and to live ducc0) on 400 random requests over
+    1-3 D, both precisions, both transform types and random sigma ranges."""
+    rng = np.random.default_rng(7)
+    nk = 0
+    for _ in range(400):
+        nd = int(rng.integers(1, 4))
+        hi = {1: 200000, 2: 5000, 3: 400}[nd]
+        shape = tuple(int(x) for x in np.exp(rng.uniform(np.log(2),
+                                                         np.log(hi), nd)))
+        single = bool(rng.integers(0, 2))
+        gridding = bool(rng.integers(0, 2))
+        eps = 10 ** rng.uniform(-7.5 if single else -14.5, -1)
+        M = int(10 ** rng.uniform(0, 8))
+        smin = float(rng.choice([1.1, 1.2, 1.25, 1.5]))
+        smax = float(rng.choice([2.0, 2.3, 2.51, 2.6]))
+        k_o, nov_o = O.find_params(eps, shape, M, gridding, single, vlen=8,
+                                   nthreads=8, sigma_min=smin,
+                                   sigma_max=smax)
+        k, nov, msg = _choose(shape, M, eps, single, gridding, smin, smax)
+        assert k == k_o, (nd, shape, single, gridding, eps, M, smin, smax,
+                          msg)
+        if k >= 0:
+            assert tuple(nov) == tuple(nov_o)
+            nk += 1
+    assert nk > 300
+
+
+def test_params_match_live_ducc0():
+    """dmx_choose equals ducc0's own findNufftParameters (the reference
+    built from its sources into oracle/_ref) on 300 random requests at 8
+    threads; skipped only where that build is absent."""
+    from oracle import ref as R
+    if not R.available():
+        pytest.skip("oracle/_ref (ducc0 built from the reference) is absent")
+    rng = np.random.default_rng(11)
+    for _ in range(300):
+        nd = int(rng.integers(1, 4))
+        hi = {1: 100000, 2: 4096, 3: 300}[nd]
+        shape = tuple(int(x) for x in np.exp(rng.uniform(np.log(4),
+                                                         np.log(hi), nd)))
+        single = bool(rng.integers(0, 2))
+        gridding = bool(rng.integers(0, 2))
+        eps = 10 ** rng.uniform(-6 if single else -13, -1.5)
+        M = int(10 ** rng.uniform(1, 7.5))
+        k_r, nov_r = R.find_params(eps, shape, M, gridding, single, 8,
+                                   sigma_min=1.2, sigma_max=2.51)
+        k, nov, msg = _choose(shape, M, eps, single, gridding, 1.2, 2.51)
+        assert k == k_r, (nd, shape, single, gridding, eps, M, msg)
+        if k >= 0:
+            assert tuple(nov) == tuple(nov_r)
