@@ -103,6 +103,10 @@ _SIG = {
     "dmx_profile_read": (ct.c_int, [ct.c_char_p, ct.c_size_t]),
     "dmx_c2c": (ct.c_int, [_AP, _AP, _P, ct.c_size_t, ct.c_int, ct.c_double,
                            ct.c_size_t, ct.POINTER(DmxOptions)]),
+    "dmx_nufft_noalloc": (ct.c_int, [ct.c_int, _AP, _AP, _AP, ct.c_int,
+                                     ct.c_double, ct.c_double, ct.c_double,
+                                     _P, ct.c_int, ct.POINTER(DmxOptions),
+                                     ct.POINTER(DmxPlanInfo)]),
     "dmx_device_count": (ct.c_int, []),
 }
 

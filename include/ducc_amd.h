@@ -224,6 +224,26 @@ int dmx_profile_enable(int on);
 int dmx_profile_reset(void);
 int dmx_profile_read(char *buf, size_t buflen);
 
+/* Allocation-free type 1 / type 2 transform (no counterpart in the
+   reference; for callers that cannot allocate: code under HIP graph
+   capture, tight serving loops, harnesses that forbid allocation).  A
+   variant of dmx_nu2u (type 1: points -> uniform) and dmx_u2nu (type 2:
+   uniform -> points) with the same argument meaning and validation, except:
+   every array must be in device memory (coord and points C-contiguous);
+   the analytic parameter choice only (opt->chooser must be 0); it runs on
+   opt->stream (NULL: the null stream) and waits for its work unless
+   DMX_ASYNC is set.  All temporaries come from one static device array of
+   DMX_NOALLOC_SCRATCH_BYTES per GPU: a request needing more is rejected
+   with an error before any work is queued.  Calls that overlap on one GPU
+   share that array and race. */
+#define DMX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dmx_nufft_noalloc(int type, const dmx_array *uniform,
+                      const dmx_array *coord, const dmx_array *points,
+                      int forward, double epsilon, double sigma_min,
+                      double sigma_max, const double *periodicity,
+                      int fft_order, const dmx_options *opt,
+                      dmx_plan_info *info);
+
 /* Number of HIP devices visible (0 when none). */
 int dmx_device_count(void);
 
