@@ -27,9 +27,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
-# ds_add_f64 with the lanes of each quarter-wave on distinct bank classes,
-# measured on MI355X by ducc_amd/csrc/probes/lds_atomics.hip (the rate the
-# spreader's LDS accumulation can reach; random addresses reach a third)
+# ds_add_f64 with the lanes of each quarter-wave on distinct bank classes:
+# a rate measured on MI355X with a microbenchmark (profiles/lds_atomics.md),
+# not a published figure; random addresses reach a third of it
 PEAK_LDS_F64_ATOMICS = 4.66e12
 
 
@@ -95,9 +95,21 @@ def time_steps(step, steps, warmup, sync):
     return time.perf_counter() - t0
 
 
+def _median_time(fn, reps=5):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
 def cpu_baseline(coords, pts, grid, eps, M_total, fast=None):
     """ducc0 (compiled from the reference sources into oracle/_ref) on all
-    host cores, on the same workload; with `fast` = (type-1 result, type-2
+    host cores, on the same workload: after one warm-up, the median of 5
+    unplanned nu2u + u2nu passes (the value), and the median of 5 planned
+    execute-only passes of ducc0's plan (its plan needs coordinates in the
+    values' precision: float32 here).  With `fast` = (type-1 result, type-2
     result) of the GPU fast path, also their L2 errors against ducc0's own
     double-precision nu2u / u2nu of the same inputs."""
     try:
@@ -108,13 +120,30 @@ def cpu_baseline(coords, pts, grid, eps, M_total, fast=None):
         return None
     nth = os.cpu_count() or 1
     shape = grid.shape
-    # warm-up on a small slice, then one full nu2u + u2nu
-    R.nu2u(pts[:10000], coords[:10000], True, eps, shape, nthreads=nth)
-    t0 = time.perf_counter()
     R.nu2u(pts, coords, True, eps, shape, nthreads=nth)
-    t1 = time.perf_counter()
     R.u2nu(grid, coords, False, eps, nthreads=nth)
-    t2 = time.perf_counter()
+    t1 = _median_time(lambda: R.nu2u(pts, coords, True, eps, shape,
+                                     nthreads=nth))
+    t2 = _median_time(lambda: R.u2nu(grid, coords, False, eps, nthreads=nth))
+    planned = None
+    try:
+        c32 = coords.astype(np.float32)
+        t0 = time.perf_counter()
+        p1 = R.Plan(True, c32, shape, eps, True, nthreads=nth,
+                    sigma_min=1.2, sigma_max=2.51)
+        p2 = R.Plan(False, c32, shape, eps, True, nthreads=nth,
+                    sigma_min=1.2, sigma_max=2.51)
+        tp = time.perf_counter() - t0
+        p1.nu2u(pts)
+        p2.u2nu(grid, False)
+        e1 = _median_time(lambda: p1.nu2u(pts))
+        e2 = _median_time(lambda: p2.u2nu(grid, False))
+        planned = dict(value=2 * len(pts) / (e1 + e2), unit="points/s",
+                       plan_s=round(tp, 4), nu2u_s=round(e1, 4),
+                       u2nu_s=round(e2, 4))
+        del p1, p2
+    except Exception as e:  # the baseline must not sink the bench line
+        planned = dict(error=str(e))
     err = None
     if fast is not None:
         # eps 1e-5 in double precision (W from ducc0's FP64 table)
@@ -123,12 +152,14 @@ def cpu_baseline(coords, pts, grid, eps, M_total, fast=None):
         r2 = R.u2nu(grid.astype(np.complex128), coords, False, eps,
                     nthreads=nth)
         err = dict(nu2u=l2rel(fast[0], r1), u2nu=l2rel(fast[1], r2))
-    return dict(value=2 * len(pts) / (t2 - t0), unit="points/s", cores=nth,
+    return dict(value=2 * len(pts) / (t1 + t2), unit="points/s", cores=nth,
                 kind="reference", err_fast_vs_ducc0_fp64=err,
+                planned_execute_only=planned,
                 sample=(f"ducc0 nu2u + u2nu, full workload (M={len(pts)}, "
                         f"{shape[0]}x{shape[1]}, eps={eps}, complex64, "
-                        f"{nth} threads), one timed pass after a warm-up; "
-                        f"nu2u {t1 - t0:.3f} s, u2nu {t2 - t1:.3f} s"))
+                        f"{nth} threads): median of 5 unplanned calls each "
+                        f"after a warm-up, nu2u {t1:.3f} s, u2nu {t2:.3f} s; "
+                        f"planned execute-only in planned_execute_only"))
 
 
 def c2c_cpu(R, a, nthreads, reps):
@@ -317,15 +348,42 @@ def main():
         nufft.u2nu(grid=g32, coord=c, forward=False, epsilon=eps, out=op32,
                    chooser=chooser, info=info2)
 
+    # timed with the phase profiler off; the phase times come from a second
+    # pass of the same steps with HIP events around each phase
+    t = time_steps(step_fast, args.steps, args.warmup, sync)
+    value = 2 * M * args.steps / t
     _lib.profile_enable(True)
-    for _ in range(args.warmup):
+    _lib.profile_reset()
+    for _ in range(args.steps):
         step_fast()
     sync()
-    _lib.profile_reset()
-    t = time_steps(step_fast, args.steps, 0, sync)
     prof = _lib.profile_read()
     _lib.profile_enable(False)
-    value = 2 * M * args.steps / t
+
+    # planned execute-only (sort once per plan, as ducc0.nufft.plan), the
+    # same two transforms per step; a plan takes coordinates in the values'
+    # precision (nufft_pymod.cc:145-150), float32 here
+    c32 = c.to(torch.float32)
+    pl1 = nufft.plan(nu2u=True, coord=c32, grid_shape=(n, n), epsilon=eps,
+                     sigma_min=1.2, sigma_max=2.51, chooser=chooser)
+    pl2 = nufft.plan(nu2u=False, coord=c32, grid_shape=(n, n), epsilon=eps,
+                     sigma_min=1.2, sigma_max=2.51, chooser=chooser)
+
+    # own output buffers: og32 / op32 keep the unplanned results, whose
+    # accuracy is checked below
+    ogp = torch.empty_like(og32)
+    opp = torch.empty_like(op32)
+
+    def step_planned():
+        pl1.nu2u(points=p32, forward=True, out=ogp)
+        pl2.u2nu(grid=g32, forward=False, out=opp)
+    tpl = time_steps(step_planned, args.steps, args.warmup, sync)
+    planned = dict(value=2 * M * args.steps / tpl,
+                   ms_per_step=1e3 * tpl / args.steps,
+                   note="ducc0.nufft.plan semantics: index built once per "
+                        "plan (float32 coordinates); each step executes "
+                        "nu2u + u2nu")
+    del pl1, pl2, c32, ogp, opp
 
     # plain FP64 path on the same geometry
     fp64 = None
@@ -361,6 +419,7 @@ def main():
         fp64 = dict(value=v64, ms_per_step=1e3 * t64 / max(1, args.steps // 2),
                     W=info3.get("W"), nover=info3.get("nover"),
                     speedup=value / v64,
+                    alg_bytes_per_step=None, bytes_ratio_fp64_over_fp32=None,
                     err_fast_vs_fp64=dict(nu2u=l2(a1, b1), u2nu=l2(a2, b2)),
                     err_vs_direct_nudft=dict(
                         nu2u_fast=l2(a1.ravel()[sel_m], d1),
@@ -414,7 +473,9 @@ def main():
         roofline["lds_atomics"] = dict(
             per_launch=int(na), achieved=float(f"{ra:.4g}"),
             peak=PEAK_LDS_F64_ATOMICS, unit="atomics/s",
-            frac=round(ra / PEAK_LDS_F64_ATOMICS, 4))
+            frac=round(ra / PEAK_LDS_F64_ATOMICS, 4),
+            peak_source="self-measured microbenchmark rate "
+                        "(profiles/lds_atomics.md), not a published figure")
 
     # whole-step roofline: per call M*(d*s_c + s_v) + N*s_v +
     # (2d+2)*N_over*s_v (coordinates and values once; the uniform array once;
@@ -423,6 +484,15 @@ def main():
         n_over = int(np.prod(inf["nover"]))
         return M * (d * 8 + sv) + n * n * sv + (2 * d + 2) * n_over * sv
     step_b = call_bytes(info1) + call_bytes(info2)
+    if fp64 is not None:
+        # the same formula with 16-byte values on the FP64 path's grids:
+        # an HBM-bound FP32 path can be at most this ratio faster
+        def call_bytes64(inf):
+            n_over = int(np.prod(inf["nover"]))
+            return M * (d * 8 + 16) + n * n * 16 + (2 * d + 2) * n_over * 16
+        b64 = call_bytes64(info3) + call_bytes64(info4)
+        fp64["alg_bytes_per_step"] = int(b64)
+        fp64["bytes_ratio_fp64_over_fp32"] = round(b64 / step_b, 3)
     step_gbs = step_b * args.steps / t / 1e9
     step_roofline = dict(alg_bytes_per_step=int(step_b),
                          achieved=round(step_gbs, 1), unit="GB/s",
@@ -454,7 +524,7 @@ def main():
                               log2tile=info2.get("log2tile"),
                               learned=info2.get("chooser_used"))),
         roofline=roofline, step_roofline=step_roofline, cpu_baseline=cpu,
-        fp64_plain=fp64,
+        fp64_plain=fp64, planned=planned,
         phases_ms_per_step={k: round(v[0] / args.steps, 4)
                             for k, v in prof.items()})
     print(json.dumps(line))

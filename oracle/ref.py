@@ -36,6 +36,12 @@ def lib():
         L.ref_nu2u.argtypes = [I, S, P, I, P, I, I, D, S, P, P, D, D, P, I]
         L.ref_u2nu.argtypes = L.ref_nu2u.argtypes
         L.ref_c2c.argtypes = [I, I, P, P, P, P, I, I, D, S]
+        L.ref_plan_create.argtypes = [ct.POINTER(P), I, I, S, P, I, P, D, S,
+                                      D, D, P, I]
+        L.ref_plan_nu2u.argtypes = [P, I, P, P]
+        L.ref_plan_u2nu.argtypes = [P, I, P, P]
+        L.ref_plan_destroy.argtypes = [P]
+        L.ref_plan_destroy.restype = None
         _lib = L
     return _lib
 
@@ -154,3 +160,50 @@ def c2c(a, axes=None, forward=True, fct=1.0, nthreads=1):
                          _p(ax), len(ax), int(bool(forward)), float(fct),
                          int(nthreads)))
     return out
+
+
+class Plan:
+    """ducc0.nufft.plan (nufft_pymod.cc:139-261) over the reference's Nufft
+    class: coordinates and values in one precision (float32 / complex64
+    when `single`)."""
+
+    def __init__(self, nu2u, coord, grid_shape, epsilon, single,
+                 nthreads=1, sigma_min=1.1, sigma_max=2.6,
+                 periodicity=2 * np.pi, fft_order=False):
+        self.single = bool(single)
+        self.c = np.ascontiguousarray(coord, np.float32 if single
+                                      else np.float64)
+        self.shape = tuple(int(x) for x in grid_shape)
+        self.M = self.c.shape[0]
+        per = np.ascontiguousarray(np.broadcast_to(
+            np.asarray(periodicity, float), (len(self.shape),)))
+        h = ct.c_void_p()
+        _check(lib().ref_plan_create(
+            ct.byref(h), int(bool(nu2u)), len(self.shape), self.M,
+            _p(self.c), int(self.single),
+            _p(np.asarray(self.shape, np.uintp)), float(epsilon),
+            int(nthreads), float(sigma_min), float(sigma_max), _p(per),
+            int(bool(fft_order))))
+        self.h = h
+
+    def _vt(self):
+        return np.complex64 if self.single else np.complex128
+
+    def nu2u(self, points, forward=True):
+        v = np.ascontiguousarray(points, self._vt())
+        out = np.zeros(self.shape, self._vt())
+        _check(lib().ref_plan_nu2u(self.h, int(bool(forward)), _p(v),
+                                   _p(out)))
+        return out
+
+    def u2nu(self, grid, forward=True):
+        g = np.ascontiguousarray(grid, self._vt())
+        out = np.zeros(self.M, self._vt())
+        _check(lib().ref_plan_u2nu(self.h, int(bool(forward)), _p(g),
+                                   _p(out)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ref_plan_destroy(self.h)
+            self.h = None

@@ -11,6 +11,7 @@
 //   bestEpsilon                     src/ducc0/math/gridding_kernel.cc:1562
 //   build_index (coord_idx order)   src/ducc0/nufft/nufft.h:892-907,1364,1862
 //   c2c                             src/ducc0/fft/fft.h:848-850
+//   plan (Nufft class)              src/ducc0/nufft/nufft.h:437-503
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -91,9 +92,79 @@ void do_u2nu(int ndim, size_t M, const Tc *coords, const complex<T> *grid,
   u2nu<T, T>(c, g, forward != 0, eps, nthreads, o, 0, smin, smax, per,
              fft_order != 0);
 }
+// a ducc0 plan (Nufft<T, T, T, nd>: values and coordinates in one
+// precision, as ducc0.nufft.plan requires, nufft_pymod.cc:145-150)
+struct RefPlanBase {
+  virtual ~RefPlanBase() = default;
+  virtual void nu2u(int forward, const void *pts, void *out) = 0;
+  virtual void u2nu(int forward, const void *grid, void *out) = 0;
+};
+template <typename T, size_t nd> struct RefPlan : RefPlanBase {
+  size_t M;
+  std::array<size_t, nd> shp;
+  Nufft<T, T, T, nd> plan;
+  RefPlan(bool gridding, size_t M_, const T *coords,
+          const std::array<size_t, nd> &shp_, double eps, size_t nthreads,
+          double smin, double smax, const vector<double> &per, bool fo)
+      : M(M_), shp(shp_),
+        plan(gridding, cmav<T, 2>(coords, {M_, nd}), shp_, eps, nthreads,
+             smin, smax, per, fo) {}
+  void nu2u(int forward, const void *pts, void *out) override {
+    cmav<complex<T>, 1> p((const complex<T> *)pts, {M});
+    vmav<complex<T>, nd> o((complex<T> *)out, shp);
+    plan.nu2u(forward != 0, 0, p, o);
+  }
+  void u2nu(int forward, const void *grid, void *out) override {
+    cmav<complex<T>, nd> g((const complex<T> *)grid, shp);
+    vmav<complex<T>, 1> o((complex<T> *)out, {M});
+    plan.u2nu(forward != 0, 0, g, o);
+  }
+};
+template <typename T, size_t nd>
+RefPlanBase *make_plan(int nu2u, size_t M, const void *coords,
+                       const size_t *nuni, double eps, size_t nthreads,
+                       double smin, double smax, const double *period,
+                       int fft_order) {
+  std::array<size_t, nd> shp;
+  for (size_t d = 0; d < nd; ++d) shp[d] = nuni[d];
+  vector<double> per(period, period + nd);
+  return new RefPlan<T, nd>(nu2u != 0, M, (const T *)coords, shp, eps,
+                            nthreads, smin, smax, per, fft_order != 0);
+}
 }  // namespace
 
 extern "C" {
+
+// ducc0.nufft.plan (nufft_pymod.cc:139-261): coords (M, ndim) and values in
+// one precision (single != 0: float32 / complex64)
+int ref_plan_create(void **plan, int nu2u, int ndim, size_t M,
+                    const void *coords, int single, const size_t *nuni,
+                    double eps, size_t nthreads, double smin, double smax,
+                    const double *period, int fft_order) {
+  return guard([&] {
+    *plan = nullptr;
+#define REF_MK(T, ND)                                                        \
+  *plan = make_plan<T, ND>(nu2u, M, coords, nuni, eps, nthreads, smin, smax, \
+                           period, fft_order)
+    if (single) {
+      if (ndim == 1) REF_MK(float, 1);
+      else if (ndim == 2) REF_MK(float, 2);
+      else REF_MK(float, 3);
+    } else {
+      if (ndim == 1) REF_MK(double, 1);
+      else if (ndim == 2) REF_MK(double, 2);
+      else REF_MK(double, 3);
+    }
+#undef REF_MK
+  });
+}
+int ref_plan_nu2u(void *plan, int forward, const void *points, void *out) {
+  return guard([&] { ((RefPlanBase *)plan)->nu2u(forward, points, out); });
+}
+int ref_plan_u2nu(void *plan, int forward, const void *grid, void *out) {
+  return guard([&] { ((RefPlanBase *)plan)->u2nu(forward, grid, out); });
+}
+void ref_plan_destroy(void *plan) { delete (RefPlanBase *)plan; }
 
 const char *ref_last_error() { return g_err; }
 
