@@ -32,7 +32,7 @@ class DmxOptions(ct.Structure):
                 ("flags", ct.c_int32), ("deterministic", ct.c_int32),
                 ("reserved", ct.c_int32 * 5)]
 
-STREAM_GIVEN, ASYNC = 1, 2
+STREAM_GIVEN, ASYNC, POINT_RECORDS = 1, 2, 4
 
 
 class DmxPlanInfo(ct.Structure):
@@ -86,6 +86,8 @@ _SIG = {
                              _P, ct.c_int, _P, ct.c_size_t, _AP]),
     "dmx_sort_points": (ct.c_int, [_AP, _P, ct.c_size_t, ct.c_int, ct.c_int,
                                    _P, _AP, _AP]),
+    "dmx_sort_records": (ct.c_int, [_AP, _P, ct.c_size_t, ct.c_int, ct.c_int,
+                                    _P, _AP, _AP, _AP]),
     "dmx_mlp_nparam": (ct.c_size_t, []),
     "dmx_mlp_forward": (ct.c_int, [_P, _P, ct.c_int, _P]),
     "dmx_mlp_grad": (ct.c_int, [_P, _P, _P, _P, ct.c_int, _P, _P]),
@@ -107,6 +109,16 @@ _SIG = {
                                      ct.c_double, ct.c_double, ct.c_double,
                                      _P, ct.c_int, ct.POINTER(DmxOptions),
                                      ct.POINTER(DmxPlanInfo)]),
+    "dmx_sht_synthesis_general": (
+        ct.c_int, [_AP, _AP, ct.c_size_t, ct.c_size_t, _P, ct.c_size_t,
+                   ct.c_int64, _AP, ct.c_double, ct.c_double, ct.c_double,
+                   ct.c_size_t, ct.c_int, ct.c_size_t,
+                   ct.POINTER(DmxOptions)]),
+    "dmx_sht_adjoint_synthesis_general": (
+        ct.c_int, [_AP, _AP, ct.c_size_t, ct.c_size_t, _P, ct.c_size_t,
+                   ct.c_int64, _AP, ct.c_double, ct.c_double, ct.c_double,
+                   ct.c_size_t, ct.c_int, ct.c_size_t,
+                   ct.POINTER(DmxOptions)]),
     "dmx_device_count": (ct.c_int, []),
 }
 
@@ -213,7 +225,7 @@ def profile_read():
 
 def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
             fp64_below=1e-6, stream=None, sigma=0.0, spot_check=True,
-            deterministic=False):
+            deterministic=False, point_records=False):
     o = DmxOptions()
     lib().dmx_default_options(ct.byref(o))
     o.spot_check = 2 if spot_check == "fail" else int(bool(spot_check))
@@ -233,4 +245,6 @@ def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
         o.flags = STREAM_GIVEN | ASYNC
     o.sigma = float(sigma)
     o.deterministic = int(bool(deterministic))
+    if point_records:
+        o.flags |= POINT_RECORDS
     return o

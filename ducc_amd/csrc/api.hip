@@ -22,6 +22,7 @@
 #include "mlp.h"
 #include "params.h"
 #include "runtime.h"
+#include "sht.h"
 
 namespace dmx {
 
@@ -457,12 +458,19 @@ int run_unplanned(bool type1, const dmx_array *uni, const dmx_array *coord,
       DMX_HIP(hipEventRecord(ax.join, ax.s));
       plan->set_stream(s);
     }
-    plan->build_index(cin.p, cf32, M, false);
+    // on request, the points' records are sorted (type 1 with their
+    // values) where the request fits the record layout
+    std::unique_ptr<DevIn> pin;
+    if (type1) pin = std::make_unique<DevIn>(*pts, s, false);
+    const bool recs =
+        !grad && (opt.flags & DMX_POINT_RECORDS) &&
+        plan->build_records(cin.p, cf32, M, type1, type1 ? pin->p : nullptr,
+                            vf64);
+    if (!recs) plan->build_index(cin.p, cf32, M, false);
     if (fork) DMX_HIP(hipStreamWaitEvent(s, aux().join, 0));
     if (type1) {
-      DevIn pin(*pts, s, false);
       DevOut uo(*uni, s, true);
-      plan->nu2u(forward != 0, pin.p, vf64, uo.p, vf64, uo.st);
+      plan->nu2u(forward != 0, pin->p, vf64, uo.p, vf64, uo.st);
       uo.finish(s, !async);
     } else if (!grad) {
       DevOut po(*pts, s, false);
@@ -826,6 +834,178 @@ int dmx_plan_destroy(dmx_plan *plan) {
   return guard([&] { delete plan; });
 }
 
+namespace {
+// Shared set-up of the general synthesis and its adjoint: validation
+// (sht_pymod.cc:805-870, sht.cc:2926-2937), the ring / mode geometry, the
+// device tables and the NUFFT parameters of the doubled-sphere grid
+// (SphereInterpol's kernel choice, sphere_interpol.h:426-428: the gridding
+// cost model on the grid the points are interpolated from).
+struct ShtJob {
+  ShtGeometry geo;
+  Params p;
+  size_t M = 0;
+  bool vf64 = true;
+  Buf mfac, mstart, accofs;
+  size_t nacc = 0;
+  ShtArgs args{};
+  ShtJob(const dmx_array &alm, const dmx_array &map, size_t spin,
+         size_t lmax, const int64_t *mstart_h, size_t nm, int64_t lstride,
+         const dmx_array &loc, double epsilon, double smin, double smax,
+         const dmx_options &opt) {
+    DMX_ASSERT(loc.ndim == 2 && loc.shape[1] == 2,
+               "last dimension of loc must have size 2");
+    DMX_ASSERT(loc.typecode == DMX_F64, "unsupported combination of data types");
+    DMX_ASSERT(is_complex(alm.typecode), "unsupported combination of data types");
+    vf64 = alm.typecode == DMX_C128;
+    DMX_ASSERT(map.typecode == (vf64 ? DMX_F64 : DMX_F32),
+               "unsupported combination of data types");
+    DMX_ASSERT(spin == 0, "not yet supported: spin > 0");
+    DMX_ASSERT(nm > 0 && mstart_h, "need at least m=0");
+    DMX_ASSERT(nm <= lmax + 1, "mmax>lmax");
+    DMX_ASSERT(alm.ndim == 2 && alm.shape[0] == 1,
+               "number of components mismatch in alm");
+    DMX_ASSERT(map.ndim == 2 && map.shape[0] == 1,
+               "number of components mismatch in map");
+    M = (size_t)loc.shape[0];
+    DMX_ASSERT(map.shape[1] == (int64_t)M, "array shape mismatch");
+    // every a_lm index inside the array (sht_pymod.cc:168-180)
+    int64_t need = 0;
+    for (size_t m = 0; m < nm; ++m) {
+      const int64_t lo = mstart_h[m] + (int64_t)m * lstride,
+                    hi = mstart_h[m] + (int64_t)lmax * lstride;
+      DMX_ASSERT(lo >= 0 && hi >= 0, "impossible a_lm memory layout");
+      need = std::max(need, std::max(lo, hi) + 1);
+    }
+    DMX_ASSERT(alm.shape[1] >= need, "a_lm array too small");
+    geo = sht_geometry(lmax, nm - 1);
+    size_t nuni[3] = {(size_t)(2 * geo.N - 2), (size_t)geo.nphi, 1};
+    const double per[2] = {2 * M_PI, 2 * M_PI};
+    p = choose_params(2, nuni, M, true, epsilon, smin, smax, per, true, vf64,
+                      opt);
+    std::vector<double> mf;
+    sht_mfac(nm - 1, mf);
+    std::vector<int64_t> ao(nm);
+    for (size_t m = 0; m < nm; ++m) {
+      ao[m] = (int64_t)nacc;
+      nacc += lmax + 1 - m;
+    }
+    mfac.reset(nm * 8);
+    mstart.reset(nm * 8);
+    accofs.reset(nm * 8);
+    DMX_HIP(hipMemcpy(mfac.p, mf.data(), nm * 8, hipMemcpyHostToDevice));
+    DMX_HIP(hipMemcpy(mstart.p, mstart_h, nm * 8, hipMemcpyHostToDevice));
+    DMX_HIP(hipMemcpy(accofs.p, ao.data(), nm * 8, hipMemcpyHostToDevice));
+    args.alm_f64 = vf64 ? 1 : 0;
+    args.mstart = mstart.as<int64_t>();
+    args.lstride = lstride;
+    args.lmax = geo.lmax;
+    args.mmax = geo.mmax;
+    args.N = geo.N;
+    args.nphi = geo.nphi;
+    args.mfac = mfac.as<double>();
+    args.accofs = accofs.as<int64_t>();
+  }
+  size_t rows() const { return (size_t)(2 * geo.N - 2); }
+  // the doubled-sphere grid's c2c over its rows (theta), scaled 1/(2N-2)
+  std::shared_ptr<void> theta_fft(void *G, bool forward, hipStream_t s) {
+    const int64_t st[2] = {geo.nphi, 1};
+    const int64_t shape[2] = {(int64_t)rows(), geo.nphi};
+    const int axes[1] = {0};
+    return c2c_exec(G, DMX_C128, st, G, st, 2, shape, axes, 1, forward,
+                    1.0 / (double)rows(), s);
+  }
+  // the points sorted for the NUFFT of this grid
+  std::unique_ptr<Plan> plan(const void *loc, bool type1, const void *vals,
+                             hipStream_t s) {
+    auto pl = std::make_unique<Plan>(p, s);
+    if (!pl->build_records(loc, false, M, type1, vals, true))
+      pl->build_index(loc, false, M, false);
+    return pl;
+  }
+};
+}  // namespace
+
+int dmx_sht_synthesis_general(const dmx_array *alm, dmx_array *map,
+                              size_t spin, size_t lmax,
+                              const int64_t *mstart, size_t nm,
+                              int64_t lstride, const dmx_array *loc,
+                              double epsilon, double sigma_min,
+                              double sigma_max, size_t nthreads, int mode,
+                              size_t verbosity, const dmx_options *opt_in) {
+  (void)nthreads;
+  return guard([&] {
+    DMX_ASSERT(alm && map && loc, "null argument");
+    DMX_ASSERT(mode == 0, "not yet supported: mode other than STANDARD");
+    reap(false);
+    dmx_options opt = opt_in ? *opt_in : defaults();
+    ShtJob job(*alm, *map, spin, lmax, mstart, nm, lstride, *loc, epsilon,
+               sigma_min, sigma_max, opt);
+    hipStream_t s = pick_stream(opt);
+    DevIn ain(*alm, s, false), lin(*loc, s, false);
+    DevOut mo(*map, s, true);
+    job.args.alm = ain.p;
+    Buf G(job.rows() * job.geo.nphi * 16, s);
+    launch_alm2leg(job.args, G.as<double2>(), s);
+    auto keep = job.theta_fft(G.p, true, s);
+    Buf z(job.M * 16, s);
+    if (job.M) {
+      auto pl = job.plan(lin.p, false, nullptr, s);
+      const int64_t ust[2] = {job.geo.nphi, 1};
+      pl->u2nu(false, G.p, true, ust, z.p, true);
+      launch_real_part(z.as<double2>(), job.M, mo.p, job.vf64, mo.st[1], s);
+      if (verbosity > 0) pl->report_params(false);
+      mo.finish(s);
+      pl->mark_done();
+    } else {
+      mo.finish(s);
+    }
+  });
+}
+
+int dmx_sht_adjoint_synthesis_general(dmx_array *alm, const dmx_array *map,
+                                      size_t spin, size_t lmax,
+                                      const int64_t *mstart, size_t nm,
+                                      int64_t lstride, const dmx_array *loc,
+                                      double epsilon, double sigma_min,
+                                      double sigma_max, size_t nthreads,
+                                      int mode, size_t verbosity,
+                                      const dmx_options *opt_in) {
+  (void)nthreads;
+  return guard([&] {
+    DMX_ASSERT(alm && map && loc, "null argument");
+    DMX_ASSERT(mode == 0, "not yet supported: mode other than STANDARD");
+    reap(false);
+    dmx_options opt = opt_in ? *opt_in : defaults();
+    ShtJob job(*alm, *map, spin, lmax, mstart, nm, lstride, *loc, epsilon,
+               sigma_min, sigma_max, opt);
+    hipStream_t s = pick_stream(opt);
+    DevIn min_(*map, s, true), lin(*loc, s, false);
+    DevOut ao(*alm, s, false);
+    // entries outside the (l, m) layout of a host array come back zero
+    if (!alm->device) launch_zero(ao.p, (size_t)nelem(*alm) * elsize(alm->typecode), s);
+    Buf Gt(job.rows() * job.geo.nphi * 16, s);
+    Buf z(std::max<size_t>(job.M, 1) * 16, s);
+    std::unique_ptr<Plan> pl;
+    if (job.M) {
+      launch_to_complex(min_.p, job.vf64, min_.st[1], job.M, z.as<double2>(),
+                        s);
+      pl = job.plan(lin.p, true, z.p, s);
+      const int64_t ust[2] = {job.geo.nphi, 1};
+      pl->nu2u(true, z.p, true, Gt.p, true, ust);
+      if (verbosity > 0) pl->report_params(true);
+    } else {
+      launch_zero(Gt.p, Gt.bytes, s);
+    }
+    auto keep = job.theta_fft(Gt.p, false, s);
+    Buf acc(job.nacc * 16, s);
+    if (leg2alm_needs_zero(job.args)) launch_zero(acc.p, acc.bytes, s);
+    launch_leg2alm(job.args, Gt.as<double2>(), acc.as<double2>(), s);
+    launch_store_alm(acc.as<double2>(), job.args, ao.p, job.vf64, s);
+    ao.finish(s);
+    if (pl) pl->mark_done();
+  });
+}
+
 int dmx_nudft(int type, const dmx_array *coord, const dmx_array *values,
               const int64_t *shape, size_t ndim, int forward,
               const double *periodicity, int fft_order, const int64_t *sel,
@@ -868,41 +1048,51 @@ int dmx_nudft(int type, const dmx_array *coord, const dmx_array *values,
   });
 }
 
+namespace {
+// parameters of a stand-alone sort with the given geometry (the sort never
+// evaluates the kernel; any kernel-table row of matching W serves)
+Params sort_params(const dmx_array *coord, const int64_t *nover, size_t ndim,
+                   int W, int log2tile, const double *periodicity) {
+  DMX_ASSERT(coord && nover && periodicity, "null argument");
+  check_coord(*coord);
+  DMX_ASSERT(ndim >= 1 && ndim <= 3 && coord->shape[1] == (int64_t)ndim,
+             "dimensionality mismatch");
+  DMX_ASSERT(W >= 1 && W <= 16 && log2tile >= 0 && log2tile <= 12,
+             "bad sort geometry");
+  Params p;
+  p.ndim = (int)ndim;
+  p.W = W;
+  p.log2tile = log2tile;
+  for (size_t d = 0; d < ndim; ++d) {
+    p.nover[d] = (size_t)nover[d];
+    p.nuni[d] = (size_t)nover[d] / 2;
+    p.period[d] = periodicity[d];
+    DMX_ASSERT(nover[d] >= 2 * ((W + 1) / 2), "oversampled length too small");
+  }
+  const auto &db = kernel_db();
+  for (size_t i = 0; i < db.size(); ++i)
+    if (db[i].W == W) {
+      p.beta = db[i].beta;
+      p.e0 = db[i].e0;
+      break;
+    }
+  DMX_ASSERT(p.beta > 0, "no kernel of this support");
+  return p;
+}
+}  // namespace
+
 int dmx_sort_points(const dmx_array *coord, const int64_t *nover,
                     size_t ndim, int W, int log2tile,
                     const double *periodicity, dmx_array *keys,
                     dmx_array *idx) {
   return guard([&] {
-    DMX_ASSERT(coord && nover && periodicity && keys && idx, "null argument");
-    check_coord(*coord);
-    DMX_ASSERT(ndim >= 1 && ndim <= 3 && coord->shape[1] == (int64_t)ndim,
-               "dimensionality mismatch");
+    DMX_ASSERT(keys && idx, "null argument");
+    Params p = sort_params(coord, nover, ndim, W, log2tile, periodicity);
     DMX_ASSERT(keys->typecode == DMX_U32 && idx->typecode == DMX_U32,
                "keys and idx must be uint32");
     const size_t M = (size_t)coord->shape[0];
     DMX_ASSERT(nelem(*keys) == (int64_t)M && nelem(*idx) == (int64_t)M,
                "number of points mismatch");
-    DMX_ASSERT(W >= 1 && W <= 16 && log2tile >= 0 && log2tile <= 12,
-               "bad sort geometry");
-    Params p;
-    p.ndim = (int)ndim;
-    p.W = W;
-    p.log2tile = log2tile;
-    for (size_t d = 0; d < ndim; ++d) {
-      p.nover[d] = (size_t)nover[d];
-      p.nuni[d] = (size_t)nover[d] / 2;
-      p.period[d] = periodicity[d];
-      DMX_ASSERT(nover[d] >= 2 * ((W + 1) / 2), "oversampled length too small");
-    }
-    // the sort never evaluates the kernel; any row of matching W serves
-    const auto &db = kernel_db();
-    for (size_t i = 0; i < db.size(); ++i)
-      if (db[i].W == W) {
-        p.beta = db[i].beta;
-        p.e0 = db[i].e0;
-        break;
-      }
-    DMX_ASSERT(p.beta > 0, "no kernel of this support");
     hipStream_t s = library_stream();
     Plan plan(p, s);
     DevIn cin(*coord, s, false);
@@ -915,6 +1105,49 @@ int dmx_sort_points(const dmx_array *coord, const int64_t *nover,
     }
     ko.finish(s);
     io.finish(s);
+  });
+}
+
+int dmx_sort_records(const dmx_array *coord, const int64_t *nover,
+                     size_t ndim, int W, int log2tile,
+                     const double *periodicity, dmx_array *words,
+                     dmx_array *idx, dmx_array *xarg) {
+  return guard([&] {
+    DMX_ASSERT(words && idx, "null argument");
+    Params p = sort_params(coord, nover, ndim, W, log2tile, periodicity);
+    p.fp64 = true;
+    p.use_lds = true;
+    DMX_ASSERT(words->typecode == DMX_U32 && idx->typecode == DMX_U32,
+               "words and idx must be uint32");
+    const size_t M = (size_t)coord->shape[0];
+    DMX_ASSERT(nelem(*words) == (int64_t)M && nelem(*idx) == (int64_t)M,
+               "number of points mismatch");
+    if (xarg)
+      DMX_ASSERT(xarg->typecode == DMX_F64 && xarg->ndim == 2 &&
+                     xarg->shape[0] == (int64_t)M &&
+                     xarg->shape[1] == (int64_t)ndim,
+                 "xarg must be a (M, ndim) float64 array");
+    hipStream_t s = library_stream();
+    Plan plan(p, s);
+    DevIn cin(*coord, s, false);
+    DevOut wo(*words, s, false), io(*idx, s, false);
+    std::unique_ptr<DevOut> xo;
+    if (xarg) xo = std::make_unique<DevOut>(*xarg, s, false);
+    if (M) {
+      DMX_ASSERT(plan.build_records(cin.p, coord->typecode == DMX_F32, M,
+                                    false, nullptr, false),
+                 "the record layout does not apply to this geometry");
+      const RecBufs &r = plan.records();
+      DMX_HIP(hipMemcpyAsync(wo.p, r.word, M * 4, hipMemcpyDeviceToDevice, s));
+      DMX_HIP(hipMemcpyAsync(io.p, r.j, M * 4, hipMemcpyDeviceToDevice, s));
+      if (xo)
+        for (size_t d = 0; d < ndim; ++d)
+          DMX_HIP(hipMemcpy2DAsync((char *)xo->p + 8 * d, 8 * ndim, r.x[d], 8,
+                                   8, M, hipMemcpyDeviceToDevice, s));
+    }
+    wo.finish(s);
+    io.finish(s);
+    if (xo) xo->finish(s);
   });
 }
 

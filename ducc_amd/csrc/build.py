@@ -18,7 +18,8 @@ GEN = os.path.join(HERE, "_gen")
 
 SOURCES = ["params.cpp", "runtime.hip", "api.hip", "sort.hip", "spread.hip",
            "interp.hip", "fft.hip", "grid.hip", "chooser.hip", "mlp.hip",
-           "c2c.hip", "noalloc.hip", "spread_det.hip"]
+           "c2c.hip", "noalloc.hip", "spread_det.hip", "records.hip",
+           "sht.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DMX_ARCH", "gfx950")
@@ -63,7 +64,7 @@ def compile_one(src, incremental=False):
     if incremental and os.path.exists(obj):
         t = os.path.getmtime(obj)
         if t > os.path.getmtime(path) and t > _newest_header():
-            return obj, 
+            return obj, ""
     if src.endswith(".hip"):
         cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE + ["-c", path, "-o", obj]
     else:

@@ -45,6 +45,8 @@ struct Sub {
 };
 
 constexpr int MAXSUB = 1024;
+
+inline size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
 // LDS one workgroup may allocate on gfx950 (160 KiB per CU, all of it
 // available to a single workgroup once the kernel opts in)
 constexpr size_t LDS_MAX_BYTES = 163840;
@@ -217,6 +219,13 @@ void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
                       bool hist0_ready = false, uint32_t *tb_start_c = nullptr,
                       uint32_t *tb_end = nullptr);
 size_t radix_sort_workspace(size_t M);
+// offsets of one radix pass: the per-block histogram of the digit
+// (keys >> shift) & 255 (unless hist_ready), its digit-major per-chunk scans
+// (offs) and the scan of the chunk sums (bsum); a block's global start of
+// digit d is offs[d*nblocks+b] + bsum[(d*nblocks+b) / 2048]
+void radix_pass_offsets(const uint32_t *keys, size_t M, int shift,
+                        uint32_t *hist, bool hist_ready, uint32_t *offs,
+                        uint32_t *bsum, hipStream_t s);
 uint32_t *radix_sort_hist0(void *work, size_t M);
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n,
                         uint32_t *total, void *work, hipStream_t s);
@@ -232,6 +241,27 @@ void launch_gather_coords(const void *coords, int coord_f32, int ndim,
                           const uint32_t *idx, size_t M, void *out,
                           hipStream_t s);
 void launch_zero(void *p, size_t bytes, hipStream_t s);
+
+// point records in tile order (records.hip): structure of arrays
+struct RecBufs {
+  uint32_t *word;  // tile << offbits | tile-local offsets (log2tile bits
+                   // per axis, axis d at bit d*log2tile)
+  void *x[3];      // kernel argument W-1-2*frac per axis, compute type
+  void *v;         // type 1: value, complex compute type
+  uint32_t *j;     // type 2: input index of the point
+};
+size_t rec_bytes_per_point(int ndim, bool fp64, bool type1);
+size_t rec_carve_bytes(size_t M, int ndim, bool fp64, bool type1);
+RecBufs rec_carve(void *p, size_t M, int ndim, bool fp64, bool type1);
+size_t rec_sort_workspace(size_t M);
+// stable sort of the points by tile, carrying their records: `out` receives
+// them in tile order (the order of radix_sort_pairs' permutation), `tmp` is
+// the ping-pong buffer; tb_start_c / tb_end as for radix_sort_pairs
+void rec_sort(const Geo &g, int offbits, int keybits, bool fp64, bool type1,
+              const void *coords, const void *values, int val_f64, size_t M,
+              const RecBufs &out, const RecBufs &tmp, void *work,
+              size_t work_bytes, uint32_t *tb_start_c, uint32_t *tb_end,
+              hipStream_t s);
 
 // spreading / interpolation (spread.hip, interp.hip)
 struct SpreadArgs {
@@ -253,6 +283,11 @@ struct SpreadArgs {
   // subproblem and subproblem count (index build tables)
   void *slab;
   const uint32_t *tile_sub0, *tile_nsub;
+  // 1: the points are records in tile order (rec_sort): spreading reads
+  // rec.word / rec.x / rec.v, interpolation rec.word / rec.x / rec.j, at
+  // the sorted position, instead of idx / coords / values
+  int rec;
+  RecBufs recs;
 };
 void launch_spread(bool fp64, const SpreadArgs &a, hipStream_t s);
 void launch_interp(bool fp64, const SpreadArgs &a, void *out, int out_f64,

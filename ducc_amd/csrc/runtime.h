@@ -36,7 +36,6 @@ class ArenaScope {
   Arena *prev_;
 };
 Arena *current_arena();
-inline size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 // Caching allocator per device: freed blocks are kept and reused for
 // requests of at most their size (and at least half of it).  A block
@@ -161,6 +160,17 @@ class Plan {
   // several times: one random gather instead of one per execution).
   void build_index(const void *coords, bool coord_f32, size_t M,
                    bool keep_sorted_coords);
+  // For a single transform: sorts the points' records (rec_sort) instead of
+  // an index, so that spreading / interpolation read them in order.
+  // values: type 1, the device values (M complex, c64 / c128), alive until
+  // the index build has run; nullptr for type 2.  Returns false, with
+  // nothing queued, where the record layout does not apply (no LDS region,
+  // deterministic mode, tile key and offsets wider than 32 bits): then
+  // build_index is the route.
+  bool build_records(const void *coords, bool coord_f32, size_t M,
+                     bool type1, const void *values, bool values_f64);
+  bool uses_records() const { return rec_; }
+  const RecBufs &records() const { return recs_; }
   // The grid-side part of a transform that does not depend on the points
   // (type 1: the zeroed grid; type 2: pad and FFT), enqueued ahead of the
   // rest, e.g. on another stream while build_index runs.  The next nu2u /
@@ -266,6 +276,11 @@ class Plan {
   static std::shared_ptr<Setup> arena_setup(const Params &p, hipStream_t s);
   std::map<int, const void *> arena_tw_;  // twiddles built in the arena
   Buf idx_, subs_, nsub_total_, sorted_coords_;
+  bool rec_ = false, rec_type1_ = false;
+  RecBufs recs_{};
+  Buf rec_buf_, rec_tmp_;
+  // the tile tables and subproblem list from the sort's tile ranges
+  void finish_index(uint32_t *tables);
   Buf tmp_keys_, tmp_tiles_, tmp_work_;
   Buf grid_, scratch_, slab_;
   const uint32_t *tile_nsub_ = nullptr, *tile_sub0_ = nullptr;

@@ -624,8 +624,7 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
   tmp_tiles_.reset(ntiles_ * 4 * 5);
   tmp_work_.reset(ws);
   uint32_t *counts = tmp_tiles_.as<uint32_t>(), *tstart = counts + ntiles_,
-           *tend = tstart + ntiles_, *nsub = tend + ntiles_,
-           *sstart = nsub + ntiles_;
+           *tend = tstart + ntiles_;
   {
     Phase ph(this, "building index");
     launch_zero(tstart, ntiles_ * 8, s_);  // tstart and tend, adjacent
@@ -634,14 +633,7 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
     idx_.reset(M * 4);
     radix_sort_pairs(tmp_keys_.as<uint32_t>(), M, keybits, idx_.as<uint32_t>(),
                      nullptr, tmp_work_.p, ws, s_, true, tstart, tend);
-    nsub_max_ = std::min(ntiles_, M) + (M + MAXSUB - 1) / MAXSUB;
-    subs_.reset(nsub_max_ * sizeof(Sub));
-    nsub_total_.reset(4);
-    launch_tile_subs(ntiles_, counts, tstart, tend, nsub, sstart,
-                     subs_.as<Sub>(), nsub_total_.as<uint32_t>(), tmp_work_.p,
-                     s_);
-    tile_nsub_ = nsub;
-    tile_sub0_ = sstart;
+    finish_index(counts);
   }
   if (keep_sorted_coords) {
     // a random gather of 1e7 points costs about as much as the gathers it
@@ -654,6 +646,62 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
     coords_sorted_ = true;
     g_.coord_v2 = (p_.ndim == 2 && !coord_f32) ? 1 : 0;  // pool: aligned
   }
+}
+
+void Plan::finish_index(uint32_t *counts) {
+  uint32_t *tstart = counts + ntiles_, *tend = tstart + ntiles_,
+           *nsub = tend + ntiles_, *sstart = nsub + ntiles_;
+  nsub_max_ = std::min(ntiles_, M_) + (M_ + MAXSUB - 1) / MAXSUB;
+  subs_.reset(nsub_max_ * sizeof(Sub));
+  nsub_total_.reset(4);
+  launch_tile_subs(ntiles_, counts, tstart, tend, nsub, sstart,
+                   subs_.as<Sub>(), nsub_total_.as<uint32_t>(), tmp_work_.p,
+                   s_);
+  tile_nsub_ = nsub;
+  tile_sub0_ = sstart;
+}
+
+bool Plan::build_records(const void *coords, bool coord_f32, size_t M,
+                         bool type1, const void *values, bool values_f64) {
+  if (!p_.use_lds || p_.deterministic || M == 0) return false;
+  int keybits = 1;
+  while (((size_t)1 << keybits) < ntiles_) ++keybits;
+  const int offbits = p_.ndim * p_.log2tile;
+  if (keybits + offbits > 32 || p_.log2tile > 10) return false;
+  tag_ = "index";
+  M_ = M;
+  coord_f32_ = coord_f32;
+  g_.coord_f32 = coord_f32 ? 1 : 0;
+  coords_ = nullptr;  // the records replace the coordinates
+  user_coords_ = coords;
+  coords_sorted_ = false;
+  g_.coord_v2 = (p_.ndim == 2 && !coord_f32 &&
+                 reinterpret_cast<uintptr_t>(coords) % 16 == 0)
+                    ? 1
+                    : 0;
+  // the record arrays and temporaries live as long as the plan, so nothing
+  // here waits for the stream
+  const size_t rb = rec_carve_bytes(M, p_.ndim, p_.fp64, type1);
+  rec_buf_.reset(rb);
+  rec_tmp_.reset(rb);
+  recs_ = rec_carve(rec_buf_.p, M, p_.ndim, p_.fp64, type1);
+  const RecBufs tmp = rec_carve(rec_tmp_.p, M, p_.ndim, p_.fp64, type1);
+  const size_t ws = std::max(rec_sort_workspace(M), scan_workspace(ntiles_));
+  tmp_tiles_.reset(ntiles_ * 4 * 5);
+  tmp_work_.reset(ws);
+  uint32_t *counts = tmp_tiles_.as<uint32_t>(), *tstart = counts + ntiles_,
+           *tend = tstart + ntiles_;
+  {
+    Phase ph(this, "building index");
+    launch_zero(tstart, ntiles_ * 8, s_);  // tstart and tend, adjacent
+    rec_sort(g_, offbits, keybits, p_.fp64, type1, coords, values,
+             values_f64 ? 1 : 0, M, recs_, tmp, tmp_work_.p, ws, tstart, tend,
+             s_);
+    finish_index(counts);
+  }
+  rec_ = true;
+  rec_type1_ = type1;
+  return true;
 }
 
 uint32_t *Plan::keys_copy_to(uint32_t *dev_keys_out) {
@@ -679,6 +727,8 @@ void Plan::fill_spread_args(SpreadArgs &a) const {
   a.coords_sorted = coords_sorted_ ? 1 : 0;
   a.tile_nsub = tile_nsub_;
   a.tile_sub0 = tile_sub0_;
+  a.rec = rec_ ? 1 : 0;
+  a.recs = recs_;
 }
 
 void Plan::prepare_grid_side() {
@@ -848,6 +898,7 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
   if (!grid_ready_) nu2u_grid();
   grid_ready_ = false;
   tag_ = "nu2u";
+  DMX_ASSERT(!rec_ || rec_type1_, "internal: type-2 records for type 1");
   if (M_ > 0) {
     Phase ph(this, "spreading");
     SpreadArgs a{};
@@ -885,6 +936,7 @@ void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
   if (!grid_ready_) u2nu_grid(forward, uni, uni_f64, ust);
   grid_ready_ = false;
   tag_ = "u2nu";
+  DMX_ASSERT(!rec_ || !rec_type1_, "internal: type-1 records for type 2");
   if (M_ > 0) {
     SpreadArgs a{};
     fill_spread_args(a);
@@ -899,6 +951,7 @@ void Plan::u2nu_grad(bool forward, const void *uni, bool uni_f64,
   if (!grid_ready_) u2nu_grid(forward, uni, uni_f64, ust, true);
   grid_ready_ = false;
   tag_ = "u2nu_grad";
+  DMX_ASSERT(!rec_, "internal: coordinate gradient needs the point index");
   if (M_ > 0) {
     SpreadArgs a{};
     fill_spread_args(a);

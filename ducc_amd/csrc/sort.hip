@@ -11,15 +11,12 @@
 // wavefront with ballots (stable) and across the block's four wavefronts in
 // wave order, so every pass is stable; the ranked block is staged in LDS and
 // written out in digit runs.
-#include "dmx.h"
+#include "radix.h"
 
 namespace dmx {
 
 namespace {
-constexpr int RB = 256;         // buckets per pass
-constexpr int RT = 256;         // threads per block
-constexpr int RITEMS = 16;      // keys per thread
-constexpr int RTILE = RT * RITEMS;
+using namespace radix;
 
 // Tile key per point (nufft.h:1364-1381 role); one 4096-point block per
 // workgroup, so that the first sort pass's per-block digit histogram comes
@@ -57,7 +54,7 @@ __global__ __launch_bounds__(RT) void k_fold_keys(Geo g, const void *coords,
       uint32_t key = t[0];
 #pragma unroll
       for (int d = 1; d < NDIM; ++d) key = key * (uint32_t)g.ntiles[d] + t[d];
-      keys[j] = key;
+      if (keys) keys[j] = key;
       if (hist0) atomicAdd(&h[key & 0xFFu], 1u);
     }
   }
@@ -91,44 +88,6 @@ __global__ __launch_bounds__(RT) void k_radix_hist(const uint32_t *keys,
   hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-__device__ __forceinline__ unsigned lane_id() {
-  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-}
-
-// lanes of the wave holding the same 8-bit digit (ballot per bit)
-__device__ __forceinline__ unsigned long long digit_peers(uint32_t d,
-                                                          bool valid) {
-  unsigned long long peers = __ballot(valid);
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const int bit = (d >> b) & 1;
-    const unsigned long long bal = __ballot(bit);
-    peers &= bit ? bal : ~bal;
-  }
-  return valid ? peers : 0ull;
-}
-
-// ------------------------------------------------------------------ scan
-constexpr int ST = 256, SITEMS = 8, STILE = ST * SITEMS;
-
-__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh,
-                                         uint32_t &total) {
-  // Hillis-Steele over 256 values in LDS
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int o = 1; o < ST; o <<= 1) {
-    uint32_t x = t >= o ? sh[t - o] : 0u;
-    __syncthreads();
-    sh[t] += x;
-    __syncthreads();
-  }
-  uint32_t incl = sh[t];
-  total = sh[ST - 1];
-  __syncthreads();
-  return incl - v;
-}
-
 // Stable scatter of one 4096-key block, staged through LDS so that the
 // global writes go out in digit runs (consecutive lanes -> consecutive
 // addresses):
@@ -145,22 +104,19 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
     const uint32_t *offs, const uint32_t *offs_chunk, int nblocks,
     uint32_t *keys_out, uint32_t *idx_out, uint32_t *tb_start_c,
     uint32_t *tb_end) {
-  constexpr int NW = RT / 64;
-  __shared__ uint32_t cnt[NW][RB];
-  __shared__ uint32_t gbase[RB];
-  __shared__ uint32_t scan_sh[RB];
+  __shared__ RankShared rs;
   __shared__ uint32_t stage[RTILE];
   const int tid = threadIdx.x, wave = tid >> 6;
   const unsigned lane = lane_id();
-#pragma unroll
-  for (int w = 0; w < NW; ++w) cnt[w][tid] = 0;
   const size_t blk0 = (size_t)blockIdx.x * RTILE;
   const size_t c0 = blk0 + (size_t)wave * (RTILE / NW);
-  uint32_t key[RITEMS], rank[RITEMS], ival[RITEMS];
+  uint32_t key[RITEMS], ival[RITEMS], dig[RITEMS];
+  bool valid[RITEMS];
 #pragma unroll
   for (int r = 0; r < RITEMS; ++r) {
     const size_t j = c0 + (size_t)r * 64 + lane;
-    key[r] = j < M ? keys_in[j] : 0xFFFFFFFFu;
+    valid[r] = j < M;
+    key[r] = valid[r] ? keys_in[j] : 0xFFFFFFFFu;
   }
   // this block's global base of digit tid (chunk scan + scanned chunk
   // sums), and the indices that travel with the keys: loaded now, so that
@@ -170,51 +126,16 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
 #pragma unroll
   for (int r = 0; r < RITEMS; ++r) {
     const size_t j = c0 + (size_t)r * 64 + lane;
-    ival[r] = idx_in ? (j < M ? idx_in[j] : 0u) : (uint32_t)j;
+    ival[r] = idx_in ? (valid[r] ? idx_in[j] : 0u) : (uint32_t)j;
+    dig[r] = (key[r] >> shift) & 0xFFu;
   }
-  __syncthreads();
-  const unsigned long long below_mask =
-      lane ? (~0ull >> (64 - lane)) : 0ull;
-#pragma unroll
-  for (int r = 0; r < RITEMS; ++r) {
-    const size_t j = c0 + (size_t)r * 64 + lane;
-    const bool valid = j < M;
-    const uint32_t d = (key[r] >> shift) & 0xFFu;
-    const unsigned long long peers = digit_peers(d, valid);
-    const unsigned long long below = peers & below_mask;
-    const uint32_t old = cnt[wave][d];
-    // every lane has read the counter before the group leader advances it
-    __builtin_amdgcn_wave_barrier();
-    if (valid && below == 0) cnt[wave][d] = old + (uint32_t)__popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-    rank[r] = old + (uint32_t)__popcll(below);
-  }
-  __syncthreads();
-  {
-    // thread tid owns digit tid: wave prefix, block-local start, global base
-    uint32_t run = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      const uint32_t c = cnt[w][tid];
-      cnt[w][tid] = run;
-      run += c;
-    }
-    uint32_t tot;
-    const uint32_t start = block_exclusive_scan(run, scan_sh, tot);
-#pragma unroll
-    for (int w = 0; w < NW; ++w) cnt[w][tid] += start;
-    gbase[tid] = gofs - start;
-  }
-  __syncthreads();
-  const int nvalid = (int)min((size_t)RTILE, M - blk0);
   uint32_t lpos[RITEMS];
+  rank_block(dig, valid, lpos, rs, gofs);
+  const uint32_t *gbase = rs.gbase;
+  const int nvalid = (int)min((size_t)RTILE, M - blk0);
 #pragma unroll
-  for (int r = 0; r < RITEMS; ++r) {
-    const size_t j = c0 + (size_t)r * 64 + lane;
-    const uint32_t d = (key[r] >> shift) & 0xFFu;
-    lpos[r] = cnt[wave][d] + rank[r];
-    if (j < M) stage[lpos[r]] = key[r];
-  }
+  for (int r = 0; r < RITEMS; ++r)
+    if (valid[r]) stage[lpos[r]] = key[r];
   __syncthreads();
   uint32_t gpos[RITEMS];
 #pragma unroll
@@ -338,6 +259,57 @@ __global__ void k_fill_subs(const uint32_t *counts, const uint32_t *tstart,
   }
 }
 
+// The three steps above in one workgroup, for tile tables of at most 4096
+// tiles (one launch instead of four; longer tables take the launches above,
+// which measured faster from about 10^4 tiles on): per 4096-tile chunk, start / count / subproblem
+// count of each tile, a block prefix of the subproblem counts (with the
+// carry of the chunks before) and the subproblems themselves
+constexpr int TS_ITEMS = 16;
+__global__ __launch_bounds__(ST) void k_tile_subs_one(
+    uint32_t *tstart, const uint32_t *tend, size_t ntiles, uint32_t *counts,
+    uint32_t *nsub, uint32_t *sstart, Sub *subs, uint32_t *nsub_total) {
+  __shared__ uint32_t sh[ST];
+  uint32_t carry = 0;
+  for (size_t c = 0; c < ntiles; c += (size_t)ST * TS_ITEMS) {
+    const size_t base = c + (size_t)threadIdx.x * TS_ITEMS;
+    uint32_t st[TS_ITEMS], cn[TS_ITEMS], ns[TS_ITEMS], sum = 0;
+#pragma unroll
+    for (int k = 0; k < TS_ITEMS; ++k) {
+      const size_t t = base + k;
+      st[k] = cn[k] = ns[k] = 0;
+      if (t < ntiles) {
+        const uint32_t e = tend[t];
+        st[k] = e ? ~tstart[t] : 0u;
+        cn[k] = e - st[k];
+        ns[k] = (cn[k] + MAXSUB - 1) / MAXSUB;
+        tstart[t] = st[k];
+        counts[t] = cn[k];
+        nsub[t] = ns[k];
+        sum += ns[k];
+      }
+    }
+    uint32_t tot;
+    uint32_t ex = block_exclusive_scan(sum, sh, tot) + carry;
+#pragma unroll
+    for (int k = 0; k < TS_ITEMS; ++k) {
+      const size_t t = base + k;
+      if (t >= ntiles) continue;
+      sstart[t] = ex;
+      for (uint32_t q = 0; q < ns[k]; ++q) {
+        Sub sb;
+        sb.tile = (uint32_t)t;
+        sb.start = st[k] + q * MAXSUB;
+        sb.end = min(st[k] + cn[k], sb.start + (uint32_t)MAXSUB);
+        sb.pad_ = 0;
+        subs[ex + q] = sb;
+      }
+      ex += ns[k];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *nsub_total = carry;
+}
+
 template <typename T>
 __global__ void k_gather_coords(const T *c, int ndim, const uint32_t *idx,
                                 size_t M, T *out) {
@@ -435,17 +407,9 @@ void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
     bool last = p == npass - 1;
     uint32_t *kout = last ? keys_out : kbuf[p & 1];
     uint32_t *iout = last ? idx_out : ibuf[p & 1];
-    if (p > 0 || !hist0_ready)
-      hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nblocks), dim3(RT), 0,
-                         s, kin, M, 8 * p, hist, (int)nblocks);
-    // digit-major offsets: per-chunk scans and the scan of the chunk sums;
-    // the scatter adds the two
-    const size_t nch = (hist_n + STILE - 1) / STILE;
     uint32_t *bsum = (uint32_t *)sw;
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nch), dim3(ST), 0, s,
-                       (const uint32_t *)hist, offs, hist_n, bsum);
-    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(ST), 0, s,
-                       (const uint32_t *)bsum, bsum, nch, nullptr);
+    radix_pass_offsets(kin, M, 8 * p, hist, p == 0 && hist0_ready, offs,
+                       bsum, s);
     hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nblocks), dim3(RT), 0,
                        s, kin, iin, M, 8 * p, (const uint32_t *)offs,
                        (const uint32_t *)bsum,
@@ -457,10 +421,34 @@ void radix_sort_pairs(const uint32_t *keys_in, size_t M, int keybits,
   }
 }
 
+void radix_pass_offsets(const uint32_t *keys, size_t M, int shift,
+                        uint32_t *hist, bool hist_ready, uint32_t *offs,
+                        uint32_t *bsum, hipStream_t s) {
+  const size_t nblocks = (M + RTILE - 1) / RTILE;
+  const size_t hist_n = (size_t)RB * nblocks;
+  if (!hist_ready)
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nblocks), dim3(RT), 0, s,
+                       keys, M, shift, hist, (int)nblocks);
+  // digit-major offsets: per-chunk scans and the scan of the chunk sums;
+  // the scatter adds the two
+  const size_t nch = (hist_n + STILE - 1) / STILE;
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nch), dim3(ST), 0, s,
+                     (const uint32_t *)hist, offs, hist_n, bsum);
+  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(ST), 0, s,
+                     (const uint32_t *)bsum, bsum, nch, nullptr);
+  DMX_HIP(hipGetLastError());
+}
+
 void launch_tile_subs(size_t ntiles, uint32_t *counts, uint32_t *tstart,
                       const uint32_t *tend, uint32_t *nsub,
                       uint32_t *sub_start, Sub *subs, uint32_t *nsub_total,
                       void *scanwork, hipStream_t s) {
+  if (ntiles <= (size_t)ST * TS_ITEMS) {
+    hipLaunchKernelGGL(k_tile_subs_one, dim3(1), dim3(ST), 0, s, tstart, tend,
+                       ntiles, counts, nsub, sub_start, subs, nsub_total);
+    DMX_HIP(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_counts_nsub, dim3(nblk(ntiles, 256)), dim3(256), 0, s,
                      tstart, tend, ntiles, counts, nsub);
   exclusive_scan_u32(nsub, sub_start, ntiles, nsub_total, scanwork, s);

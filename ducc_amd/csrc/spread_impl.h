@@ -165,49 +165,83 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
         // the previous chunk's phase 2 is done with the stash (and the
         // accumulators are zero before the first chunk)
         __syncthreads();
-        // loads of all this thread's points first (memory-level parallelism)
-        uint32_t jj[PT];
-        double cc[PT][NDIM];
-        T vr_[PT], vi_[PT];
-#pragma unroll
-        for (int k = 0; k < PT; ++k) {
-          const int i = tid + 256 * k;
-          jj[k] = i < nc ? a.idx[c0 + i] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < PT; ++k) {
-          const int i = tid + 256 * k;
-          if (i < nc) {
-            const uint32_t p = c0 + i;
-            const size_t cj = a.coords_sorted ? (size_t)p : (size_t)jj[k];
-            load_point(a.g, a.coords, cj, NDIM, cc[k]);
-            load_value<T>(a.values,
-                          a.values_sorted ? (size_t)p : (size_t)jj[k],
-                          a.val_f64, vr_[k], vi_[k]);
-          }
-        }
         T xs[PT][NDIM];
         uint32_t os[PT], rnk[PT];
         int cls[PT];
+        T vr_[PT], vi_[PT];
+        if (a.rec) {
+          // records in tile order: offsets, kernel arguments and values
+          // read in sequence
+          const int L = a.g.log2tile;
+          const uint32_t tm = (1u << L) - 1u;
+#pragma unroll
+          for (int k = 0; k < PT; ++k) {
+            const int i = tid + 256 * k;
+            os[k] = 0;
+            if (i < nc) {
+              const uint32_t p = c0 + i;
+              const uint32_t w = a.recs.word[p];
+              uint32_t o = 0;
+#pragma unroll
+              for (int d = 0; d < NDIM; ++d) {
+                o |= ((w >> (d * L)) & tm) << (10 * d);
+                xs[k][d] = ((const T *)a.recs.x[d])[p];
+              }
+              os[k] = o;
+              const typename cplx2<T>::type v =
+                  ((const typename cplx2<T>::type *)a.recs.v)[p];
+              vr_[k] = v.x;
+              vi_[k] = v.y;
+            }
+          }
+        } else {
+          // loads of all this thread's points first (memory-level
+          // parallelism)
+          uint32_t jj[PT];
+          double cc[PT][NDIM];
+#pragma unroll
+          for (int k = 0; k < PT; ++k) {
+            const int i = tid + 256 * k;
+            jj[k] = i < nc ? a.idx[c0 + i] : 0u;
+          }
+#pragma unroll
+          for (int k = 0; k < PT; ++k) {
+            const int i = tid + 256 * k;
+            if (i < nc) {
+              const uint32_t p = c0 + i;
+              const size_t cj = a.coords_sorted ? (size_t)p : (size_t)jj[k];
+              load_point(a.g, a.coords, cj, NDIM, cc[k]);
+              load_value<T>(a.values,
+                            a.values_sorted ? (size_t)p : (size_t)jj[k],
+                            a.val_f64, vr_[k], vi_[k]);
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < PT; ++k) {
+            const int i = tid + 256 * k;
+            os[k] = 0;
+            if (i < nc) {
+              uint32_t o = 0;
+#pragma unroll
+              for (int d = 0; d < NDIM; ++d) {
+                double frac;
+                const int i0 = fold_axis(a.g, d, cc[k][d], frac);
+                xs[k][d] = (T)((double)(W - 1) - 2.0 * frac);
+                int od = i0 - b0[d];
+                od = od < 0 ? 0 : (od >= tile ? tile - 1 : od);
+                o |= (uint32_t)od << (10 * d);
+              }
+              os[k] = o;
+            }
+          }
+        }
 #pragma unroll
         for (int k = 0; k < PT; ++k) {
           const int i = tid + 256 * k;
-          os[k] = 0;
           cls[k] = 0;
           rnk[k] = 0;
           if (i < nc) {
-            uint32_t o = 0;
-#pragma unroll
-            for (int d = 0; d < NDIM; ++d) {
-              double frac;
-              const int i0 = fold_axis(a.g, d, cc[k][d], frac);
-              xs[k][d] = (T)((double)(W - 1) - 2.0 * frac);
-              int od = i0 - b0[d];
-              od = od < 0 ? 0 : (od >= tile ? tile - 1 : od);
-              o |= (uint32_t)od << (10 * d);
-            }
-            os[k] = o;
-            cls[k] = region_cell<T, NDIM>(o, S) & 15;
+            cls[k] = region_cell<T, NDIM>(os[k], S) & 15;
             rnk[k] = atomicAdd(&st.cnt[cls[k]], 1u);
           }
         }
@@ -427,23 +461,41 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
     // one point per lane and round (tpp == 1): the indices and coordinates
     // of all of a lane's rounds (at most MAXSUB / 256) are loaded up front,
     // so that their latencies overlap
+    // (records: index, word and kernel arguments; the arguments are kept
+    // as doubles, exact for either compute type)
     constexpr int PF = tpp == 1 ? MAXSUB / 256 : 1;
-    uint32_t jpf[PF];
+    uint32_t jpf[PF], wpf[PF];
     double xpf[PF][NDIM];
     if constexpr (tpp == 1) {
+      if (a.rec) {
 #pragma unroll
-      for (int k = 0; k < PF; ++k) {
-        const uint32_t q = lp + (uint32_t)k * pstep;
-        jpf[k] = q < nc ? a.idx[sb.start + q] : 0u;
-      }
+        for (int k = 0; k < PF; ++k) {
+          const uint32_t q = lp + (uint32_t)k * pstep;
+          jpf[k] = wpf[k] = 0u;
+          if (q < nc) {
+            const uint32_t p = sb.start + q;
+            jpf[k] = a.recs.j[p];
+            wpf[k] = a.recs.word[p];
 #pragma unroll
-      for (int k = 0; k < PF; ++k) {
-        const uint32_t q = lp + (uint32_t)k * pstep;
-        if (q < nc)
-          load_point(a.g, a.coords,
-                     a.coords_sorted ? (size_t)(sb.start + q)
-                                     : (size_t)jpf[k],
-                     NDIM, xpf[k]);
+            for (int d = 0; d < NDIM; ++d)
+              xpf[k][d] = (double)((const T *)a.recs.x[d])[p];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+          const uint32_t q = lp + (uint32_t)k * pstep;
+          jpf[k] = q < nc ? a.idx[sb.start + q] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+          const uint32_t q = lp + (uint32_t)k * pstep;
+          if (q < nc)
+            load_point(a.g, a.coords,
+                       a.coords_sorted ? (size_t)(sb.start + q)
+                                       : (size_t)jpf[k],
+                       NDIM, xpf[k]);
+        }
       }
     }
     for (uint32_t pb = 0; pb < nc; pb += pstep) {
@@ -458,6 +510,7 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
       T w[NDIM][W];
       T dw[MODE ? NDIM : 1][W];
       double x_[NDIM];
+      uint32_t wd = 0;
       if constexpr (tpp == 1) {
         // round pb / pstep (< PF: nc <= MAXSUB, pstep = 256), selected
         // with constant indices so the arrays stay in registers
@@ -466,29 +519,48 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
         for (int k = 0; k < PF; ++k)
           if (k == kk) {
             j = jpf[k];
+            wd = wpf[k];
 #pragma unroll
             for (int d = 0; d < NDIM; ++d) x_[d] = xpf[k][d];
           }
+      } else if (a.rec) {
+        j = a.recs.j[p];
+        wd = a.recs.word[p];
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d)
+          x_[d] = (double)((const T *)a.recs.x[d])[p];
       } else {
         j = a.idx[p];
         load_point(a.g, a.coords, a.coords_sorted ? (size_t)p : (size_t)j,
                    NDIM, x_);
       }
-#pragma unroll
-      for (int d = 0; d < NDIM; ++d) {
-        double frac;
-        i0[d] = fold_axis(a.g, d, x_[d], frac);
-        T x = (T)((double)(W - 1) - 2.0 * frac);
-        if constexpr (MODE == 0)
-          kernel_weights<T, W>(x, cf, w[d]);
-        else
-          kernel_weights_deriv<T, W>(x, cf, w[d], dw[d]);
-      }
       int o[3];
+      T xk[NDIM];
+      if (a.rec) {
+        // the record's tile-local offsets and kernel arguments
+        const int L = a.g.log2tile;
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+          o[d] = (int)((wd >> (d * L)) & ((1u << L) - 1u));
+          i0[d] = b0[d] + o[d];
+          xk[d] = (T)x_[d];
+        }
+      } else {
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+          double frac;
+          i0[d] = fold_axis(a.g, d, x_[d], frac);
+          xk[d] = (T)((double)(W - 1) - 2.0 * frac);
+          o[d] = i0[d] - b0[d];
+          o[d] = o[d] < 0 ? 0 : (o[d] >= tile ? tile - 1 : o[d]);
+        }
+      }
 #pragma unroll
       for (int d = 0; d < NDIM; ++d) {
-        o[d] = i0[d] - b0[d];
-        o[d] = o[d] < 0 ? 0 : (o[d] >= tile ? tile - 1 : o[d]);
+        if constexpr (MODE == 0)
+          kernel_weights<T, W>(xk[d], cf, w[d]);
+        else
+          kernel_weights_deriv<T, W>(xk[d], cf, w[d], dw[d]);
       }
       // cell value accessor
       auto cell = [&](int c0, int c1, int c2, T &re, T &im) {

@@ -72,18 +72,19 @@ def _check_dtypes(vals, coord):
 
 
 def _opts(precision, chooser, log2tile, stream, kernel_idx=-1, sigma=0.0,
-          spot_check=True, deterministic=False):
+          spot_check=True, deterministic=False, point_records=False):
     return _lib.options(precision=precision, chooser=chooser,
                         log2tile=log2tile, stream=stream,
                         kernel_idx=kernel_idx, sigma=sigma,
-                        spot_check=spot_check, deterministic=deterministic)
+                        spot_check=spot_check, deterministic=deterministic,
+                        point_records=point_records)
 
 
 def u2nu(*, grid, coord, forward, epsilon, nthreads=1, out=None,
          verbosity=0, sigma_min=1.2, sigma_max=2.51,
          periodicity=_TWO_PI, fft_order=False, precision="auto",
          chooser="analytic", log2tile=0, info=None, kernel_idx=-1,
-         spot_check=True, deterministic=False):
+         spot_check=True, deterministic=False, point_records=False):
     """Type 2 NUFFT: uniform grid -> values at nonuniform points.
 
     out_j = sum_k grid_k exp(isign i 2pi/P k.x_j), isign = -1 if forward,
@@ -105,7 +106,7 @@ def u2nu(*, grid, coord, forward, epsilon, nthreads=1, out=None,
     o, ko = _lib.describe(out)
     opt = _opts(precision, chooser, log2tile, _stream_for(grid, coord, out),
                 kernel_idx=kernel_idx, spot_check=spot_check,
-                deterministic=deterministic)
+                deterministic=deterministic, point_records=point_records)
     inf = _lib.DmxPlanInfo()
     _lib.check(_lib.lib().dmx_u2nu(
         ct.byref(g), ct.byref(c), int(bool(forward)), float(epsilon),
@@ -123,7 +124,7 @@ def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
          verbosity=0, sigma_min=1.2, sigma_max=2.51,
          periodicity=_TWO_PI, fft_order=False, precision="auto",
          chooser="analytic", log2tile=0, info=None, kernel_idx=-1,
-         spot_check=True, deterministic=False):
+         spot_check=True, deterministic=False, point_records=False):
     """Type 1 NUFFT: values at nonuniform points -> uniform grid.
 
     out_k = sum_j points_j exp(isign i 2pi/P k.x_j).  `out` is required: its
@@ -140,7 +141,7 @@ def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
     o, ko = _lib.describe(out)
     opt = _opts(precision, chooser, log2tile, _stream_for(points, coord, out),
                 kernel_idx=kernel_idx, spot_check=spot_check,
-                deterministic=deterministic)
+                deterministic=deterministic, point_records=point_records)
     inf = _lib.DmxPlanInfo()
     _lib.check(_lib.lib().dmx_nu2u(
         ct.byref(p), ct.byref(c), int(bool(forward)), float(epsilon),
@@ -322,6 +323,30 @@ def nudft_direct(type_, coord, values, shape, forward, sel,
                                     int(bool(fft_order)), sel.ctypes.data,
                                     len(sel), ct.byref(o)))
     return out
+
+
+def sort_records(coord, nover, W, log2tile, periodicity=_TWO_PI):
+    """The record sort of unplanned transforms on the GPU: per sorted
+    position the record word (tile << ndim*log2tile | tile-local offsets),
+    the point's input index and its kernel arguments W-1-2*frac (float64),
+    for bit-exact parity checks."""
+    _lib.require_gpu()
+    coord = np.ascontiguousarray(coord)
+    M, nd = coord.shape
+    words = np.zeros(M, np.uint32)
+    idx = np.zeros(M, np.uint32)
+    xarg = np.zeros((M, nd), np.float64)
+    per = _periodicity(periodicity, nd)
+    c, _ = _lib.describe(coord)
+    w, _ = _lib.describe(words)
+    i, _ = _lib.describe(idx)
+    x, _ = _lib.describe(xarg)
+    nov = np.asarray(nover, np.int64)
+    _lib.check(_lib.lib().dmx_sort_records(ct.byref(c), nov.ctypes.data, nd,
+                                           int(W), int(log2tile),
+                                           per.ctypes.data, ct.byref(w),
+                                           ct.byref(i), ct.byref(x)))
+    return words, idx, xarg
 
 
 def sort_points(coord, nover, W, log2tile, periodicity=_TWO_PI):

@@ -73,6 +73,15 @@ typedef struct dmx_options {
                               once the work is queued on the stream (results
                               are ordered on that stream, as torch
                               operations are); host arrays always wait */
+#define DMX_POINT_RECORDS 4 /* unplanned transforms sort the points' records
+                              (tile-local offsets, kernel arguments and
+                              value or input index) instead of a
+                              permutation, so that spreading and
+                              interpolation read them in order; same results
+                              (type 2 bit for bit).  Off by default: on
+                              MI355X the record sort moves 2.5x the bytes
+                              of the index sort and costs more than the
+                              gathers it saves */
 
 typedef struct dmx_plan dmx_plan;
 
@@ -167,6 +176,19 @@ int dmx_sort_points(const dmx_array *coord, const int64_t *nover,
                     const double *periodicity, dmx_array *keys,
                     dmx_array *idx);
 
+/* The same sort carrying the points' records (the index build of unplanned
+   transforms): for the sorted position p, words[p] = tile << (ndim*log2tile)
+   | tile-local offsets (log2tile bits per axis, axis d at bit d*log2tile),
+   idx[p] = the point's input index (the permutation of dmx_sort_points) and,
+   when xarg is not NULL, xarg[p, d] = W-1-2*frac of axis d (float64
+   compute type).  Exposed for bit-exact parity tests.  words/idx: uint32
+   arrays of M, xarg: (M, ndim) float64, device or host.  Fails where the
+   tile key and offsets need more than 32 bits. */
+int dmx_sort_records(const dmx_array *coord, const int64_t *nover,
+                     size_t ndim, int W, int log2tile,
+                     const double *periodicity, dmx_array *words,
+                     dmx_array *idx, dmx_array *xarg);
+
 /* Complex FFT along chosen axes: replaces ducc0::c2c(cfmav in, vfmav out,
    shape_t axes, bool forward, T fct, size_t nthreads)
    (src/ducc0/fft/fft.h:848-850), as bound by ducc0.fft.c2c
@@ -243,6 +265,42 @@ int dmx_nufft_noalloc(int type, const dmx_array *uniform,
                       double sigma_max, const double *periodicity,
                       int fft_order, const dmx_options *opt,
                       dmx_plan_info *info);
+
+/* ---- spherical harmonic synthesis at arbitrary points (spin 0) ----
+   ducc0.sht.synthesis_general (src/ducc0/sht/sht.cc:2926-2960, bound at
+   python/sht_pymod.cc:805-845, 2299): map[0, i] = sum_{l, m} a_lm
+   Y_lm(loc[i, 0], loc[i, 1]) for the real field whose a_lm (m >= 0) are
+   given, i.e. theta = loc[i, 0] in [0, pi], phi = loc[i, 1].
+   alm: (1, nalm) complex64 / complex128, a_lm at index mstart[m] +
+   l * lstride (m = 0 .. nm-1 = mmax, l = m .. lmax); map: (1, M) float32 /
+   float64 matching the a_lm precision (any stride); loc: (M, 2) float64.
+   The points are interpolated from the doubled sphere's Fourier
+   coefficients with the NUFFT to accuracy epsilon (sigma_min / sigma_max
+   as for the NUFFT; ducc0's defaults are 1.1 / 2.6).  spin must be 0 and
+   mode 0 (STANDARD): anything else fails with "not yet supported".
+   nthreads is accepted and ignored; verbosity > 0 prints the NUFFT
+   parameters. */
+int dmx_sht_synthesis_general(const dmx_array *alm, dmx_array *map,
+                              size_t spin, size_t lmax,
+                              const int64_t *mstart, size_t nm,
+                              int64_t lstride, const dmx_array *loc,
+                              double epsilon, double sigma_min,
+                              double sigma_max, size_t nthreads, int mode,
+                              size_t verbosity, const dmx_options *opt);
+/* ducc0.sht.adjoint_synthesis_general (src/ducc0/sht/sht.cc:2962-2993,
+   python/sht_pymod.cc:847-870, 2300): the adjoint of the above with respect
+   to the real inner products of the map and of the a_lm (m > 0 entries
+   counted twice), written into alm at the (l, m) positions of the layout;
+   for host arrays the other entries come back zero, device arrays keep
+   them.  Same arguments. */
+int dmx_sht_adjoint_synthesis_general(dmx_array *alm, const dmx_array *map,
+                                      size_t spin, size_t lmax,
+                                      const int64_t *mstart, size_t nm,
+                                      int64_t lstride, const dmx_array *loc,
+                                      double epsilon, double sigma_min,
+                                      double sigma_max, size_t nthreads,
+                                      int mode, size_t verbosity,
+                                      const dmx_options *opt);
 
 /* Number of HIP devices visible (0 when none). */
 int dmx_device_count(void);
