@@ -171,6 +171,93 @@ def c2c_cpu(R, a, nthreads, reps):
     return (time.perf_counter() - t0) / reps, out
 
 
+def fibonacci_lattice(npoints):
+    """(theta, phi) of a Fibonacci lattice (python/demos/sht_general.py:20-26)"""
+    gold = (1. + np.sqrt(5.)) / 2.
+    r = np.arange(npoints, dtype=np.float64)
+    loc = np.empty((npoints, 2))
+    loc[:, 0] = np.arccos(1. - (2 / npoints * r))
+    loc[:, 1] = np.mod(r / gold, 1.) * (2 * np.pi)
+    return loc
+
+
+def bench_sht(args):
+    """ducc0.sht.synthesis_general + adjoint_synthesis_general, spin 0, on
+    the reference demo's sky (python/demos/sht_general.py:29-41: lmax = mmax
+    = 1187, 2,000,000 points of a Fibonacci lattice, random a_lm), complex128
+    a_lm, eps 1e-7.  One step is one synthesis plus one adjoint with inputs
+    on the device; ducc0 itself (oracle/_ref) on all host cores is the CPU
+    baseline (median of 3 after a warm-up)."""
+    import torch
+    from ducc_amd import _lib, sht
+    from oracle import sht as S
+    dev = torch.device("cuda:0")
+    lmax, npt, eps = 1187, 2_000_000, 1e-7
+    rng = np.random.default_rng(42)
+    a_np = S.random_alm(lmax, lmax, rng)
+    loc_np = fibonacci_lattice(npt)
+    r_np = rng.uniform(-1, 1, npt)
+    a = torch.from_numpy(a_np[None, :]).to(dev)
+    loc = torch.from_numpy(loc_np).to(dev)
+    r = torch.from_numpy(r_np[None, :]).to(dev)
+    mp = torch.empty((1, npt), dtype=torch.float64, device=dev)
+    at = torch.empty_like(a)
+    sync = torch.cuda.synchronize
+
+    def step():
+        sht.synthesis_general(alm=a, spin=0, lmax=lmax, loc=loc, epsilon=eps,
+                              map=mp)
+        sht.adjoint_synthesis_general(map=r, spin=0, lmax=lmax, loc=loc,
+                                      epsilon=eps, alm=at)
+    steps = max(1, args.steps // 2)
+    t = time_steps(step, steps, max(1, args.warmup // 2), sync)
+    _lib.profile_enable(True)
+    _lib.profile_reset()
+    step()
+    sync()
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            from oracle import ref as R
+            if R.available():
+                nth = os.cpu_count() or 1
+                f_ref = R.sht_synthesis_general(a_np, lmax, loc_np, eps,
+                                                nthreads=nth)
+                b_ref = R.sht_adjoint_synthesis_general(
+                    r_np, lmax, loc_np, eps, len(a_np), nthreads=nth)
+                ts = _median_time(lambda: R.sht_synthesis_general(
+                    a_np, lmax, loc_np, eps, nthreads=nth), reps=3)
+                ta = _median_time(lambda: R.sht_adjoint_synthesis_general(
+                    r_np, lmax, loc_np, eps, len(a_np), nthreads=nth), reps=3)
+                cpu = dict(value=npt * 2 / (ts + ta), unit="points/s",
+                           cores=nth, kind="reference",
+                           synthesis_s=round(ts, 4), adjoint_s=round(ta, 4),
+                           err_vs_ducc0=dict(
+                               synthesis=l2rel(mp.cpu().numpy()[0], f_ref),
+                               adjoint=l2rel(at.cpu().numpy()[0], b_ref)),
+                           sample="ducc0 synthesis_general + "
+                                  "adjoint_synthesis_general, same inputs, "
+                                  "median of 3 after a warm-up")
+        except Exception as e:
+            cpu = dict(error=str(e))
+    line = dict(
+        metric="points/s, spherical harmonic synthesis + adjoint at "
+               "arbitrary points (spin 0)",
+        value=round(2 * npt * steps / t, 1), unit="points/s", n_gpus=1,
+        steps=steps, warmup=max(1, args.warmup // 2),
+        ms_per_step=round(1e3 * t / steps, 4), higher_is_better=True,
+        scaling="weak", vs_baseline=None, dtype="fp64", data="synthetic",
+        config=dict(workload="ducc0.sht.synthesis_general + "
+                             "adjoint_synthesis_general, spin 0, lmax = mmax "
+                             "= 1187, 2,000,000 Fibonacci-lattice points, "
+                             "eps 1e-7, complex128 a_lm"),
+        cpu_baseline=cpu,
+        phases_ms={k: round(v[0], 4) for k, v in prof.items()})
+    print(json.dumps(line))
+
+
 def bench_c2c(args):
     """ducc0.fft.c2c drop-in: in-place forward c2c of a 4096 x 4096
     complex64 array (the headline of this mode, HBM-bound: each transformed
@@ -290,9 +377,11 @@ def bench_c2c(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="nufft", choices=["nufft", "c2c"],
+    ap.add_argument("--workload", default="nufft",
+                    choices=["nufft", "c2c", "sht"],
                     help="nufft: the 2-D type-1 + type-2 workload (default); "
-                         "c2c: the ducc0.fft.c2c drop-in")
+                         "c2c: the ducc0.fft.c2c drop-in; sht: spherical "
+                         "harmonic synthesis at arbitrary points")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--M", type=int, default=10_000_000)
@@ -322,6 +411,9 @@ def main():
         del x, y
     if args.workload == "c2c":
         bench_c2c(args)
+        return
+    if args.workload == "sht":
+        bench_sht(args)
         return
     from ducc_amd import _lib, nufft
     from ducc_amd import chooser as learned

@@ -11,11 +11,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libducc_amd.so")
 
-DMX_F32, DMX_F64, DMX_C64, DMX_C128, DMX_U32 = 3, 7, 67, 71, 34
+DMX_F32, DMX_F64, DMX_C64, DMX_C128 = 3, 7, 67, 71
+DMX_U8, DMX_U32 = 32, 35
 
 _NP_TC = {np.dtype(np.float32): DMX_F32, np.dtype(np.float64): DMX_F64,
           np.dtype(np.complex64): DMX_C64, np.dtype(np.complex128): DMX_C128,
-          np.dtype(np.uint32): DMX_U32}
+          np.dtype(np.uint8): DMX_U8, np.dtype(np.uint32): DMX_U32}
 
 
 class DmxArray(ct.Structure):
@@ -119,6 +120,18 @@ _SIG = {
                    ct.c_int64, _AP, ct.c_double, ct.c_double, ct.c_double,
                    ct.c_size_t, ct.c_int, ct.c_size_t,
                    ct.POINTER(DmxOptions)]),
+    "dmx_vis2dirty": (ct.c_int, [_AP, _AP, _AP, _AP, _AP, ct.c_double,
+                                 ct.c_double, ct.c_double, ct.c_int,
+                                 ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                                 ct.c_double, ct.c_double, ct.c_double,
+                                 ct.c_double, ct.c_int, ct.c_size_t,
+                                 ct.c_size_t, _AP, ct.POINTER(DmxOptions)]),
+    "dmx_dirty2vis": (ct.c_int, [_AP, _AP, _AP, _AP, _AP, ct.c_double,
+                                 ct.c_double, ct.c_double, ct.c_int,
+                                 ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                                 ct.c_double, ct.c_double, ct.c_double,
+                                 ct.c_double, ct.c_int, ct.c_size_t,
+                                 ct.c_size_t, _AP, ct.POINTER(DmxOptions)]),
     "dmx_device_count": (ct.c_int, []),
 }
 
@@ -175,7 +188,8 @@ def describe(x):
     if _is_torch(x):
         import torch
         tmap = {torch.float32: DMX_F32, torch.float64: DMX_F64,
-                torch.complex64: DMX_C64, torch.complex128: DMX_C128}
+                torch.complex64: DMX_C64, torch.complex128: DMX_C128,
+                torch.uint8: DMX_U8}
         if x.dtype not in tmap:
             raise RuntimeError("not yet supported")
         a.data = x.data_ptr() if x.numel() else None

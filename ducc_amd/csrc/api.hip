@@ -23,6 +23,7 @@
 #include "params.h"
 #include "runtime.h"
 #include "sht.h"
+#include "wgridder.h"
 
 namespace dmx {
 
@@ -51,6 +52,7 @@ size_t elsize(int tc) {
     case DMX_C64: return 8;
     case DMX_C128: return 16;
     case DMX_U32: return 4;
+    case DMX_U8: return 1;
   }
   throw Error("not yet supported");
 }
@@ -915,11 +917,9 @@ struct ShtJob {
                     1.0 / (double)rows(), s);
   }
   // the points sorted for the NUFFT of this grid
-  std::unique_ptr<Plan> plan(const void *loc, bool type1, const void *vals,
-                             hipStream_t s) {
+  std::unique_ptr<Plan> plan(const void *loc, hipStream_t s) {
     auto pl = std::make_unique<Plan>(p, s);
-    if (!pl->build_records(loc, false, M, type1, vals, true))
-      pl->build_index(loc, false, M, false);
+    pl->build_index(loc, false, M, false);
     return pl;
   }
 };
@@ -949,7 +949,7 @@ int dmx_sht_synthesis_general(const dmx_array *alm, dmx_array *map,
     auto keep = job.theta_fft(G.p, true, s);
     Buf z(job.M * 16, s);
     if (job.M) {
-      auto pl = job.plan(lin.p, false, nullptr, s);
+      auto pl = job.plan(lin.p, s);
       const int64_t ust[2] = {job.geo.nphi, 1};
       pl->u2nu(false, G.p, true, ust, z.p, true);
       launch_real_part(z.as<double2>(), job.M, mo.p, job.vf64, mo.st[1], s);
@@ -989,7 +989,7 @@ int dmx_sht_adjoint_synthesis_general(dmx_array *alm, const dmx_array *map,
     if (job.M) {
       launch_to_complex(min_.p, job.vf64, min_.st[1], job.M, z.as<double2>(),
                         s);
-      pl = job.plan(lin.p, true, z.p, s);
+      pl = job.plan(lin.p, s);
       const int64_t ust[2] = {job.geo.nphi, 1};
       pl->nu2u(true, z.p, true, Gt.p, true, ust);
       if (verbosity > 0) pl->report_params(true);
@@ -1003,6 +1003,162 @@ int dmx_sht_adjoint_synthesis_general(dmx_array *alm, const dmx_array *map,
     launch_store_alm(acc.as<double2>(), job.args, ao.p, job.vf64, s);
     ao.finish(s);
     if (pl) pl->mark_done();
+  });
+}
+
+namespace {
+// shared validation and set-up of vis2dirty / dirty2vis
+// (wgridder_pymod.cc:483-560, wgridder.h:1626-1700)
+void wg_common(WgJob &job, const dmx_array &uvw, const dmx_array &freq,
+               const dmx_array *wgt, const dmx_array *mask,
+               const dmx_array &dirty, int64_t nrow, int64_t nchan,
+               double pixsize_x, double pixsize_y, int do_wgridding,
+               int flip_u, int flip_v, int flip_w, int divide_by_n,
+               double center_x, double center_y, int allow_nshift,
+               bool single, hipStream_t s, std::unique_ptr<DevIn> &uin,
+               std::unique_ptr<DevIn> &win, std::unique_ptr<DevIn> &min_) {
+  DMX_ASSERT(uvw.ndim == 2 && uvw.shape[1] == 3 && uvw.typecode == DMX_F64,
+             "uvw must be a (nrow, 3) float64 array");
+  DMX_ASSERT(freq.ndim == 1 && freq.typecode == DMX_F64,
+             "freq must be a float64 vector");
+  DMX_ASSERT(uvw.shape[0] == nrow && freq.shape[0] == nchan,
+             "dimension mismatch");
+  DMX_ASSERT(dirty.ndim == 2, "dirty must be a 2D array");
+  DMX_ASSERT(dirty.typecode == (single ? DMX_F32 : DMX_F64),
+             "unsupported combination of data types");
+  if (wgt) {
+    DMX_ASSERT(wgt->ndim == 2 && wgt->shape[0] == nrow &&
+                   wgt->shape[1] == nchan,
+               "wgt: dimension mismatch");
+    DMX_ASSERT(wgt->typecode == (single ? DMX_F32 : DMX_F64),
+               "unsupported combination of data types");
+  }
+  if (mask) {
+    DMX_ASSERT(mask->ndim == 2 && mask->shape[0] == nrow &&
+                   mask->shape[1] == nchan,
+               "mask: dimension mismatch");
+    DMX_ASSERT(mask->typecode == DMX_U8, "mask must be uint8");
+  }
+  // frequencies on the host for the checks and f / c
+  std::vector<double> fh((size_t)nchan);
+  {
+    dmx_array f = freq;
+    if (freq.device) {
+      DMX_ASSERT(contiguous(freq), "device arrays must be C-contiguous here");
+      if (nchan)
+        DMX_HIP(hipMemcpy(fh.data(), freq.data, (size_t)nchan * 8,
+                          hipMemcpyDeviceToHost));
+    } else {
+      pack(f, fh.data());
+    }
+  }
+  uin = std::make_unique<DevIn>(uvw, s, false);
+  if (wgt) win = std::make_unique<DevIn>(*wgt, s, true);
+  if (mask) min_ = std::make_unique<DevIn>(*mask, s, true);
+  WgArgs &a = job.a;
+  a.uvw = (const double *)uin->p;
+  a.wgt = wgt ? win->p : nullptr;
+  a.wgt_f64 = single ? 0 : 1;
+  if (wgt) {
+    a.wst0 = win->st[0];
+    a.wst1 = win->st[1];
+  }
+  a.mask = mask ? (const uint8_t *)min_->p : nullptr;
+  if (mask) {
+    a.mst0 = min_->st[0];
+    a.mst1 = min_->st[1];
+  }
+  a.nx = (int)dirty.shape[0];
+  a.ny = (int)dirty.shape[1];
+  a.px = pixsize_x;
+  a.py = pixsize_y;
+  a.ufac = flip_u ? -1.0 : 1.0;
+  a.vfac = flip_v ? -1.0 : 1.0;
+  a.wfac = flip_w ? -1.0 : 1.0;
+  a.lshift = flip_u ? -center_x : center_x;
+  a.mshift = flip_v ? -center_y : center_y;
+  a.wgrid = do_wgridding ? 1 : 0;
+  a.divide_by_n = divide_by_n ? 1 : 0;
+  a.allow_nshift = allow_nshift ? 1 : 0;
+  job.setup(fh.data(), (size_t)nchan, (size_t)nrow, s);
+}
+}  // namespace
+
+int dmx_vis2dirty(const dmx_array *uvw, const dmx_array *freq,
+                  const dmx_array *vis, const dmx_array *wgt,
+                  const dmx_array *mask, double pixsize_x, double pixsize_y,
+                  double epsilon, int do_wgridding, int flip_u, int flip_v,
+                  int flip_w, int divide_by_n, double sigma_min,
+                  double sigma_max, double center_x, double center_y,
+                  int allow_nshift, size_t nthreads, size_t verbosity,
+                  dmx_array *dirty, const dmx_options *opt_in) {
+  (void)nthreads;
+  (void)verbosity;
+  return guard([&] {
+    DMX_ASSERT(uvw && freq && vis && dirty, "null argument");
+    DMX_ASSERT(epsilon > 0, "epsilon must be positive");
+    reap(false);
+    dmx_options opt = opt_in ? *opt_in : defaults();
+    DMX_ASSERT(vis->ndim == 2 && is_complex(vis->typecode),
+               "vis must be a complex (nrow, nchan) array");
+    const bool single = vis->typecode == DMX_C64;
+    hipStream_t s = pick_stream(opt);
+    WgJob job;
+    job.a.gridding = 1;
+    std::unique_ptr<DevIn> uin, win, min_;
+    wg_common(job, *uvw, *freq, wgt, mask, *dirty, vis->shape[0],
+              vis->shape[1], pixsize_x, pixsize_y, do_wgridding, flip_u,
+              flip_v, flip_w, divide_by_n, center_x, center_y, allow_nshift,
+              single, s, uin, win, min_);
+    DevIn vin(*vis, s, true);
+    DevOut dout(*dirty, s, true);
+    job.prep(vin.p, !single, vin.st[0], vin.st[1]);
+    if (job.nvis == 0) {
+      job.zero_dirty(dout.p, !single, dout.st[0], dout.st[1]);
+    } else {
+      job.choose(epsilon, sigma_min, sigma_max, !single, opt);
+      job.grid(dout.p, !single, dout.st[0], dout.st[1]);
+    }
+    dout.finish(s);
+  });
+}
+
+int dmx_dirty2vis(const dmx_array *uvw, const dmx_array *freq,
+                  const dmx_array *dirty, const dmx_array *wgt,
+                  const dmx_array *mask, double pixsize_x, double pixsize_y,
+                  double epsilon, int do_wgridding, int flip_u, int flip_v,
+                  int flip_w, int divide_by_n, double sigma_min,
+                  double sigma_max, double center_x, double center_y,
+                  int allow_nshift, size_t nthreads, size_t verbosity,
+                  dmx_array *vis, const dmx_options *opt_in) {
+  (void)nthreads;
+  (void)verbosity;
+  return guard([&] {
+    DMX_ASSERT(uvw && freq && vis && dirty, "null argument");
+    DMX_ASSERT(epsilon > 0, "epsilon must be positive");
+    reap(false);
+    dmx_options opt = opt_in ? *opt_in : defaults();
+    DMX_ASSERT(vis->ndim == 2 && is_complex(vis->typecode),
+               "vis must be a complex (nrow, nchan) array");
+    const bool single = vis->typecode == DMX_C64;
+    hipStream_t s = pick_stream(opt);
+    WgJob job;
+    job.a.gridding = 0;
+    std::unique_ptr<DevIn> uin, win, min_;
+    wg_common(job, *uvw, *freq, wgt, mask, *dirty, vis->shape[0],
+              vis->shape[1], pixsize_x, pixsize_y, do_wgridding, flip_u,
+              flip_v, flip_w, divide_by_n, center_x, center_y, allow_nshift,
+              single, s, uin, win, min_);
+    DevIn din(*dirty, s, true);
+    DevOut vout(*vis, s, true);
+    job.prep(nullptr, !single, 0, 0);
+    if (job.M) {
+      // (inactive visibilities, all of them when nvis == 0, come back zero)
+      job.choose(epsilon, sigma_min, sigma_max, !single, opt);
+      job.degrid(din.p, !single, din.st[0], din.st[1], vout.p, !single,
+                 vout.st[0], vout.st[1]);
+    }
+    vout.finish(s);
   });
 }
 

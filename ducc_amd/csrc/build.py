@@ -19,7 +19,7 @@ GEN = os.path.join(HERE, "_gen")
 SOURCES = ["params.cpp", "runtime.hip", "api.hip", "sort.hip", "spread.hip",
            "interp.hip", "fft.hip", "grid.hip", "chooser.hip", "mlp.hip",
            "c2c.hip", "noalloc.hip", "spread_det.hip", "records.hip",
-           "sht.hip"]
+           "sht.hip", "wgridder.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DMX_ARCH", "gfx950")

@@ -194,6 +194,22 @@ __device__ __forceinline__ void kernel_weights_deriv(T x,
   }
 }
 
+// weight of kernel piece q alone at local coordinate x (the same even / odd
+// Horner in x^2 as kernel_weights, for a run-time support W)
+template <typename T>
+__device__ __forceinline__ T piece_weight(T x, const T *__restrict__ cf, int W,
+                                          int q) {
+  const int DP = W + 3 + (W & 1), NE = (DP + 1) / 2, NH = (W + 1) / 2;
+  const int i = q < NH ? q : W - 1 - q;
+  const T x2 = x * x;
+  T o = cf[i], e = cf[NH + i];
+  for (int s = 1; s < NE; ++s) {
+    o = o * x2 + cf[(2 * s) * NH + i];
+    e = e * x2 + cf[(2 * s + 1) * NH + i];
+  }
+  return q < NH ? e + x * o : e - x * o;
+}
+
 __device__ __forceinline__ int wrapi(int i, int n) {
   i %= n;
   return i < 0 ? i + n : i;

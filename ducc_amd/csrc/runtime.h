@@ -150,6 +150,14 @@ Params choose_params(int ndim, const size_t *nuni, size_t M, bool gridding,
                      const double *period, bool fft_order, bool values_f64,
                      const dmx_options &opt);
 
+// the parameters for kernel-table row kidx (of any dimensionality) at
+// oversampling factor ofactor, with the tile logic of choose_params
+Params kernel_params(int ndim, const size_t *nuni, const double *period,
+                     bool fft_order, bool fp64, double eps, int kidx,
+                     double ofactor, const dmx_options &opt);
+// choose_params' common tail: kernel row, limits, tile
+void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt);
+
 class Plan {
  public:
   Plan(const Params &p, hipStream_t s);

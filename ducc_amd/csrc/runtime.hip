@@ -356,7 +356,13 @@ Params choose_params(int ndim, const size_t *nuni, size_t M, bool gridding,
                        p.nover);
     DMX_ASSERT(kidx >= 0, "no appropriate kernel found");
   }
-  const KRow &k = db[kidx];
+  finish_params(p, kidx, l2t, opt);
+  return p;
+}
+
+void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt) {
+  const int ndim = p.ndim;
+  const KRow &k = kernel_db()[kidx];
   p.kidx = kidx;
   p.W = k.W;
   p.beta = k.beta;
@@ -382,6 +388,27 @@ Params choose_params(int ndim, const size_t *nuni, size_t M, bool gridding,
   size_t ntot = 1;
   for (int d = 0; d < ndim; ++d) ntot *= ((p.nover[d] >> l2t) + 3);
   DMX_ASSERT(ntot < 0xFFFFFFFFull, "oversampled grid too large");
+}
+
+Params kernel_params(int ndim, const size_t *nuni, const double *period,
+                     bool fft_order, bool fp64, double eps, int kidx,
+                     double ofactor, const dmx_options &opt) {
+  DMX_ASSERT(ndim >= 1 && ndim <= 3, "transform must be 1D/2D/3D");
+  DMX_ASSERT(kidx >= 0 && kidx < (int)kernel_db().size(),
+             "no appropriate kernel found");
+  DMX_ASSERT(fp64 || kernel_db()[kidx].W <= 8, "FP32 path supports W <= 8");
+  Params p;
+  p.ndim = ndim;
+  for (int d = 0; d < ndim; ++d) {
+    p.nuni[d] = nuni[d];
+    p.period[d] = period[d];
+    p.nover[d] = oversampled_length(nuni[d], ofactor);
+  }
+  p.fft_order = fft_order;
+  p.eps = eps;
+  p.fp64 = fp64;
+  finish_params(p, kidx, 0, opt);
+  p.ofactor = ofactor;
   return p;
 }
 

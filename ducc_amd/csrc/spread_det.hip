@@ -24,22 +24,6 @@ namespace dmx {
 
 namespace {
 
-// weight of kernel piece q at local coordinate x (the same even / odd
-// Horner in x^2 as kernel_weights, for one piece and a run-time support)
-template <typename T>
-__device__ __forceinline__ T piece_weight(T x, const T *__restrict__ cf, int W,
-                                          int q) {
-  const int DP = W + 3 + (W & 1), NE = (DP + 1) / 2, NH = (W + 1) / 2;
-  const int i = q < NH ? q : W - 1 - q;
-  const T x2 = x * x;
-  T o = cf[i], e = cf[NH + i];
-  for (int s = 1; s < NE; ++s) {
-    o = o * x2 + cf[(2 * s) * NH + i];
-    e = e * x2 + cf[(2 * s + 1) * NH + i];
-  }
-  return q < NH ? e + x * o : e - x * o;
-}
-
 template <typename T, int NDIM>
 __global__ __launch_bounds__(64) void k_spread_det(SpreadArgs a) {
   extern __shared__ unsigned char smem_raw[];

@@ -23,13 +23,15 @@
 extern "C" {
 #endif
 
-/* typecodes (role of src/ducc0/bindings/typecode.h:44-80) */
+/* typecodes (the values of src/ducc0/bindings/typecode.h:44-80: element
+   size - 1, plus 32 for unsigned integers, plus 64 for complex) */
 enum {
   DMX_F32 = 3,
   DMX_F64 = 7,
   DMX_C64 = 67,
   DMX_C128 = 71,
-  DMX_U32 = 34
+  DMX_U8 = 32,
+  DMX_U32 = 35
 };
 
 typedef struct dmx_array {
@@ -301,6 +303,46 @@ int dmx_sht_adjoint_synthesis_general(dmx_array *alm, const dmx_array *map,
                                       double sigma_max, size_t nthreads,
                                       int mode, size_t verbosity,
                                       const dmx_options *opt);
+
+/* ---- radio-interferometric gridding with the w-term ----
+   ducc0.wgridder.vis2dirty (src/ducc0/wgridder/wgridder.h:1626-1714, bound
+   at python/wgridder_pymod.cc:561-640; ms2dirty :483-493 is the same call
+   with fixed options): for pixel (i, j) at x = cx + (i - nx/2) pixsize_x,
+   y = cy + (j - ny/2) pixsize_y, n = sqrt(1 - x^2 - y^2),
+     dirty(i, j) = sum_{row, chan} Re(vis wgt e^{2 pi i (x u + y v - w (n-1))}) / n
+   with (u, v, w) = uvw[row] * freq[chan] / c (each sign flipped by flip_u /
+   v / w, cx / cy flipped with u / v); without do_wgridding the w term and
+   the division are absent; divide_by_n = 0 skips the division.  Masked
+   (mask == 0) or zero-weight visibilities are skipped.
+   uvw: (nrow, 3) float64; freq: (nchan,) float64, positive, ascending;
+   vis: (nrow, nchan) complex64 / complex128 (the precision of the call);
+   wgt: (nrow, nchan) real of that precision or NULL; mask: (nrow, nchan)
+   uint8 or NULL; dirty: (npix_x, npix_y) real of that precision, both even.
+   allow_nshift = 1 lets the w planes be centred on the image's n - 1
+   range (wgridder.h:1520), as ducc0 does by default.  Accuracy epsilon;
+   sigma_min / sigma_max bound the oversampling (ducc0's defaults 1.1 /
+   2.6).  nthreads and verbosity are accepted and ignored. */
+int dmx_vis2dirty(const dmx_array *uvw, const dmx_array *freq,
+                  const dmx_array *vis, const dmx_array *wgt,
+                  const dmx_array *mask, double pixsize_x, double pixsize_y,
+                  double epsilon, int do_wgridding, int flip_u, int flip_v,
+                  int flip_w, int divide_by_n, double sigma_min,
+                  double sigma_max, double center_x, double center_y,
+                  int allow_nshift, size_t nthreads, size_t verbosity,
+                  dmx_array *dirty, const dmx_options *opt);
+/* ducc0.wgridder.dirty2vis (wgridder.h:1716-1735, wgridder_pymod.cc:
+   594-620; dirty2ms :544-550): the adjoint,
+     vis(row, chan) = wgt sum_{i, j} dirty(i, j) / n e^{-2 pi i (x u + y v - w (n-1))},
+   zero for masked or zero-weight visibilities.  Same arguments; `vis` is
+   the output. */
+int dmx_dirty2vis(const dmx_array *uvw, const dmx_array *freq,
+                  const dmx_array *dirty, const dmx_array *wgt,
+                  const dmx_array *mask, double pixsize_x, double pixsize_y,
+                  double epsilon, int do_wgridding, int flip_u, int flip_v,
+                  int flip_w, int divide_by_n, double sigma_min,
+                  double sigma_max, double center_x, double center_y,
+                  int allow_nshift, size_t nthreads, size_t verbosity,
+                  dmx_array *vis, const dmx_options *opt);
 
 /* Number of HIP devices visible (0 when none). */
 int dmx_device_count(void);

@@ -42,6 +42,10 @@ def lib():
         L.ref_plan_u2nu.argtypes = [P, I, P, P]
         L.ref_plan_destroy.argtypes = [P]
         L.ref_plan_destroy.restype = None
+        L.ref_sht_general.argtypes = [I, I, P, S, P, S, P, S, ct.c_ssize_t, P,
+                                      S, D, D, D, S]
+        L.ref_wgridder.argtypes = [I, I, P, S, P, S, P, P, P, P, S, S, D, D,
+                                   D, I, I, I, I, I, D, D, D, D, S]
         _lib = L
     return _lib
 
@@ -160,6 +164,83 @@ def c2c(a, axes=None, forward=True, fct=1.0, nthreads=1):
                          _p(ax), len(ax), int(bool(forward)), float(fct),
                          int(nthreads)))
     return out
+
+
+def _mstart(lmax, mstart):
+    if mstart is None:
+        ms, idx = [], 0
+        for m in range(lmax + 1):
+            ms.append(idx)
+            idx += lmax - m
+        return np.asarray(ms, np.uintp)
+    return np.ascontiguousarray(mstart, np.uintp)
+
+
+def sht_synthesis_general(alm, lmax, loc, epsilon, mstart=None, lstride=1,
+                          sigma_min=1.1, sigma_max=2.6, nthreads=1):
+    """ducc0.sht.synthesis_general (sht.cc:2926-2960), spin 0: alm (nalm,)
+    complex64 / complex128 -> map (npoints,) float32 / float64"""
+    a = np.ascontiguousarray(alm)
+    single = a.dtype == np.complex64
+    if not single:
+        a = a.astype(np.complex128, copy=False)
+    ms = _mstart(lmax, mstart)
+    lc = np.ascontiguousarray(loc, np.float64)
+    out = np.zeros(len(lc), np.float32 if single else np.float64)
+    _check(lib().ref_sht_general(int(single), 0, _p(a), a.size, _p(out),
+                                 lmax, _p(ms), len(ms), lstride, _p(lc),
+                                 len(lc), epsilon, sigma_min, sigma_max,
+                                 nthreads))
+    return out
+
+
+def sht_adjoint_synthesis_general(mapv, lmax, loc, epsilon, nalm,
+                                  mstart=None, lstride=1, sigma_min=1.1,
+                                  sigma_max=2.6, nthreads=1):
+    """ducc0.sht.adjoint_synthesis_general (sht.cc:2962-2993), spin 0"""
+    m = np.ascontiguousarray(mapv)
+    single = m.dtype == np.float32
+    if not single:
+        m = m.astype(np.float64, copy=False)
+    ms = _mstart(lmax, mstart)
+    lc = np.ascontiguousarray(loc, np.float64)
+    out = np.zeros(nalm, np.complex64 if single else np.complex128)
+    _check(lib().ref_sht_general(int(single), 1, _p(out), nalm, _p(m), lmax,
+                                 _p(ms), len(ms), lstride, _p(lc), len(lc),
+                                 epsilon, sigma_min, sigma_max, nthreads))
+    return out
+
+
+def wgridder(adjoint, uvw, freq, vis_or_dirty, wgt, mask, npix_x, npix_y,
+             pixsize_x, pixsize_y, epsilon, do_wgridding, flip_u=False,
+             flip_v=False, flip_w=False, divide_by_n=True, sigma_min=1.1,
+             sigma_max=2.6, center_x=0., center_y=0., nthreads=1):
+    """ducc0's ms2dirty (adjoint False: vis (nrow, nchan) complex -> dirty)
+    or dirty2ms (adjoint True: dirty (npix_x, npix_y) real -> vis),
+    wgridder.h:1700-1735; precision from the input's dtype"""
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    freq = np.ascontiguousarray(freq, np.float64)
+    x = np.ascontiguousarray(vis_or_dirty)
+    single = x.dtype in (np.complex64, np.float32)
+    rt = np.float32 if single else np.float64
+    ct_ = np.complex64 if single else np.complex128
+    nrow, nchan = uvw.shape[0], freq.shape[0]
+    if adjoint:
+        dirty = x.astype(rt, copy=False)
+        vis = np.zeros((nrow, nchan), ct_)
+    else:
+        vis = x.astype(ct_, copy=False)
+        dirty = np.zeros((npix_x, npix_y), rt)
+    w = None if wgt is None else np.ascontiguousarray(wgt, rt)
+    m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+    _check(lib().ref_wgridder(
+        int(single), int(adjoint), _p(uvw), nrow, _p(freq), nchan, _p(vis),
+        None if w is None else _p(w), None if m is None else _p(m),
+        _p(dirty), npix_x, npix_y, pixsize_x, pixsize_y, epsilon,
+        int(do_wgridding), int(flip_u), int(flip_v), int(flip_w),
+        int(divide_by_n), sigma_min, sigma_max, center_x, center_y,
+        nthreads))
+    return vis if adjoint else dirty
 
 
 class Plan:

@@ -12,6 +12,8 @@
 //   build_index (coord_idx order)   src/ducc0/nufft/nufft.h:892-907,1364,1862
 //   c2c                             src/ducc0/fft/fft.h:848-850
 //   plan (Nufft class)              src/ducc0/nufft/nufft.h:437-503
+//   synthesis_general / adjoint     src/ducc0/sht/sht.cc:2926-2993
+//   ms2dirty / dirty2ms (general)   src/ducc0/wgridder/wgridder.h:1700-1735
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -19,6 +21,14 @@
 
 #include "ducc0/nufft/nufft.h"
 #include "ducc0/fft/fftnd_impl.h"
+#include "ducc0/sht/sht.h"
+#include "ducc0/wgridder/wgridder.h"
+
+// sht.cc calls the real-FFT passes whose definitions live in fft1d_impl.h;
+// the library build instantiates them in the translation unit that includes
+// that header, which here is this one
+template class ducc0::detail_fft::rfftpass<float>;
+template class ducc0::detail_fft::rfftpass<double>;
 
 using namespace ducc0;
 using std::complex;
@@ -296,6 +306,88 @@ int ref_c2c(int single, int ndim, const int64_t *shape, const void *in,
       vfmav<complex<double>> o((complex<double> *)out, shp);
       c2c(a, o, ax, forward != 0, fct, nthreads);
     }
+  });
+}
+
+// ducc0.sht.synthesis_general (adjoint = 0) or adjoint_synthesis_general
+// (adjoint = 1), spin 0, one component: alm (1, nalm) complex, map (1, npt)
+// real, loc (npt, 2) float64; single != 0 -> complex64 / float32
+int ref_sht_general(int single, int adjoint, void *alm, size_t nalm,
+                    void *map, size_t lmax, const size_t *mstart, size_t nm,
+                    ptrdiff_t lstride, const double *loc, size_t npt,
+                    double eps, double smin, double smax, size_t nthreads) {
+  return guard([&] {
+    cmav<size_t, 1> ms(mstart, {nm});
+    cmav<double, 2> lc(loc, {npt, 2});
+    if (single) {
+      using T = float;
+      if (adjoint)
+        adjoint_synthesis_general(
+            vmav<complex<T>, 2>((complex<T> *)alm, {1, nalm}),
+            cmav<T, 2>((const T *)map, {1, npt}), 0, lmax, ms, lstride, lc,
+            eps, smin, smax, nthreads, STANDARD, false);
+      else
+        synthesis_general(cmav<complex<T>, 2>((const complex<T> *)alm,
+                                              {1, nalm}),
+                          vmav<T, 2>((T *)map, {1, npt}), 0, lmax, ms,
+                          lstride, lc, eps, smin, smax, nthreads, STANDARD,
+                          false);
+    } else {
+      using T = double;
+      if (adjoint)
+        adjoint_synthesis_general(
+            vmav<complex<T>, 2>((complex<T> *)alm, {1, nalm}),
+            cmav<T, 2>((const T *)map, {1, npt}), 0, lmax, ms, lstride, lc,
+            eps, smin, smax, nthreads, STANDARD, false);
+      else
+        synthesis_general(cmav<complex<T>, 2>((const complex<T> *)alm,
+                                              {1, nalm}),
+                          vmav<T, 2>((T *)map, {1, npt}), 0, lmax, ms,
+                          lstride, lc, eps, smin, smax, nthreads, STANDARD,
+                          false);
+    }
+  });
+}
+
+// the w-gridder (vis2dirty when adjoint == 0, dirty2vis when 1): uvw
+// (nrow, 3), freq (nchan), vis (nrow, nchan) complex, wgt (nrow, nchan)
+// real or NULL, mask (nrow, nchan) uint8 or NULL, dirty (nx, ny) real;
+// single != 0 -> float32 / complex64 throughout
+int ref_wgridder(int single, int adjoint, const double *uvw, size_t nrow,
+                 const double *freq, size_t nchan, void *vis,
+                 const void *wgt, const uint8_t *mask, void *dirty, size_t nx,
+                 size_t ny, double px, double py, double eps, int do_wg,
+                 int flip_u, int flip_v, int flip_w, int divide_by_n,
+                 double smin, double smax, double cx, double cy,
+                 size_t nthreads) {
+  return guard([&] {
+    cmav<double, 2> uv(uvw, {nrow, 3});
+    cmav<double, 1> fr(freq, {nchan});
+    // an empty array means "absent" to the reference
+    auto mk = mask ? cmav<uint8_t, 2>(mask, {nrow, nchan})
+                   : cmav<uint8_t, 2>(nullptr, {0, 0});
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      auto wg = wgt ? cmav<T, 2>((const T *)wgt, {nrow, nchan})
+                    : cmav<T, 2>(nullptr, {0, 0});
+      if (adjoint) {
+        cmav<T, 2> d((const T *)dirty, {nx, ny});
+        vmav<complex<T>, 2> v((complex<T> *)vis, {nrow, nchan});
+        dirty2ms<T, T>(uv, fr, d, wg, mk, px, py, eps, do_wg != 0, nthreads,
+                       v, 0, flip_u != 0, flip_v != 0, flip_w != 0,
+                       divide_by_n != 0, smin, smax, cx, cy, true);
+      } else {
+        cmav<complex<T>, 2> v((const complex<T> *)vis, {nrow, nchan});
+        vmav<T, 2> d((T *)dirty, {nx, ny});
+        ms2dirty<T, T>(uv, fr, v, wg, mk, px, py, eps, do_wg != 0, nthreads,
+                       d, 0, flip_u != 0, flip_v != 0, flip_w != 0,
+                       divide_by_n != 0, smin, smax, cx, cy, true);
+      }
+    };
+    if (single)
+      run(float(0));
+    else
+      run(double(0));
   });
 }
 

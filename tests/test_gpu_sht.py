@@ -175,3 +175,34 @@ def test_errors(gpu):
     with pytest.raises(RuntimeError, match="mmax>lmax"):
         sht.synthesis_general(alm=a, spin=0, lmax=4, mmax=5, loc=loc,
                               epsilon=1e-5)
+
+
+@pytest.mark.parametrize("cdt,eps", [(np.complex128, 1e-8),
+                                     (np.complex64, 1e-4)])
+def test_matches_ducc0(gpu, cdt, eps):
+    """lmax 500, 10^5 points: synthesis and adjoint within 2 eps of ducc0's
+    own synthesis_general / adjoint_synthesis_general (oracle/_ref, all host
+    cores)"""
+    import os
+    from oracle import ref as R
+    from ducc_amd import sht
+    if not R.available():
+        pytest.skip("reference library not built")
+    rng = np.random.default_rng(500)
+    lmax, npt = 500, 100_000
+    a = S.random_alm(lmax, lmax, rng).astype(cdt)
+    loc = _loc(rng, npt)
+    nth = os.cpu_count() or 1
+    got = sht.synthesis_general(alm=a[None, :], spin=0, lmax=lmax, loc=loc,
+                                epsilon=eps)[0]
+    ref = R.sht_synthesis_general(a, lmax, loc, eps, nthreads=nth)
+    assert O.l2error(got.astype(np.complex128),
+                     ref.astype(np.complex128)) <= 2 * eps
+    rdt = np.float64 if cdt == np.complex128 else np.float32
+    r = rng.uniform(-1, 1, npt).astype(rdt)
+    got_a = sht.adjoint_synthesis_general(map=r[None, :], spin=0, lmax=lmax,
+                                          loc=loc, epsilon=eps)[0]
+    ref_a = R.sht_adjoint_synthesis_general(r, lmax, loc, eps, len(a),
+                                            nthreads=nth)
+    assert O.l2error(got_a.astype(np.complex128),
+                     ref_a.astype(np.complex128)) <= 2 * eps

@@ -90,3 +90,23 @@ def test_a_planted_sign_error_is_caught():
     lam = -S.lambda_m(lmax, m, th)
     y = sph_harm_y(4, m, th, 0.0 * th)
     assert not np.allclose(lam[4 - m], y.real)
+
+
+def test_matches_ducc0_itself():
+    """the restatement against ducc0's own synthesis_general and
+    adjoint_synthesis_general (compiled from the reference sources into
+    oracle/_ref) at eps 1e-10, both precisions' layouts"""
+    from oracle import ref as R
+    from oracle import core as O
+    if not R.available():
+        pytest.skip("reference library not built")
+    rng = np.random.default_rng(1)
+    lmax = 40
+    a = S.random_alm(lmax, lmax, rng)
+    loc = _loc(rng, 500)
+    f = R.sht_synthesis_general(a, lmax, loc, 1e-10)
+    assert O.l2error(f.astype(complex), S.synthesis_general(a, lmax, loc)) \
+        <= 1e-10
+    r = rng.uniform(-1, 1, 500)
+    b = R.sht_adjoint_synthesis_general(r, lmax, loc, 1e-10, len(a))
+    assert O.l2error(b, S.adjoint_synthesis_general(r, lmax, loc)) <= 1e-10
