@@ -258,6 +258,98 @@ def bench_sht(args):
     print(json.dumps(line))
 
 
+def bench_wgridder(args):
+    """ducc0.wgridder vis2dirty + dirty2vis with w-gridding on a synthetic
+    observation shaped like the reference's benchmark
+    (python/demos/wgridder_bench.py: eps 1e-4, single precision, w-gridding,
+    image size from the field of view and the longest baseline, :11-17):
+    20,000 rows x 64 channels from 1 to 2 GHz (1.28e6 visibilities), u and v
+    uniform in +-1500 m, w in +-500 m, a 5 degree field, complex64.  One
+    step is one gridding plus one degridding with inputs on the device;
+    ducc0 itself (oracle/_ref) on all host cores is the CPU baseline
+    (median of 3 after a warm-up)."""
+    import torch
+    from ducc_amd import _lib, wgridder as W
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(7)
+    nrow, nchan, eps, fov = 20_000, 64, 1e-4, 5.0
+    c = 299792458.
+    freq = 1e9 + np.arange(nchan) * (1e9 / nchan)
+    uvw = (rng.random((nrow, 3)) - 0.5) * np.array([3000., 3000., 1000.])
+    vis = ((rng.random((nrow, nchan)) - 0.5) +
+           1j * (rng.random((nrow, nchan)) - 0.5)).astype(np.complex64)
+    wgt = rng.uniform(0.5, 1.5, (nrow, nchan)).astype(np.float32)
+    bl = np.sqrt(np.sum(uvw ** 2, axis=1))
+    maxbl = np.max(bl[:, None] * freq[None, :] / c)
+    npix = int(2 * fov * np.pi / 180 * maxbl) + 1
+    npix += npix % 2
+    pix = fov / npix * np.pi / 180
+    dirty = (rng.random((npix, npix)) - 0.5).astype(np.float32)
+    t = lambda x: torch.from_numpy(x).to(dev)
+    uvw_d, freq_d, vis_d, wgt_d, dirty_d = t(uvw), t(freq), t(vis), t(wgt), \
+        t(dirty)
+    out_d = torch.empty((npix, npix), dtype=torch.float32, device=dev)
+    vout_d = torch.empty((nrow, nchan), dtype=torch.complex64, device=dev)
+    sync = torch.cuda.synchronize
+    kw = dict(pixsize_x=pix, pixsize_y=pix, epsilon=eps, do_wgridding=True)
+
+    def step():
+        W.vis2dirty(uvw=uvw_d, freq=freq_d, vis=vis_d, wgt=wgt_d,
+                    dirty=out_d, **kw)
+        W.dirty2vis(uvw=uvw_d, freq=freq_d, dirty=dirty_d, wgt=wgt_d,
+                    vis=vout_d, **kw)
+    steps = max(1, args.steps // 2)
+    tt = time_steps(step, steps, max(1, args.warmup // 2), sync)
+    _lib.profile_enable(True)
+    _lib.profile_reset()
+    step()
+    sync()
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    nvis = nrow * nchan
+    cpu = None
+    if not args.no_cpu:
+        try:
+            from oracle import ref as R
+            if R.available():
+                nth = os.cpu_count() or 1
+                d_ref = R.wgridder(False, uvw, freq, vis, wgt, None, npix,
+                                   npix, pix, pix, eps, True, nthreads=nth)
+                v_ref = R.wgridder(True, uvw, freq, dirty, wgt, None, npix,
+                                   npix, pix, pix, eps, True, nthreads=nth)
+                tg = _median_time(lambda: R.wgridder(
+                    False, uvw, freq, vis, wgt, None, npix, npix, pix, pix,
+                    eps, True, nthreads=nth), reps=3)
+                td = _median_time(lambda: R.wgridder(
+                    True, uvw, freq, dirty, wgt, None, npix, npix, pix, pix,
+                    eps, True, nthreads=nth), reps=3)
+                cpu = dict(value=2 * nvis / (tg + td), unit="visibilities/s",
+                           cores=nth, kind="reference",
+                           vis2dirty_s=round(tg, 4),
+                           dirty2vis_s=round(td, 4),
+                           err_vs_ducc0=dict(
+                               vis2dirty=l2rel(out_d.cpu().numpy(), d_ref),
+                               dirty2vis=l2rel(vout_d.cpu().numpy(), v_ref)),
+                           sample="ducc0 vis2dirty + dirty2vis (ms2dirty / "
+                                  "dirty2ms), same inputs, median of 3 after "
+                                  "a warm-up")
+        except Exception as e:
+            cpu = dict(error=str(e))
+    line = dict(
+        metric="visibilities/s, w-gridding vis2dirty + dirty2vis",
+        value=round(2 * nvis * steps / tt, 1), unit="visibilities/s",
+        n_gpus=1, steps=steps, warmup=max(1, args.warmup // 2),
+        ms_per_step=round(1e3 * tt / steps, 4), higher_is_better=True,
+        scaling="weak", vs_baseline=None, dtype="fp32", data="synthetic",
+        config=dict(workload=f"ducc0.wgridder vis2dirty + dirty2vis, "
+                             f"w-gridding, {nrow} rows x {nchan} channels, "
+                             f"{npix}x{npix} pixels, 5 deg field, eps 1e-4, "
+                             f"complex64"),
+        cpu_baseline=cpu,
+        phases_ms={k: round(v[0], 4) for k, v in prof.items()})
+    print(json.dumps(line))
+
+
 def bench_c2c(args):
     """ducc0.fft.c2c drop-in: in-place forward c2c of a 4096 x 4096
     complex64 array (the headline of this mode, HBM-bound: each transformed
@@ -378,10 +470,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="nufft",
-                    choices=["nufft", "c2c", "sht"],
+                    choices=["nufft", "c2c", "sht", "wgridder"],
                     help="nufft: the 2-D type-1 + type-2 workload (default); "
                          "c2c: the ducc0.fft.c2c drop-in; sht: spherical "
-                         "harmonic synthesis at arbitrary points")
+                         "harmonic synthesis at arbitrary points; wgridder: "
+                         "radio-interferometric gridding + degridding")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--M", type=int, default=10_000_000)
@@ -414,6 +507,9 @@ def main():
         return
     if args.workload == "sht":
         bench_sht(args)
+        return
+    if args.workload == "wgridder":
+        bench_wgridder(args)
         return
     from ducc_amd import _lib, nufft
     from ducc_amd import chooser as learned

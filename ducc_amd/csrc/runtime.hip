@@ -647,8 +647,11 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
   // temporaries live as long as the plan, so nothing here waits for the
   // stream (the pool may hand freed blocks to other streams)
   size_t ws = std::max(radix_sort_workspace(M), scan_workspace(ntiles_));
+  tmp_keys_.stream = s_;
   tmp_keys_.reset(M * 4);
+  tmp_tiles_.stream = s_;
   tmp_tiles_.reset(ntiles_ * 4 * 5);
+  tmp_work_.stream = s_;
   tmp_work_.reset(ws);
   uint32_t *counts = tmp_tiles_.as<uint32_t>(), *tstart = counts + ntiles_,
            *tend = tstart + ntiles_;
@@ -657,6 +660,7 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
     launch_zero(tstart, ntiles_ * 8, s_);  // tstart and tend, adjacent
     launch_fold_keys(g_, coords, M, tmp_keys_.as<uint32_t>(),
                      radix_sort_hist0(tmp_work_.p, M), s_);
+    idx_.stream = s_;
     idx_.reset(M * 4);
     radix_sort_pairs(tmp_keys_.as<uint32_t>(), M, keybits, idx_.as<uint32_t>(),
                      nullptr, tmp_work_.p, ws, s_, true, tstart, tend);
@@ -666,6 +670,7 @@ void Plan::build_index(const void *coords, bool coord_f32, size_t M,
     // a random gather of 1e7 points costs about as much as the gathers it
     // saves in one spreading plus one interpolation, so only plans keep it
     Phase ps(this, "sorting coords");
+    sorted_coords_.stream = s_;
     sorted_coords_.reset(M * p_.ndim * (coord_f32 ? 4 : 8));
     launch_gather_coords(coords, coord_f32, p_.ndim, idx_.as<uint32_t>(), M,
                          sorted_coords_.p, s_);
@@ -679,7 +684,9 @@ void Plan::finish_index(uint32_t *counts) {
   uint32_t *tstart = counts + ntiles_, *tend = tstart + ntiles_,
            *nsub = tend + ntiles_, *sstart = nsub + ntiles_;
   nsub_max_ = std::min(ntiles_, M_) + (M_ + MAXSUB - 1) / MAXSUB;
+  subs_.stream = s_;
   subs_.reset(nsub_max_ * sizeof(Sub));
+  nsub_total_.stream = s_;
   nsub_total_.reset(4);
   launch_tile_subs(ntiles_, counts, tstart, tend, nsub, sstart,
                    subs_.as<Sub>(), nsub_total_.as<uint32_t>(), tmp_work_.p,
@@ -709,12 +716,16 @@ bool Plan::build_records(const void *coords, bool coord_f32, size_t M,
   // the record arrays and temporaries live as long as the plan, so nothing
   // here waits for the stream
   const size_t rb = rec_carve_bytes(M, p_.ndim, p_.fp64, type1);
+  rec_buf_.stream = s_;
   rec_buf_.reset(rb);
+  rec_tmp_.stream = s_;
   rec_tmp_.reset(rb);
   recs_ = rec_carve(rec_buf_.p, M, p_.ndim, p_.fp64, type1);
   const RecBufs tmp = rec_carve(rec_tmp_.p, M, p_.ndim, p_.fp64, type1);
   const size_t ws = std::max(rec_sort_workspace(M), scan_workspace(ntiles_));
+  tmp_tiles_.stream = s_;
   tmp_tiles_.reset(ntiles_ * 4 * 5);
+  tmp_work_.stream = s_;
   tmp_work_.reset(ws);
   uint32_t *counts = tmp_tiles_.as<uint32_t>(), *tstart = counts + ntiles_,
            *tend = tstart + ntiles_;
@@ -762,9 +773,11 @@ void Plan::prepare_grid_side() {
   size_t n = 1;
   for (int d = 0; d < p_.ndim; ++d) n *= p_.nover[d];
   size_t bytes = n * (p_.fp64 ? 16 : 8);
+  grid_.stream = s_;
   if (grid_.bytes != bytes) grid_.reset(bytes);
   // the second buffer serves lines longer than LDS and the out-of-place
   // half of the two-pass column FFT (2-D)
+  scratch_.stream = s_;
   if (needs_scratch_grid(p_) && scratch_.bytes != bytes) scratch_.reset(bytes);
   cur_ = grid_.p;
 }
@@ -935,6 +948,7 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
     a.grid = grid_.p;
     if (p_.deterministic) {
       const size_t sb = spread_det_slab_bytes(a);
+      slab_.stream = s_;
       if (slab_.bytes != sb) slab_.reset(sb);
       a.slab = slab_.p;
       launch_spread_det(p_.fp64, a, s_);  // writes every grid cell

@@ -88,18 +88,23 @@ __device__ __forceinline__ double corfunc(const CorfacArgs &c, double v) {
 }
 
 // per visibility: NUFFT coordinates, w, activity, and (gridding) the value
-// times weight and shift phase; min / max of w over the active ones
-__global__ void k_wg_prep(WgArgs a, WgPoints o) {
-  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// times weight and shift phase; min / max of w over the active ones and
+// their count (grid-stride, reduced per workgroup: one atomic per
+// workgroup and quantity)
+constexpr int PREP_BLOCKS = 1024;
+__global__ __launch_bounds__(256) void k_wg_prep(WgArgs a, WgPoints o) {
+  __shared__ double slo[4], shi[4];
+  __shared__ unsigned long long scnt[4];
   const size_t M = a.nrow * a.nchan;
   double wlo = 1e300, whi = -1e300;
-  bool active = false;
-  if (q < M) {
+  unsigned long long cnt = 0;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < M;
+       q += (size_t)gridDim.x * blockDim.x) {
     const size_t r = q / a.nchan, c = q - r * a.nchan;
     const double f = a.fovc[c];
     const double u = a.uvw[3 * r] * a.ufac * f, v = a.uvw[3 * r + 1] * a.vfac * f,
                  w = a.uvw[3 * r + 2] * a.wfac * f;
-    active = !a.mask || a.mask[r * a.mst0 + c * a.mst1] != 0;
+    bool active = !a.mask || a.mask[r * a.mst0 + c * a.mst1] != 0;
     double wt = 1.0;
     if (a.wgt) {
       wt = a.wgt_f64 ? ((const double *)a.wgt)[r * a.wst0 + c * a.wst1]
@@ -126,14 +131,33 @@ __global__ void k_wg_prep(WgArgs a, WgPoints o) {
     o.active[q] = active ? 1 : 0;
     if (!active) val = make_double2(0.0, 0.0);
     if (a.gridding) ((double2 *)o.val)[q] = val;
-    if (active) wlo = whi = w;
+    if (active) {
+      wlo = fmin(wlo, w);
+      whi = fmax(whi, w);
+      ++cnt;
+    }
   }
   wave_minmax(wlo, whi);
-  const unsigned long long n = __ballot(active);
-  if ((threadIdx.x & 63) == 0 && n) {
-    atomicMin(&o.wminmax[0], ordered_key(wlo));
-    atomicMax(&o.wminmax[1], ordered_key(whi));
-    atomicAdd(&o.wminmax[2], (unsigned long long)__popcll(n));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    slo[wave] = wlo;
+    shi[wave] = whi;
+    scnt[wave] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      wlo = fmin(wlo, slo[k]);
+      whi = fmax(whi, shi[k]);
+      cnt += scnt[k];
+    }
+    if (cnt) {
+      atomicMin(&o.wminmax[0], ordered_key(wlo));
+      atomicMax(&o.wminmax[1], ordered_key(whi));
+      atomicAdd(&o.wminmax[2], cnt);
+    }
   }
 }
 
@@ -434,7 +458,9 @@ void WgJob::prep(const void *vis, bool vis_f64, int64_t vst0, int64_t vst1) {
   a.vst0 = vst0;
   a.vst1 = vst1;
   if (M)
-    hipLaunchKernelGGL(k_wg_prep, dim3(nblk(M, 256)), dim3(256), 0, s, a, o);
+    hipLaunchKernelGGL(k_wg_prep,
+                       dim3(std::min<unsigned>(nblk(M, 256), PREP_BLOCKS)),
+                       dim3(256), 0, s, a, o);
   DMX_HIP(hipGetLastError());
   unsigned long long h[3];
   DMX_HIP(hipMemcpyAsync(h, mm_.p, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -578,6 +604,9 @@ void WgJob::grid(void *dirty, bool dirty_f64, int64_t ds0, int64_t ds1) {
   sort_planes();
   Buf acc(nimg * 16, s), img(nimg * es, s), vals(std::max<size_t>(M, 1) * es, s);
   launch_zero(acc.p, acc.bytes, s);
+  // one plan for all planes: each plane re-sorts its range into the plan's
+  // buffers, which are reused in the order of the one stream
+  Plan pln(p, s);
   for (int pp = 0; pp < pl.nplanes; ++pp) {
     size_t lo, hi;
     plane_range(pp, lo, hi);
@@ -585,22 +614,19 @@ void WgJob::grid(void *dirty, bool dirty_f64, int64_t ds0, int64_t ds1) {
     hipLaunchKernelGGL(k_wg_plane_vals, dim3(nblk(hi - lo, 256)), dim3(256), 0,
                        s, so, (const uint32_t *)skeys_.p, lo, hi, pl, pp,
                        vals.p, p.fp64 ? 1 : 0);
-    Plan pln(p, s);
     pln.build_index(so.coord + 2 * lo, false, hi - lo, false);
     pln.nu2u(false, vals.p, p.fp64, img.p, p.fp64, ust);
     hipLaunchKernelGGL(k_wg_accum_image, g2, dim3(256), 0, s, a,
                        (const void *)img.p, p.fp64 ? 1 : 0,
                        pl.wmin + pp * dw, acc.as<double2>());
     DMX_HIP(hipGetLastError());
-    // the plan's buffers go back to the pool ordered on s
-    DMX_HIP(hipStreamSynchronize(s));
-    pln.mark_done();
   }
   hipLaunchKernelGGL(k_wg_finish_dirty, g2, dim3(256), 0, s, a,
                      (const void *)acc.p, 1, 1, cw, dw, dirty,
                      dirty_f64 ? 1 : 0, ds0, ds1);
   DMX_HIP(hipGetLastError());
   DMX_HIP(hipStreamSynchronize(s));
+  pln.mark_done();
 }
 
 void WgJob::degrid(const void *dirty, bool dirty_f64, int64_t ds0,
@@ -632,6 +658,7 @@ void WgJob::degrid(const void *dirty, bool dirty_f64, int64_t ds0,
   sort_planes();
   Buf acc(std::max<size_t>(M, 1) * 16, s), res(std::max<size_t>(M, 1) * es, s);
   launch_zero(acc.p, acc.bytes, s);
+  Plan pln(p, s);  // one plan for all planes, as in grid()
   for (int pp = 0; pp < pl.nplanes; ++pp) {
     size_t lo, hi;
     plane_range(pp, lo, hi);
@@ -639,7 +666,6 @@ void WgJob::degrid(const void *dirty, bool dirty_f64, int64_t ds0,
     hipLaunchKernelGGL(k_wg_plane_image, g2, dim3(256), 0, s, a,
                        (const double *)imgd.p, 1, pl.wmin + pp * dw, img.p,
                        p.fp64 ? 1 : 0);
-    Plan pln(p, s);
     pln.build_index(so.coord + 2 * lo, false, hi - lo, false);
     pln.u2nu(true, img.p, p.fp64, ust, res.p, p.fp64);
     hipLaunchKernelGGL(k_wg_plane_accum_vis, dim3(nblk(hi - lo, 256)),
@@ -647,14 +673,13 @@ void WgJob::degrid(const void *dirty, bool dirty_f64, int64_t ds0,
                        pl, pp, (const void *)res.p, p.fp64 ? 1 : 0,
                        acc.as<double2>());
     DMX_HIP(hipGetLastError());
-    DMX_HIP(hipStreamSynchronize(s));
-    pln.mark_done();
   }
   hipLaunchKernelGGL(k_wg_finish_vis, dim3(nblk(M, 256)), dim3(256), 0, s, a,
                      o, (const void *)acc.p, 1, (const uint32_t *)idx_.p, M,
                      vis, vis_f64 ? 1 : 0, vs0, vs1);
   DMX_HIP(hipGetLastError());
   DMX_HIP(hipStreamSynchronize(s));
+  pln.mark_done();
 }
 
 void WgJob::zero_dirty(void *dirty, bool dirty_f64, int64_t ds0,

@@ -43,7 +43,7 @@ def lib():
         L.ref_plan_destroy.argtypes = [P]
         L.ref_plan_destroy.restype = None
         L.ref_sht_general.argtypes = [I, I, P, S, P, S, P, S, ct.c_ssize_t, P,
-                                      S, D, D, D, S]
+                                      S, D, D, D, S, S, I]
         L.ref_wgridder.argtypes = [I, I, P, S, P, S, P, P, P, P, S, S, D, D,
                                    D, I, I, I, I, I, D, D, D, D, S]
         _lib = L
@@ -176,39 +176,52 @@ def _mstart(lmax, mstart):
     return np.ascontiguousarray(mstart, np.uintp)
 
 
+_MODES = {"STANDARD": 0, "GRAD_ONLY": 1, "DERIV1": 2}
+
+
 def sht_synthesis_general(alm, lmax, loc, epsilon, mstart=None, lstride=1,
-                          sigma_min=1.1, sigma_max=2.6, nthreads=1):
-    """ducc0.sht.synthesis_general (sht.cc:2926-2960), spin 0: alm (nalm,)
-    complex64 / complex128 -> map (npoints,) float32 / float64"""
+                          sigma_min=1.1, sigma_max=2.6, nthreads=1, spin=0,
+                          mode="STANDARD"):
+    """ducc0.sht.synthesis_general (sht.cc:2926-2960): alm (ncomp, nalm) or
+    (nalm,) complex64 / complex128 -> map (nmaps, npoints) float32 /
+    float64 (nmaps = 1 for spin 0, else 2; spin 0 returns (npoints,))"""
     a = np.ascontiguousarray(alm)
     single = a.dtype == np.complex64
     if not single:
         a = a.astype(np.complex128, copy=False)
+    a2 = a.reshape(-1, a.shape[-1])
     ms = _mstart(lmax, mstart)
     lc = np.ascontiguousarray(loc, np.float64)
-    out = np.zeros(len(lc), np.float32 if single else np.float64)
-    _check(lib().ref_sht_general(int(single), 0, _p(a), a.size, _p(out),
-                                 lmax, _p(ms), len(ms), lstride, _p(lc),
-                                 len(lc), epsilon, sigma_min, sigma_max,
-                                 nthreads))
-    return out
+    nmp = 1 if spin == 0 else 2
+    out = np.zeros((nmp, len(lc)), np.float32 if single else np.float64)
+    _check(lib().ref_sht_general(int(single), 0, _p(a2), a2.shape[1],
+                                 _p(out), lmax, _p(ms), len(ms), lstride,
+                                 _p(lc), len(lc), epsilon, sigma_min,
+                                 sigma_max, nthreads, spin,
+                                 _MODES[mode]))
+    return out[0] if spin == 0 else out
 
 
 def sht_adjoint_synthesis_general(mapv, lmax, loc, epsilon, nalm,
                                   mstart=None, lstride=1, sigma_min=1.1,
-                                  sigma_max=2.6, nthreads=1):
-    """ducc0.sht.adjoint_synthesis_general (sht.cc:2962-2993), spin 0"""
+                                  sigma_max=2.6, nthreads=1, spin=0,
+                                  mode="STANDARD"):
+    """ducc0.sht.adjoint_synthesis_general (sht.cc:2962-2993): map
+    (nmaps, npoints) or (npoints,) -> alm (ncomp, nalm) (spin 0: (nalm,))"""
     m = np.ascontiguousarray(mapv)
     single = m.dtype == np.float32
     if not single:
         m = m.astype(np.float64, copy=False)
+    m2 = m.reshape(-1, m.shape[-1])
     ms = _mstart(lmax, mstart)
     lc = np.ascontiguousarray(loc, np.float64)
-    out = np.zeros(nalm, np.complex64 if single else np.complex128)
-    _check(lib().ref_sht_general(int(single), 1, _p(out), nalm, _p(m), lmax,
+    na = 1 if (spin == 0 or mode != "STANDARD") else 2
+    out = np.zeros((na, nalm), np.complex64 if single else np.complex128)
+    _check(lib().ref_sht_general(int(single), 1, _p(out), nalm, _p(m2), lmax,
                                  _p(ms), len(ms), lstride, _p(lc), len(lc),
-                                 epsilon, sigma_min, sigma_max, nthreads))
-    return out
+                                 epsilon, sigma_min, sigma_max, nthreads,
+                                 spin, _MODES[mode]))
+    return out[0] if spin == 0 else out
 
 
 def wgridder(adjoint, uvw, freq, vis_or_dirty, wgt, mask, npix_x, npix_y,

@@ -310,42 +310,38 @@ int ref_c2c(int single, int ndim, const int64_t *shape, const void *in,
 }
 
 // ducc0.sht.synthesis_general (adjoint = 0) or adjoint_synthesis_general
-// (adjoint = 1), spin 0, one component: alm (1, nalm) complex, map (1, npt)
-// real, loc (npt, 2) float64; single != 0 -> complex64 / float32
+// (adjoint = 1): alm (nalm_c, nalm) complex, map (nmap, npt) real, loc
+// (npt, 2) float64; mode 0 STANDARD, 1 GRAD_ONLY, 2 DERIV1; nalm_c = 1 for
+// spin 0 or mode GRAD_ONLY / DERIV1, else 2;
+// nmap = 1 for spin 0, else 2; single != 0 -> complex64 / float32
 int ref_sht_general(int single, int adjoint, void *alm, size_t nalm,
                     void *map, size_t lmax, const size_t *mstart, size_t nm,
                     ptrdiff_t lstride, const double *loc, size_t npt,
-                    double eps, double smin, double smax, size_t nthreads) {
+                    double eps, double smin, double smax, size_t nthreads,
+                    size_t spin, int mode) {
   return guard([&] {
     cmav<size_t, 1> ms(mstart, {nm});
     cmav<double, 2> lc(loc, {npt, 2});
-    if (single) {
-      using T = float;
+    const auto md = mode == 1 ? GRAD_ONLY : (mode == 2 ? DERIV1 : STANDARD);
+    const size_t na = (spin == 0 || mode != 0) ? 1 : 2,
+                 nmp = spin == 0 ? 1 : 2;
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
       if (adjoint)
         adjoint_synthesis_general(
-            vmav<complex<T>, 2>((complex<T> *)alm, {1, nalm}),
-            cmav<T, 2>((const T *)map, {1, npt}), 0, lmax, ms, lstride, lc,
-            eps, smin, smax, nthreads, STANDARD, false);
+            vmav<complex<T>, 2>((complex<T> *)alm, {na, nalm}),
+            cmav<T, 2>((const T *)map, {nmp, npt}), spin, lmax, ms, lstride,
+            lc, eps, smin, smax, nthreads, md, false);
       else
         synthesis_general(cmav<complex<T>, 2>((const complex<T> *)alm,
-                                              {1, nalm}),
-                          vmav<T, 2>((T *)map, {1, npt}), 0, lmax, ms,
-                          lstride, lc, eps, smin, smax, nthreads, STANDARD,
-                          false);
-    } else {
-      using T = double;
-      if (adjoint)
-        adjoint_synthesis_general(
-            vmav<complex<T>, 2>((complex<T> *)alm, {1, nalm}),
-            cmav<T, 2>((const T *)map, {1, npt}), 0, lmax, ms, lstride, lc,
-            eps, smin, smax, nthreads, STANDARD, false);
-      else
-        synthesis_general(cmav<complex<T>, 2>((const complex<T> *)alm,
-                                              {1, nalm}),
-                          vmav<T, 2>((T *)map, {1, npt}), 0, lmax, ms,
-                          lstride, lc, eps, smin, smax, nthreads, STANDARD,
-                          false);
-    }
+                                              {na, nalm}),
+                          vmav<T, 2>((T *)map, {nmp, npt}), spin, lmax, ms,
+                          lstride, lc, eps, smin, smax, nthreads, md, false);
+    };
+    if (single)
+      run(float(0));
+    else
+      run(double(0));
   });
 }
 

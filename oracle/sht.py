@@ -120,3 +120,108 @@ def random_alm(lmax, mmax, rng):
     res = rng.uniform(-1., 1., n) + 1j * rng.uniform(-1., 1., n)
     res[0:lmax + 1].imag = 0.
     return res
+
+
+# ---------------------------------------------------------------- spin > 0
+# Spin-s fields (sht.cc:803-1010 computes them by its own recursion): with
+# the spin-weighted harmonics sY_lm = lambda^s_lm(theta) e^{i m phi}
+# (Goldberg's convention),
+#   Q + i U = - sum_{l >= s, m = -l..l} (E_lm + i B_lm) sY_lm,
+#   E_{l,-m} = (-1)^m conj(E_lm), B likewise,
+# which is what ducc0 computes for spin s in STANDARD mode (GRAD_ONLY: B = 0;
+# checked against ducc0 itself in tests/test_sht_oracle.py).
+# lambda^s_lm for m >= 0 by the three-term recursion in l of the Wigner
+# functions d^l_{m, m'} with m' = -s, lambda = (-1)^s sqrt((2l+1)/(4 pi))
+# d^l_{m,-s}, started at l0 = max(m, |s|) from the closed forms of
+# d^{l0}_{l0, m'} and its symmetries.
+
+def _log_binom_sqrt(j, a):
+    """log sqrt((2j)! / ((j+a)! (j-a)!))"""
+    from math import lgamma
+    return 0.5 * (lgamma(2 * j + 1) - lgamma(j + a + 1) - lgamma(j - a + 1))
+
+
+def lambda_spin(lmax, m, s, theta):
+    """lambda^s_lm(theta), rows l = 0..lmax (zero below max(m, |s|))"""
+    th = np.asarray(theta, np.float64)
+    mp = -s
+    x = np.cos(th)
+    c, sn = np.cos(th / 2), np.sin(th / 2)
+    l0 = max(m, abs(mp))
+    out = np.zeros((lmax + 1, len(th)))
+    if l0 > lmax:
+        return out
+    with np.errstate(under="ignore", divide="ignore"):
+        lc, ls = np.log(c), np.log(sn)
+        if m >= abs(mp):
+            sgn = (-1.0) ** (m - mp)
+            lg = _log_binom_sqrt(m, mp) + (m + mp) * lc + (m - mp) * ls
+        elif mp > 0:
+            sgn = 1.0
+            lg = _log_binom_sqrt(l0, m) + (l0 + m) * lc + (l0 - m) * ls
+        else:
+            sgn = (-1.0) ** (l0 + m)
+            lg = _log_binom_sqrt(l0, m) + (l0 - m) * lc + (l0 + m) * ls
+        d0 = sgn * np.exp(lg)
+    N = lambda l: np.sqrt((2 * l + 1) / (4 * np.pi))
+    R = lambda l: np.sqrt(max(l * l - m * m, 0) * max(l * l - mp * mp, 0))
+    lam = (-1.0) ** s * N(l0) * d0
+    lamp = np.zeros_like(lam)
+    for l in range(l0, lmax + 1):
+        out[l] = lam
+        if l == lmax:
+            break
+        r1 = R(l + 1)
+        a = N(l + 1) / N(l) * (2 * l + 1) * (l + 1) / r1
+        b = -N(l + 1) / N(l) * (2 * l + 1) * m * mp / (l * r1)
+        cc = N(l + 1) / N(l - 1) * (l + 1) * R(l) / (l * r1)
+        lamp, lam = lam, (a * x + b) * lam - cc * lamp
+    return out
+
+
+def synthesis_general_spin(alm, lmax, loc, spin, mstart=None, lstride=1):
+    """(Q, U) (2, npoints) from alm (2, nalm) = (E, B), or (1, nalm) = E
+    alone (GRAD_ONLY)"""
+    alm = np.atleast_2d(np.asarray(alm))
+    ms = default_mstart(lmax) if mstart is None else np.asarray(mstart)
+    th, ph = loc[:, 0], loc[:, 1]
+    P = np.zeros(len(th), np.complex128)
+    for m in range(len(ms)):
+        ls = np.arange(lmax + 1)
+        E = alm[0, ms[m] + ls * lstride]
+        B = alm[1, ms[m] + ls * lstride] if alm.shape[0] > 1 else 0 * E
+        lp = lambda_spin(lmax, m, spin, th)
+        a = E + 1j * B
+        P -= (a @ lp) * np.exp(1j * m * ph)
+        if m > 0:
+            lmn = (-1.0) ** (spin + m) * lambda_spin(lmax, m, -spin, th)
+            an = (-1.0) ** m * (np.conj(E) + 1j * np.conj(B))
+            P -= (an @ lmn) * np.exp(-1j * m * ph)
+    return np.stack([P.real, P.imag])
+
+
+def adjoint_synthesis_general_spin(mapv, lmax, loc, spin, ncomp=2,
+                                   mstart=None, lstride=1, size=None):
+    """the adjoint of synthesis_general_spin with respect to the real inner
+    products of (Q, U) and of the a_lm (m > 0 twice): (ncomp, nalm)"""
+    mapv = np.asarray(mapv, np.float64)
+    r = mapv[0] + 1j * mapv[1]
+    ms = default_mstart(lmax) if mstart is None else np.asarray(mstart)
+    th, ph = loc[:, 0], loc[:, 1]
+    n = size
+    if n is None:
+        n = max(int(ms[m]) + lmax * lstride for m in range(len(ms))) + 1
+    out = np.zeros((ncomp, n), np.complex128)
+    for m in range(len(ms)):
+        ls = np.arange(m, lmax + 1)
+        lp = lambda_spin(lmax, m, spin, th)[m:]
+        lm = lambda_spin(lmax, m, -spin, th)[m:]
+        hp = lp @ (r * np.exp(-1j * m * ph))
+        hm = lm @ (r * np.exp(1j * m * ph))
+        sg = (-1.0) ** spin
+        bE = (-hp - sg * np.conj(hm)) / 2
+        bB = 1j * (hp - sg * np.conj(hm)) / 2
+        out[0, ms[m] + ls * lstride] = bE
+        if ncomp > 1:
+            out[1, ms[m] + ls * lstride] = bB
+    return out
