@@ -847,13 +847,14 @@ struct ShtJob {
   Params p;
   size_t M = 0;
   bool vf64 = true;
-  Buf mfac, mstart, accofs;
+  int spin = 0;
+  Buf mfac, mstart, accofs, sstart;
   size_t nacc = 0;
   ShtArgs args{};
-  ShtJob(const dmx_array &alm, const dmx_array &map, size_t spin,
+  ShtJob(const dmx_array &alm, const dmx_array &map, size_t spin_,
          size_t lmax, const int64_t *mstart_h, size_t nm, int64_t lstride,
          const dmx_array &loc, double epsilon, double smin, double smax,
-         const dmx_options &opt) {
+         int mode, const dmx_options &opt) {
     DMX_ASSERT(loc.ndim == 2 && loc.shape[1] == 2,
                "last dimension of loc must have size 2");
     DMX_ASSERT(loc.typecode == DMX_F64, "unsupported combination of data types");
@@ -861,12 +862,20 @@ struct ShtJob {
     vf64 = alm.typecode == DMX_C128;
     DMX_ASSERT(map.typecode == (vf64 ? DMX_F64 : DMX_F32),
                "unsupported combination of data types");
-    DMX_ASSERT(spin == 0, "not yet supported: spin > 0");
+    DMX_ASSERT(mode >= 0 && mode <= 2, "unknown SHT mode");
+    // sht.cc:2931-2935, 1857-1868: GRAD_ONLY / DERIV1 need spin > 0 (DERIV1
+    // is the gradient of a spin-0 field: spin 1)
+    if (mode != 0) DMX_ASSERT(spin_ > 0, "DERIV and GRAD_ONLY modes require spin>0");
+    if (mode == 2) DMX_ASSERT(spin_ == 1, "DERIV1 mode requires spin 1");
+    DMX_ASSERT(spin_ < (1u << 20), "spin too large");
+    spin = (int)spin_;
     DMX_ASSERT(nm > 0 && mstart_h, "need at least m=0");
     DMX_ASSERT(nm <= lmax + 1, "mmax>lmax");
-    DMX_ASSERT(alm.ndim == 2 && alm.shape[0] == 1,
+    const int64_t nalm_c = spin == 0 ? 1 : (mode == 0 ? 2 : 1),
+                  nmaps = spin == 0 ? 1 : 2;
+    DMX_ASSERT(alm.ndim == 2 && alm.shape[0] == nalm_c,
                "number of components mismatch in alm");
-    DMX_ASSERT(map.ndim == 2 && map.shape[0] == 1,
+    DMX_ASSERT(map.ndim == 2 && map.shape[0] == nmaps,
                "number of components mismatch in map");
     M = (size_t)loc.shape[0];
     DMX_ASSERT(map.shape[1] == (int64_t)M, "array shape mismatch");
@@ -897,6 +906,14 @@ struct ShtJob {
     DMX_HIP(hipMemcpy(mfac.p, mf.data(), nm * 8, hipMemcpyHostToDevice));
     DMX_HIP(hipMemcpy(mstart.p, mstart_h, nm * 8, hipMemcpyHostToDevice));
     DMX_HIP(hipMemcpy(accofs.p, ao.data(), nm * 8, hipMemcpyHostToDevice));
+    if (spin > 0) {
+      std::vector<SpinStart> ss;
+      sht_spin_start(nm - 1, spin, ss);
+      sstart.reset(nm * sizeof(SpinStart));
+      DMX_HIP(hipMemcpy(sstart.p, ss.data(), nm * sizeof(SpinStart),
+                        hipMemcpyHostToDevice));
+      args.sstart = sstart.as<SpinStart>();
+    }
     args.alm_f64 = vf64 ? 1 : 0;
     args.mstart = mstart.as<int64_t>();
     args.lstride = lstride;
@@ -906,6 +923,10 @@ struct ShtJob {
     args.nphi = geo.nphi;
     args.mfac = mfac.as<double>();
     args.accofs = accofs.as<int64_t>();
+    args.spin = spin;
+    args.nalm_c = (int)nalm_c;
+    args.deriv1 = mode == 2 ? 1 : 0;
+    args.nacc = (int64_t)nacc;
   }
   size_t rows() const { return (size_t)(2 * geo.N - 2); }
   // the doubled-sphere grid's c2c over its rows (theta), scaled 1/(2N-2)
@@ -935,24 +956,28 @@ int dmx_sht_synthesis_general(const dmx_array *alm, dmx_array *map,
   (void)nthreads;
   return guard([&] {
     DMX_ASSERT(alm && map && loc, "null argument");
-    DMX_ASSERT(mode == 0, "not yet supported: mode other than STANDARD");
     reap(false);
     dmx_options opt = opt_in ? *opt_in : defaults();
     ShtJob job(*alm, *map, spin, lmax, mstart, nm, lstride, *loc, epsilon,
-               sigma_min, sigma_max, opt);
+               sigma_min, sigma_max, mode, opt);
     hipStream_t s = pick_stream(opt);
     DevIn ain(*alm, s, false), lin(*loc, s, false);
     DevOut mo(*map, s, true);
     job.args.alm = ain.p;
+    job.args.acomp = ain.st[0];
     Buf G(job.rows() * job.geo.nphi * 16, s);
-    launch_alm2leg(job.args, G.as<double2>(), s);
+    if (job.spin > 0)
+      launch_alm2leg_spin(job.args, G.as<double2>(), s);
+    else
+      launch_alm2leg(job.args, G.as<double2>(), s);
     auto keep = job.theta_fft(G.p, true, s);
     Buf z(job.M * 16, s);
     if (job.M) {
       auto pl = job.plan(lin.p, s);
       const int64_t ust[2] = {job.geo.nphi, 1};
       pl->u2nu(false, G.p, true, ust, z.p, true);
-      launch_real_part(z.as<double2>(), job.M, mo.p, job.vf64, mo.st[1], s);
+      launch_real_part(z.as<double2>(), job.M, mo.p, job.vf64, mo.st[1],
+                       job.spin > 0 ? mo.st[0] : 0, s);
       if (verbosity > 0) pl->report_params(false);
       mo.finish(s);
       pl->mark_done();
@@ -973,22 +998,23 @@ int dmx_sht_adjoint_synthesis_general(dmx_array *alm, const dmx_array *map,
   (void)nthreads;
   return guard([&] {
     DMX_ASSERT(alm && map && loc, "null argument");
-    DMX_ASSERT(mode == 0, "not yet supported: mode other than STANDARD");
     reap(false);
     dmx_options opt = opt_in ? *opt_in : defaults();
     ShtJob job(*alm, *map, spin, lmax, mstart, nm, lstride, *loc, epsilon,
-               sigma_min, sigma_max, opt);
+               sigma_min, sigma_max, mode, opt);
     hipStream_t s = pick_stream(opt);
     DevIn min_(*map, s, true), lin(*loc, s, false);
     DevOut ao(*alm, s, false);
+    job.args.acomp = ao.st[0];
     // entries outside the (l, m) layout of a host array come back zero
     if (!alm->device) launch_zero(ao.p, (size_t)nelem(*alm) * elsize(alm->typecode), s);
     Buf Gt(job.rows() * job.geo.nphi * 16, s);
     Buf z(std::max<size_t>(job.M, 1) * 16, s);
     std::unique_ptr<Plan> pl;
     if (job.M) {
-      launch_to_complex(min_.p, job.vf64, min_.st[1], job.M, z.as<double2>(),
-                        s);
+      launch_to_complex(min_.p, job.vf64, min_.st[1],
+                        job.spin > 0 ? min_.st[0] : 0, job.M,
+                        z.as<double2>(), s);
       pl = job.plan(lin.p, s);
       const int64_t ust[2] = {job.geo.nphi, 1};
       pl->nu2u(true, z.p, true, Gt.p, true, ust);
@@ -997,10 +1023,16 @@ int dmx_sht_adjoint_synthesis_general(dmx_array *alm, const dmx_array *map,
       launch_zero(Gt.p, Gt.bytes, s);
     }
     auto keep = job.theta_fft(Gt.p, false, s);
-    Buf acc(job.nacc * 16, s);
+    const size_t nacc_c = job.spin > 0 ? 2 : 1;
+    Buf acc(job.nacc * 16 * nacc_c, s);
     if (leg2alm_needs_zero(job.args)) launch_zero(acc.p, acc.bytes, s);
-    launch_leg2alm(job.args, Gt.as<double2>(), acc.as<double2>(), s);
-    launch_store_alm(acc.as<double2>(), job.args, ao.p, job.vf64, s);
+    if (job.spin > 0) {
+      launch_leg2alm_spin(job.args, Gt.as<double2>(), acc.as<double2>(), s);
+      launch_store_alm_spin(acc.as<double2>(), job.args, ao.p, job.vf64, s);
+    } else {
+      launch_leg2alm(job.args, Gt.as<double2>(), acc.as<double2>(), s);
+      launch_store_alm(acc.as<double2>(), job.args, ao.p, job.vf64, s);
+    }
     ao.finish(s);
     if (pl) pl->mark_done();
   });

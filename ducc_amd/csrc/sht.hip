@@ -35,6 +35,7 @@
 // m near the poles; the recursion then runs on values scaled by 2^(512 k)
 // (k < 0) until they reach the normal range, and adds nothing before
 // (role of the scaled recursion of sht.cc:77-82).  All in FP64.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -296,20 +297,28 @@ __global__ __launch_bounds__(SHT_THREADS) void k_leg2alm(ShtArgs g,
   }
 }
 
-// complex results -> real map (real part), strided
+// complex results -> real map: the real parts, and with cstride != 0 the
+// imaginary parts as the second component; strided
 template <typename TO>
 __global__ void k_real_part(const double2 *z, size_t M, TO *map,
-                            int64_t mstride) {
+                            int64_t mstride, int64_t cstride) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < M) map[(int64_t)i * mstride] = (TO)z[i].x;
+  if (i >= M) return;
+  const double2 v = z[i];
+  map[(int64_t)i * mstride] = (TO)v.x;
+  if (cstride) map[cstride + (int64_t)i * mstride] = (TO)v.y;
 }
 
-// real map -> complex values (imaginary part zero)
+// real map -> complex values: the second component (cstride != 0) as the
+// imaginary part, else zero
 template <typename TI>
-__global__ void k_to_complex(const TI *map, int64_t mstride, size_t M,
-                             double2 *z) {
+__global__ void k_to_complex(const TI *map, int64_t mstride, int64_t cstride,
+                             size_t M, double2 *z) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < M) z[i] = make_double2((double)map[(int64_t)i * mstride], 0.0);
+  if (i >= M) return;
+  const double re = (double)map[(int64_t)i * mstride];
+  const double im = cstride ? (double)map[cstride + (int64_t)i * mstride] : 0.0;
+  z[i] = make_double2(re, im);
 }
 
 // accumulated a_lm -> output array (m = 0: the real part, as the real
@@ -326,7 +335,421 @@ __global__ void k_store_alm(const double2 *acc, ShtArgs g, TO *alm) {
   alm[2 * ix + 1] = (TO)v.y;
 }
 
+
+// ------------------------------------------------------------- spin > 0
+// Spin-s fields Q + i U = -sum_{l, |m| <= l} (E_lm + i B_lm) sY_lm with
+// sY_lm = lambda^s_lm(theta) e^{i m phi} and E_{l,-m} = (-1)^m conj(E_lm)
+// (B likewise): the convention of sht.cc's spin transforms (its get_norm,
+// sht.cc:85-101, and the E / B combinations of its inner loops).  Per m >= 0
+// the two columns of the field on ring theta are
+//   G_m(theta)  = -sum_l a_l lambda^{+s}_lm(theta),      a_l = E + i B,
+//   G_-m(theta) = -(-1)^s sum_l abar_l lambda^{-s}_lm(theta),
+//                                     abar_l = conj(E) + i conj(B)   (m > 0),
+// using lambda^s_{l,-m} = (-1)^(s+m) lambda^{-s}_lm.  lambda^{+-s}_lm =
+// (-1)^s sqrt((2l+1)/(4 pi)) d^l_{m, -+s}(theta) follows the three-term
+// recursion of the Wigner functions in l,
+//   lambda_{l+1} = (A_l cos theta +- B_l) lambda_l - C_l lambda_{l-1},
+//   R_l = sqrt((l^2 - m^2)(l^2 - s^2)),
+//   A_l = sqrt((2l+3)/(2l+1)) (2l+1)(l+1) / R_{l+1},
+//   B_l = sqrt((2l+3)/(2l+1)) (2l+1) m s / (l R_{l+1}),
+//   C_l = sqrt((2l+3)/(2l-1)) (l+1) R_l / (l R_{l+1}),
+// from the closed form at l0 = max(m, s) (SpinStart), both chains in the
+// scaled representation of lambda_mm above.  The ring pair (theta, pi -
+// theta) shares the chains: lambda^{+-s}_lm(pi - theta) = (-1)^(l+m)
+// lambda^{-+s}_lm(theta).  On the doubled sphere row 2N-2-j holds (-1)^(m+s)
+// times row j (d^l_{m,m'}(-theta) = (-1)^(m-m') d^l_{m,m'}(theta)).
+constexpr int SP_CH = 256;  // l values per LDS chunk
+constexpr int SP_R = 4;     // ring pairs per thread
+
+struct SpinChunk {
+  double2 a[SP_CH];   // E + i B (synthesis)
+  double2 ab[SP_CH];  // conj(E) + i conj(B)
+  double A[SP_CH], B[SP_CH], C[SP_CH];
+};
+
+// base^n as pm * 2^pe, pm in [0.5, 1) (or 0); base in [0, 1]
+__device__ __forceinline__ void pow_me(double base, int n, double &pm,
+                                       int &pe) {
+  pm = 1.0;
+  pe = 0;
+  if (n == 0) return;
+  if (base == 0.0) {
+    pm = 0.0;
+    return;
+  }
+  int be;
+  double b = frexp(base, &be);
+  unsigned e = (unsigned)n;
+  while (e) {
+    if (e & 1u) {
+      int t;
+      pm = frexp(pm * b, &t);
+      pe += be + t;
+    }
+    e >>= 1;
+    if (e) {
+      int t;
+      b = frexp(b * b, &t);
+      be = 2 * be + t;
+    }
+  }
+}
+
+// sgn * mant * 2^e * c^na * s^nb as v * 2^(512 k)
+__device__ __forceinline__ void spin_start(double mant, int e, int sgn,
+                                           double c, int na, double s, int nb,
+                                           double &v, int &k) {
+  double pc, ps;
+  int ec, es;
+  pow_me(c, na, pc, ec);
+  pow_me(s, nb, ps, es);
+  int t;
+  const double pm = frexp(mant * pc * ps, &t);
+  if (pm == 0.0) {
+    v = 0.0;
+    k = 0;
+    return;
+  }
+  const int E = e + ec + es + t;
+  k = E >= 0 ? 0 : -((-E) / SHT_SB);
+  v = ldexp(sgn < 0 ? -pm : pm, E - SHT_SB * k);
+}
+
+__device__ __forceinline__ void spin_step(double &lam, double &lamp, int &k,
+                                          double axb, double C) {
+  const double nl = fma(axb, lam, -C * lamp);
+  lamp = lam;
+  lam = nl;
+  if (k < 0 && fabs(nl) > SHT_BIG) {
+    lam *= SHT_DOWN;
+    lamp *= SHT_DOWN;
+    ++k;
+  }
+}
+
+// recursion coefficients of chunk [lb, le) and, for the synthesis, the
+// (weighted) a_lm; entries up to the next multiple of 8 zero
+__device__ void load_spin_chunk(SpinChunk &c, int lb, int le, int m,
+                                const ShtArgs &g, bool with_alm) {
+  const int s = g.spin;
+  const int l0 = max(m, s);
+  const int n = le - lb, npad = (n + 7) & ~7;
+  for (int i = threadIdx.x; i < npad; i += blockDim.x) {
+    double A = 0.0, B = 0.0, C = 0.0;
+    double2 a = make_double2(0.0, 0.0), ab = a;
+    if (i < n) {
+      const int l = lb + i;
+      const double L = l, M2 = (double)m * m, S2 = (double)s * s;
+      const double L1 = L + 1.0;
+      const double r1 = sqrt((L1 * L1 - M2) * (L1 * L1 - S2));
+      const double n1 = sqrt((2.0 * L + 3.0) / (2.0 * L + 1.0));
+      A = n1 * (2.0 * L + 1.0) * L1 / r1;
+      B = n1 * (2.0 * L + 1.0) * (double)m * (double)s / (L * r1);
+      if (l > l0) {
+        const double rl = sqrt((L * L - M2) * (L * L - S2));
+        C = sqrt((2.0 * L + 3.0) / (2.0 * L - 1.0)) * L1 * rl / (L * r1);
+      }
+      if (with_alm) {
+        const int64_t ix = g.mstart[m] + (int64_t)l * g.lstride;
+        double2 e, b = make_double2(0.0, 0.0);
+        if (g.alm_f64) {
+          const double2 *p = (const double2 *)g.alm;
+          e = p[ix];
+          if (g.nalm_c > 1) b = p[g.acomp + ix];
+        } else {
+          const float2 *p = (const float2 *)g.alm;
+          const float2 ef = p[ix];
+          e = make_double2(ef.x, ef.y);
+          if (g.nalm_c > 1) {
+            const float2 bf = p[g.acomp + ix];
+            b = make_double2(bf.x, bf.y);
+          }
+        }
+        if (g.deriv1) {
+          const double w = sqrt(L * L1);
+          e.x *= w;
+          e.y *= w;
+        }
+        a = make_double2(e.x - b.y, e.y + b.x);
+        ab = make_double2(e.x + b.y, b.x - e.y);
+      }
+    }
+    c.A[i] = A;
+    c.B[i] = B;
+    c.C[i] = C;
+    c.a[i] = a;
+    c.ab[i] = ab;
+  }
+}
+
+// ring j: cos theta, cos(theta/2), sin(theta/2), theta = j pi/(N-1)
+__device__ __forceinline__ void ring_half(int j, int N, double &x, double &c,
+                                          double &sn) {
+  double s1, c1;
+  sincospi((double)j / (double)(N - 1), &s1, &c1);
+  x = c1;
+  sincospi((double)j / (double)(2 * (N - 1)), &sn, &c);
+}
+
+__global__ __launch_bounds__(SHT_THREADS) void k_alm2leg_spin(ShtArgs g,
+                                                              double2 *G) {
+  __shared__ SpinChunk ch;
+  const int m = blockIdx.y, s = g.spin;
+  const int N = g.N, nphi = g.nphi, npairs = (N + 1) / 2;
+  const int q0 = (blockIdx.x * blockDim.x + threadIdx.x) * SP_R;
+  const SpinStart ss = g.sstart[m];
+  double x[SP_R], lp[SP_R], lpp[SP_R], lm[SP_R], lmp[SP_R];
+  int kp[SP_R], km[SP_R];
+  double2 nA[SP_R], sA[SP_R], nB[SP_R], sB[SP_R];
+#pragma unroll
+  for (int r = 0; r < SP_R; ++r) {
+    const int j = min(q0 + r, npairs - 1);
+    double c, sn;
+    ring_half(j, N, x[r], c, sn);
+    spin_start(ss.mant, ss.exp, ss.sgnp, c, ss.a, sn, ss.b, lp[r], kp[r]);
+    spin_start(ss.mant, ss.exp, ss.sgnm, c, ss.b, sn, ss.a, lm[r], km[r]);
+    lpp[r] = lmp[r] = 0.0;
+    nA[r] = sA[r] = nB[r] = sB[r] = make_double2(0.0, 0.0);
+  }
+  for (int lb = max(m, s); lb <= g.lmax; lb += SP_CH) {
+    const int le = min(lb + SP_CH, g.lmax + 1);
+    __syncthreads();
+    load_spin_chunk(ch, lb, le, m, g, true);
+    __syncthreads();
+    for (int i = 0; i < le - lb; ++i) {
+      const double2 a = ch.a[i], ab = ch.ab[i];
+      const double A = ch.A[i], B = ch.B[i], C = ch.C[i];
+      const bool odd = ((lb + i + m) & 1) != 0;
+#pragma unroll
+      for (int r = 0; r < SP_R; ++r) {
+        const double wp = kp[r] == 0 ? lp[r] : 0.0;
+        const double wm = km[r] == 0 ? lm[r] : 0.0;
+        const double wps = odd ? -wp : wp, wms = odd ? -wm : wm;
+        nA[r].x = fma(a.x, wp, nA[r].x);
+        nA[r].y = fma(a.y, wp, nA[r].y);
+        sA[r].x = fma(a.x, wms, sA[r].x);
+        sA[r].y = fma(a.y, wms, sA[r].y);
+        nB[r].x = fma(ab.x, wm, nB[r].x);
+        nB[r].y = fma(ab.y, wm, nB[r].y);
+        sB[r].x = fma(ab.x, wps, sB[r].x);
+        sB[r].y = fma(ab.y, wps, sB[r].y);
+        const double ax = A * x[r];
+        spin_step(lp[r], lpp[r], kp[r], ax + B, C);
+        spin_step(lm[r], lmp[r], km[r], ax - B, C);
+      }
+    }
+  }
+  const double sgs = (s & 1) ? 1.0 : -1.0;      // -(-1)^s
+  const double sgd = ((m + s) & 1) ? -1.0 : 1.0;  // doubled rows
+#pragma unroll
+  for (int r = 0; r < SP_R; ++r) {
+    const int j = q0 + r;
+    if (j >= npairs) break;
+    const int jm = N - 1 - j;
+    auto put = [&](int row, int col, double2 v) {
+      G[(size_t)row * nphi + col] = v;
+      if (row > 0 && row < N - 1)
+        G[(size_t)(2 * N - 2 - row) * nphi + col] =
+            make_double2(sgd * v.x, sgd * v.y);
+    };
+    put(j, m, make_double2(-nA[r].x, -nA[r].y));
+    if (jm != j) put(jm, m, make_double2(-sA[r].x, -sA[r].y));
+    if (m > 0) {
+      put(j, nphi - m, make_double2(sgs * nB[r].x, sgs * nB[r].y));
+      if (jm != j) put(jm, nphi - m, make_double2(sgs * sB[r].x, sgs * sB[r].y));
+    }
+  }
+}
+
+// Sum over the 64 lanes of eight values per lane: on return lane L holds,
+// in out, the total of v[idx] with idx = ((L >> 5) & 1) * 4 + ((L >> 4) & 1)
+// * 2 + ((L >> 3) & 1), in every lane of its group of 8.  Three halving
+// exchanges (32, 16, 8) then a butterfly over 4, 2, 1: 10 shuffles of NV
+// doubles for 8 sums instead of 48.
+template <int NV>
+__device__ __forceinline__ void reduce8(double (&v)[8][NV], double (&out)[NV],
+                                        int lane) {
+  const bool h1 = (lane >> 5) & 1, h2 = (lane >> 4) & 1, h3 = (lane >> 3) & 1;
+  double u[4][NV], w[2][NV];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const double keep = h1 ? v[i + 4][c] : v[i][c];
+      const double send = h1 ? v[i][c] : v[i + 4][c];
+      u[i][c] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const double keep = h2 ? u[i + 2][c] : u[i][c];
+      const double send = h2 ? u[i][c] : u[i + 2][c];
+      w[i][c] = keep + __shfl_xor(send, 16);
+    }
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    const double keep = h3 ? w[1][c] : w[0][c];
+    const double send = h3 ? w[0][c] : w[1][c];
+    double z = keep + __shfl_xor(send, 8);
+    z += __shfl_xor(z, 4);
+    z += __shfl_xor(z, 2);
+    z += __shfl_xor(z, 1);
+    out[c] = z;
+  }
+}
+
+// adjoint: H+_l = sum_rings g_m lambda^{+s}_l, H-_l = sum_rings g_-m
+// lambda^{-s}_l (g the doubled rows folded back, sign (-1)^(m+s)), into
+// acc[accofs[m] + l - m] and acc[nacc + accofs[m] + l - m]
+__global__ __launch_bounds__(SHT_THREADS) void k_leg2alm_spin(
+    ShtArgs g, const double2 *Gt, double2 *acc) {
+  __shared__ SpinChunk ch;
+  __shared__ double4 red[SHT_THREADS / 64][SP_CH];
+  const int m = blockIdx.y, s = g.spin;
+  const int N = g.N, nphi = g.nphi, npairs = (N + 1) / 2;
+  const int q0 = (blockIdx.x * blockDim.x + threadIdx.x) * SP_R;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const double sgd = ((m + s) & 1) ? -1.0 : 1.0;
+  const int colm = m, coln = m ? nphi - m : 0;
+  const SpinStart ss = g.sstart[m];
+  double x[SP_R], lp[SP_R], lpp[SP_R], lm[SP_R], lmp[SP_R];
+  int kp[SP_R], km[SP_R];
+  double2 nm_[SP_R], sm_[SP_R], nn_[SP_R], sn_[SP_R];
+  auto gval = [&](int j, int col) {
+    double2 v = Gt[(size_t)j * nphi + col];
+    if (j > 0 && j < N - 1) {
+      const double2 w = Gt[(size_t)(2 * N - 2 - j) * nphi + col];
+      v.x += sgd * w.x;
+      v.y += sgd * w.y;
+    }
+    return v;
+  };
+#pragma unroll
+  for (int r = 0; r < SP_R; ++r) {
+    const int j = q0 + r;
+    double c, sn;
+    ring_half(min(j, npairs - 1), N, x[r], c, sn);
+    spin_start(ss.mant, ss.exp, ss.sgnp, c, ss.a, sn, ss.b, lp[r], kp[r]);
+    spin_start(ss.mant, ss.exp, ss.sgnm, c, ss.b, sn, ss.a, lm[r], km[r]);
+    lpp[r] = lmp[r] = 0.0;
+    const double2 z = make_double2(0.0, 0.0);
+    nm_[r] = sm_[r] = nn_[r] = sn_[r] = z;
+    if (j < npairs) {
+      const int jm = N - 1 - j;
+      nm_[r] = gval(j, colm);
+      nn_[r] = gval(j, coln);
+      if (jm != j) {
+        sm_[r] = gval(jm, colm);
+        sn_[r] = gval(jm, coln);
+      }
+    }
+  }
+  for (int lb = max(m, s); lb <= g.lmax; lb += SP_CH) {
+    const int le = min(lb + SP_CH, g.lmax + 1);
+    __syncthreads();
+    load_spin_chunk(ch, lb, le, m, g, false);
+    __syncthreads();
+    for (int i0 = 0; i0 < le - lb; i0 += 8) {
+      double v[8][4];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u;
+        const double A = ch.A[i], B = ch.B[i], C = ch.C[i];
+        const bool odd = ((lb + i + m) & 1) != 0;
+        double hpx = 0.0, hpy = 0.0, hmx = 0.0, hmy = 0.0;
+#pragma unroll
+        for (int r = 0; r < SP_R; ++r) {
+          const double wp = kp[r] == 0 ? lp[r] : 0.0;
+          const double wm = km[r] == 0 ? lm[r] : 0.0;
+          const double wps = odd ? -wp : wp, wms = odd ? -wm : wm;
+          hpx = fma(wp, nm_[r].x, fma(wms, sm_[r].x, hpx));
+          hpy = fma(wp, nm_[r].y, fma(wms, sm_[r].y, hpy));
+          hmx = fma(wm, nn_[r].x, fma(wps, sn_[r].x, hmx));
+          hmy = fma(wm, nn_[r].y, fma(wps, sn_[r].y, hmy));
+          const double ax = A * x[r];
+          spin_step(lp[r], lpp[r], kp[r], ax + B, C);
+          spin_step(lm[r], lmp[r], km[r], ax - B, C);
+        }
+        v[u][0] = hpx;
+        v[u][1] = hpy;
+        v[u][2] = hmx;
+        v[u][3] = hmy;
+      }
+      double o[4];
+      reduce8<4>(v, o, lane);
+      if ((lane & 7) == 0) {
+        const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 +
+                        ((lane >> 3) & 1);
+        red[wave][i0 + idx] = make_double4(o[0], o[1], o[2], o[3]);
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < le - lb; i += blockDim.x) {
+      double4 t = red[0][i];
+      for (int w = 1; w < nwaves; ++w) {
+        const double4 q = red[w][i];
+        t.x += q.x;
+        t.y += q.y;
+        t.z += q.z;
+        t.w += q.w;
+      }
+      double2 *dp = acc + g.accofs[m] + (lb + i - m);
+      double2 *dm = dp + g.nacc;
+      if (gridDim.x == 1) {
+        *dp = make_double2(t.x, t.y);
+        *dm = make_double2(t.z, t.w);
+      } else {
+        atomicAdd(&dp->x, t.x);
+        atomicAdd(&dp->y, t.y);
+        atomicAdd(&dm->x, t.z);
+        atomicAdd(&dm->y, t.w);
+      }
+    }
+  }
+}
+
+// (H+, H-) -> E = -(H+ + (-1)^s conj(H-)) / 2, B = i (H+ - (-1)^s conj(H-))
+// / 2 (zero for l < max(m, s); m = 0 real; deriv1: E weighted by
+// sqrt(l (l + 1)))
+template <typename TO>
+__global__ void k_store_alm_spin(const double2 *acc, ShtArgs g, TO *alm) {
+  const int m = blockIdx.y, s = g.spin;
+  const int l = m + blockIdx.x * blockDim.x + threadIdx.x;
+  if (l > g.lmax) return;
+  double2 e = make_double2(0.0, 0.0), b = e;
+  if (l >= max(m, s)) {
+    const size_t o = g.accofs[m] + (l - m);
+    const double2 hp = acc[o], hm = acc[g.nacc + o];
+    const double sg = (s & 1) ? -1.0 : 1.0;
+    const double cx = sg * hm.x, cy = -sg * hm.y;  // (-1)^s conj(H-)
+    e = make_double2(-0.5 * (hp.x + cx), -0.5 * (hp.y + cy));
+    b = make_double2(-0.5 * (hp.y - cy), 0.5 * (hp.x - cx));
+    if (g.deriv1) {
+      const double w = sqrt((double)l * (l + 1.0));
+      e.x *= w;
+      e.y *= w;
+    }
+    if (m == 0) e.y = b.y = 0.0;
+  }
+  const int64_t ix = g.mstart[m] + (int64_t)l * g.lstride;
+  alm[2 * ix] = (TO)e.x;
+  alm[2 * ix + 1] = (TO)e.y;
+  if (g.nalm_c > 1) {
+    alm[2 * (g.acomp + ix)] = (TO)b.x;
+    alm[2 * (g.acomp + ix) + 1] = (TO)b.y;
+  }
+}
+
 unsigned nblk(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+// threads per block of the spin kernels: the ring pairs over SP_R per
+// thread, whole waves, at most SHT_THREADS
+unsigned spin_threads(int npairs) {
+  const unsigned t = nblk((size_t)npairs, SP_R);
+  return std::min<unsigned>(SHT_THREADS, (t + 63) / 64 * 64);
+}
 
 }  // namespace
 
@@ -371,30 +794,92 @@ void launch_leg2alm(const ShtArgs &g, const double2 *Gt, double2 *acc,
 
 bool leg2alm_needs_zero(const ShtArgs &g) {
   const int npairs = (g.N + 1) / 2;
+  if (g.spin > 0) return nblk((size_t)npairs, spin_threads(npairs) * SP_R) > 1;
   return nblk((size_t)npairs, SHT_THREADS * SHT_R) > 1;
 }
 
+void sht_spin_start(size_t mmax, int spin, std::vector<SpinStart> &out) {
+  out.resize(mmax + 1);
+  for (size_t mm = 0; mm <= mmax; ++mm) {
+    const int m = (int)mm, s = spin;
+    const int l0 = std::max(m, s), lo = std::min(m, s);
+    // binomial(2 l0, l0 - lo) as bm * 2^e, in long double
+    long double bm = 1.0L;
+    long e = 0;
+    const int n = 2 * l0, k = l0 - lo;
+    for (int i = 1; i <= k; ++i) {
+      int t;
+      bm = std::frexp(bm * (long double)(n - k + i) / (long double)i, &t);
+      e += t;
+    }
+    if (e & 1) {
+      bm *= 2.0L;
+      e -= 1;
+    }
+    bm = std::sqrt(bm) * std::sqrt((2.0L * l0 + 1.0L) / (4.0L * (long double)M_PI));
+    int t;
+    SpinStart st;
+    st.mant = std::frexp((double)bm, &t);
+    st.exp = (int)(e / 2) + t;
+    st.a = std::abs(m - s);
+    st.b = m + s;
+    st.sgnp = (m & 1) ? -1 : 1;
+    st.sgnm = (m >= s) ? st.sgnp : ((s & 1) ? -1 : 1);
+    st.pad = 0;
+    out[mm] = st;
+  }
+}
+
+void launch_alm2leg_spin(const ShtArgs &g, double2 *G, hipStream_t s) {
+  const int npairs = (g.N + 1) / 2;
+  const unsigned th = spin_threads(npairs);
+  const dim3 grid(nblk((size_t)npairs, th * SP_R), (unsigned)(g.mmax + 1));
+  hipLaunchKernelGGL(k_alm2leg_spin, grid, dim3(th), 0, s, g, G);
+  DMX_HIP(hipGetLastError());
+}
+
+void launch_leg2alm_spin(const ShtArgs &g, const double2 *Gt, double2 *acc,
+                         hipStream_t s) {
+  const int npairs = (g.N + 1) / 2;
+  const unsigned th = spin_threads(npairs);
+  const dim3 grid(nblk((size_t)npairs, th * SP_R), (unsigned)(g.mmax + 1));
+  hipLaunchKernelGGL(k_leg2alm_spin, grid, dim3(th), 0, s, g, Gt, acc);
+  DMX_HIP(hipGetLastError());
+}
+
+void launch_store_alm_spin(const double2 *acc, const ShtArgs &g, void *alm,
+                           bool alm_f64, hipStream_t s) {
+  const dim3 grid(nblk((size_t)g.lmax + 1, 256), (unsigned)(g.mmax + 1));
+  if (alm_f64)
+    hipLaunchKernelGGL(k_store_alm_spin<double>, grid, dim3(256), 0, s, acc,
+                       g, (double *)alm);
+  else
+    hipLaunchKernelGGL(k_store_alm_spin<float>, grid, dim3(256), 0, s, acc, g,
+                       (float *)alm);
+  DMX_HIP(hipGetLastError());
+}
+
 void launch_real_part(const double2 *z, size_t M, void *map, bool map_f64,
-                      int64_t mstride, hipStream_t s) {
+                      int64_t mstride, int64_t cstride, hipStream_t s) {
   if (!M) return;
   if (map_f64)
     hipLaunchKernelGGL(k_real_part<double>, dim3(nblk(M, 256)), dim3(256), 0,
-                       s, z, M, (double *)map, mstride);
+                       s, z, M, (double *)map, mstride, cstride);
   else
     hipLaunchKernelGGL(k_real_part<float>, dim3(nblk(M, 256)), dim3(256), 0,
-                       s, z, M, (float *)map, mstride);
+                       s, z, M, (float *)map, mstride, cstride);
   DMX_HIP(hipGetLastError());
 }
 
 void launch_to_complex(const void *map, bool map_f64, int64_t mstride,
-                       size_t M, double2 *z, hipStream_t s) {
+                       int64_t cstride, size_t M, double2 *z, hipStream_t s) {
   if (!M) return;
   if (map_f64)
     hipLaunchKernelGGL(k_to_complex<double>, dim3(nblk(M, 256)), dim3(256), 0,
-                       s, (const double *)map, mstride, M, z);
+                       s, (const double *)map, mstride, cstride, M, z);
   else
     hipLaunchKernelGGL(k_to_complex<float>, dim3(nblk(M, 256)), dim3(256), 0,
-                       s, (const float *)map, mstride, M, z);
+                       s, (const float *)map, mstride, cstride, M, z);
   DMX_HIP(hipGetLastError());
 }
 

@@ -1,12 +1,14 @@
-"""ducc0.sht general synthesis on the GPU (spin 0): the Python surface of
+"""ducc0.sht general synthesis on the GPU: the Python surface of
 ducc0.sht.synthesis_general / adjoint_synthesis_general
 (python/sht_pymod.cc:805-870, 2299-2300), keyword-only, with ducc0's
 defaults.
 
 Arrays may be numpy (copied to and from the device) or torch tensors on the
 GPU (used in place, ordered on torch's current stream).  a_lm are complex
-(1, nalm), the map real (1, npoints) of the same precision, loc (npoints, 2)
-float64 (theta, phi)."""
+(ncomp, nalm), the map real (nmaps, npoints) of the same precision, loc
+(npoints, 2) float64 (theta, phi).  nmaps is 1 for spin 0, else 2 (Q, U);
+ncomp is 1 for spin 0 and in the modes GRAD_ONLY (E alone) and DERIV1 (the
+gradient of a spin-0 field, spin 1), else 2 (E, B)."""
 import ctypes as ct
 
 import numpy as np
@@ -47,10 +49,20 @@ def _min_almdim(lmax, ms, lstride):
     return res + 1
 
 
+_MODES = {"STANDARD": 0, "GRAD_ONLY": 1, "DERIV1": 2}
+
+
 def _mode(mode):
-    if mode != "STANDARD":
-        raise RuntimeError("not yet supported: mode other than STANDARD")
-    return 0
+    """sht_pymod.cc:50-56"""
+    if mode not in _MODES:
+        raise RuntimeError("unknown SHT mode")
+    return _MODES[mode]
+
+
+def _ncomp(spin, mode):
+    """(a_lm components, map components) (sht.cc:2931-2935)"""
+    nalm = 1 if (spin == 0 or mode != "STANDARD") else 2
+    return nalm, (1 if spin == 0 else 2)
 
 
 def _stream_opts(*arrays):
@@ -60,21 +72,17 @@ def _stream_opts(*arrays):
     return _lib.options()
 
 
-def _check_spin(spin):
-    if spin != 0:
-        raise RuntimeError("not yet supported: spin > 0")
-
-
 def synthesis_general(*, alm, spin, lmax, loc, epsilon, mstart=None,
                       lstride=1, mmax=None, nthreads=1, map=None,
                       sigma_min=1.1, sigma_max=2.6, mode="STANDARD",
                       verbose=False):
-    """Values of the real field sum_{l,m} a_lm Y_lm at the points
-    loc[i] = (theta_i, phi_i) (sht.cc:2926-2960).  Returns `map` (the same
-    object when given), shape (1, npoints)."""
+    """Values of the real field sum_{l,m} a_lm Y_lm (spin 0), or of the spin
+    field Q + i U = -sum_{l,m} (E_lm + i B_lm) sY_lm, at the points loc[i] =
+    (theta_i, phi_i) (sht.cc:2926-2960).  Returns `map` (the same object
+    when given), shape (nmaps, npoints)."""
     _lib.require_gpu()
-    _check_spin(spin)
     mo = _mode(mode)
+    _, nmaps = _ncomp(spin, mode)
     ms = _mstart(lmax, mmax, mstart)
     _min_almdim(lmax, ms, lstride)
     npt = loc.shape[0]
@@ -83,11 +91,11 @@ def synthesis_general(*, alm, spin, lmax, loc, epsilon, mstart=None,
             import torch
             rdt = torch.float64 if alm.dtype == torch.complex128 \
                 else torch.float32
-            map = torch.empty((1, npt), dtype=rdt, device=alm.device)
+            map = torch.empty((nmaps, npt), dtype=rdt, device=alm.device)
         else:
             a = np.asarray(alm)
             rdt = np.float64 if a.dtype == np.complex128 else np.float32
-            map = np.empty((1, npt), dtype=rdt)
+            map = np.empty((nmaps, npt), dtype=rdt)
     a, ka = _lib.describe(alm)
     m_, km = _lib.describe(map)
     lc, kl = _lib.describe(loc)
@@ -105,11 +113,11 @@ def adjoint_synthesis_general(*, map, spin, lmax, loc, epsilon, mstart=None,
                               sigma_min=1.1, sigma_max=2.6,
                               mode="STANDARD", verbose=False):
     """The adjoint of synthesis_general (sht.cc:2962-2993): a_lm from the
-    values `map` (1, npoints) at loc.  Returns `alm` (the same object when
-    given), shape (1, nalm)."""
+    values `map` (nmaps, npoints) at loc.  Returns `alm` (the same object
+    when given), shape (ncomp, nalm)."""
     _lib.require_gpu()
-    _check_spin(spin)
     mo = _mode(mode)
+    ncomp, _ = _ncomp(spin, mode)
     ms = _mstart(lmax, mmax, mstart)
     nalm = _min_almdim(lmax, ms, lstride)
     if alm is None:
@@ -117,11 +125,11 @@ def adjoint_synthesis_general(*, map, spin, lmax, loc, epsilon, mstart=None,
             import torch
             cdt = torch.complex128 if map.dtype == torch.float64 \
                 else torch.complex64
-            alm = torch.zeros((1, nalm), dtype=cdt, device=map.device)
+            alm = torch.zeros((ncomp, nalm), dtype=cdt, device=map.device)
         else:
             cdt = np.complex128 if np.asarray(map).dtype == np.float64 \
                 else np.complex64
-            alm = np.zeros((1, nalm), dtype=cdt)
+            alm = np.zeros((ncomp, nalm), dtype=cdt)
     a, ka = _lib.describe(alm)
     m_, km = _lib.describe(map)
     lc, kl = _lib.describe(loc)

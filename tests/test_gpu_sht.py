@@ -167,7 +167,7 @@ def test_errors(gpu):
     rng = np.random.default_rng(1)
     a = S.random_alm(4, 4, rng)[None, :]
     loc = _loc(rng, 10)
-    with pytest.raises(RuntimeError, match="not yet supported"):
+    with pytest.raises(RuntimeError, match="number of components"):
         sht.synthesis_general(alm=a, spin=1, lmax=4, loc=loc, epsilon=1e-5)
     with pytest.raises(RuntimeError, match="loc"):
         sht.synthesis_general(alm=a, spin=0, lmax=4, loc=loc[:, :1].copy(),
