@@ -182,32 +182,38 @@ def fibonacci_lattice(npoints):
 
 
 def bench_sht(args):
-    """ducc0.sht.synthesis_general + adjoint_synthesis_general, spin 0, on
-    the reference demo's sky (python/demos/sht_general.py:29-41: lmax = mmax
-    = 1187, 2,000,000 points of a Fibonacci lattice, random a_lm), complex128
-    a_lm, eps 1e-7.  One step is one synthesis plus one adjoint with inputs
+    """ducc0.sht.synthesis_general + adjoint_synthesis_general on the
+    reference demo's sky (python/demos/sht_general.py:29-41: lmax = mmax =
+    1187, 2,000,000 points of a Fibonacci lattice, random a_lm), complex128
+    a_lm, eps 1e-7; spin 0 by default, --spin S for a spin-S field (E and B,
+    mode STANDARD).  One step is one synthesis plus one adjoint with inputs
     on the device; ducc0 itself (oracle/_ref) on all host cores is the CPU
     baseline (median of 3 after a warm-up)."""
     import torch
     from ducc_amd import _lib, sht
     from oracle import sht as S
     dev = torch.device("cuda:0")
-    lmax, npt, eps = 1187, 2_000_000, 1e-7
+    lmax, npt, eps, spin = 1187, 2_000_000, 1e-7, args.spin
+    nc = 1 if spin == 0 else 2
     rng = np.random.default_rng(42)
-    a_np = S.random_alm(lmax, lmax, rng)
+    a_np = np.stack([S.random_alm(lmax, lmax, rng) for _ in range(nc)])
+    ofs = 0
+    for s_ in range(spin):  # l < spin: zero (test_sht.py:31-40)
+        a_np[:, ofs:ofs + spin - s_] = 0
+        ofs += lmax + 1 - s_
     loc_np = fibonacci_lattice(npt)
-    r_np = rng.uniform(-1, 1, npt)
-    a = torch.from_numpy(a_np[None, :]).to(dev)
+    r_np = rng.uniform(-1, 1, (nc, npt))
+    a = torch.from_numpy(a_np).to(dev)
     loc = torch.from_numpy(loc_np).to(dev)
-    r = torch.from_numpy(r_np[None, :]).to(dev)
-    mp = torch.empty((1, npt), dtype=torch.float64, device=dev)
+    r = torch.from_numpy(r_np).to(dev)
+    mp = torch.empty((nc, npt), dtype=torch.float64, device=dev)
     at = torch.empty_like(a)
     sync = torch.cuda.synchronize
 
     def step():
-        sht.synthesis_general(alm=a, spin=0, lmax=lmax, loc=loc, epsilon=eps,
-                              map=mp)
-        sht.adjoint_synthesis_general(map=r, spin=0, lmax=lmax, loc=loc,
+        sht.synthesis_general(alm=a, spin=spin, lmax=lmax, loc=loc,
+                              epsilon=eps, map=mp)
+        sht.adjoint_synthesis_general(map=r, spin=spin, lmax=lmax, loc=loc,
                                       epsilon=eps, alm=at)
     steps = max(1, args.steps // 2)
     t = time_steps(step, steps, max(1, args.warmup // 2), sync)
@@ -223,20 +229,24 @@ def bench_sht(args):
             from oracle import ref as R
             if R.available():
                 nth = os.cpu_count() or 1
+                na = a_np.shape[1]
                 f_ref = R.sht_synthesis_general(a_np, lmax, loc_np, eps,
-                                                nthreads=nth)
+                                                nthreads=nth, spin=spin)
                 b_ref = R.sht_adjoint_synthesis_general(
-                    r_np, lmax, loc_np, eps, len(a_np), nthreads=nth)
+                    r_np, lmax, loc_np, eps, na, nthreads=nth, spin=spin)
                 ts = _median_time(lambda: R.sht_synthesis_general(
-                    a_np, lmax, loc_np, eps, nthreads=nth), reps=3)
+                    a_np, lmax, loc_np, eps, nthreads=nth, spin=spin), reps=3)
                 ta = _median_time(lambda: R.sht_adjoint_synthesis_general(
-                    r_np, lmax, loc_np, eps, len(a_np), nthreads=nth), reps=3)
+                    r_np, lmax, loc_np, eps, na, nthreads=nth, spin=spin),
+                    reps=3)
                 cpu = dict(value=npt * 2 / (ts + ta), unit="points/s",
                            cores=nth, kind="reference",
                            synthesis_s=round(ts, 4), adjoint_s=round(ta, 4),
                            err_vs_ducc0=dict(
-                               synthesis=l2rel(mp.cpu().numpy()[0], f_ref),
-                               adjoint=l2rel(at.cpu().numpy()[0], b_ref)),
+                               synthesis=l2rel(mp.cpu().numpy().ravel(),
+                                               np.ravel(f_ref)),
+                               adjoint=l2rel(at.cpu().numpy().ravel(),
+                                             np.ravel(b_ref))),
                            sample="ducc0 synthesis_general + "
                                   "adjoint_synthesis_general, same inputs, "
                                   "median of 3 after a warm-up")
@@ -244,15 +254,15 @@ def bench_sht(args):
             cpu = dict(error=str(e))
     line = dict(
         metric="points/s, spherical harmonic synthesis + adjoint at "
-               "arbitrary points (spin 0)",
+               f"arbitrary points (spin {spin})",
         value=round(2 * npt * steps / t, 1), unit="points/s", n_gpus=1,
         steps=steps, warmup=max(1, args.warmup // 2),
         ms_per_step=round(1e3 * t / steps, 4), higher_is_better=True,
         scaling="weak", vs_baseline=None, dtype="fp64", data="synthetic",
         config=dict(workload="ducc0.sht.synthesis_general + "
-                             "adjoint_synthesis_general, spin 0, lmax = mmax "
-                             "= 1187, 2,000,000 Fibonacci-lattice points, "
-                             "eps 1e-7, complex128 a_lm"),
+                             f"adjoint_synthesis_general, spin {spin}, lmax "
+                             "= mmax = 1187, 2,000,000 Fibonacci-lattice "
+                             "points, eps 1e-7, complex128 a_lm"),
         cpu_baseline=cpu,
         phases_ms={k: round(v[0], 4) for k, v in prof.items()})
     print(json.dumps(line))
@@ -475,6 +485,9 @@ def main():
                          "c2c: the ducc0.fft.c2c drop-in; sht: spherical "
                          "harmonic synthesis at arbitrary points; wgridder: "
                          "radio-interferometric gridding + degridding")
+    ap.add_argument("--spin", type=int, default=0,
+                    help="sht workload: spin of the field (0, or > 0 for "
+                         "E and B)")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--M", type=int, default=10_000_000)

@@ -90,6 +90,85 @@ __device__ __forceinline__ void lambda_mm(double st, int m, double mf,
   v = ldexp(((m & 1) ? -mf : mf) * pm, pe - SHT_SB * k);
 }
 
+// Sum over the 64 lanes of eight values per lane: on return lane L holds,
+// in out, the total of v[idx] with idx = ((L >> 5) & 1) * 4 + ((L >> 4) & 1)
+// * 2 + ((L >> 3) & 1), in every lane of its group of 8.  Three halving
+// exchanges (32, 16, 8) then a butterfly over 4, 2, 1: 10 shuffles of NV
+// doubles for 8 sums instead of 48.
+template <int NV>
+__device__ __forceinline__ void reduce8(double (&v)[8][NV], double (&out)[NV],
+                                        int lane) {
+  const bool h1 = (lane >> 5) & 1, h2 = (lane >> 4) & 1, h3 = (lane >> 3) & 1;
+  double u[4][NV], w[2][NV];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const double keep = h1 ? v[i + 4][c] : v[i][c];
+      const double send = h1 ? v[i][c] : v[i + 4][c];
+      u[i][c] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const double keep = h2 ? u[i + 2][c] : u[i][c];
+      const double send = h2 ? u[i][c] : u[i + 2][c];
+      w[i][c] = keep + __shfl_xor(send, 16);
+    }
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    const double keep = h3 ? w[1][c] : w[0][c];
+    const double send = h3 ? w[0][c] : w[1][c];
+    double z = keep + __shfl_xor(send, 8);
+    z += __shfl_xor(z, 4);
+    z += __shfl_xor(z, 2);
+    z += __shfl_xor(z, 1);
+    out[c] = z;
+  }
+}
+
+// The same for four values per lane: lane L holds the total of v[idx],
+// idx = ((L >> 5) & 1) * 2 + ((L >> 4) & 1), in every lane of its group of
+// 16; 7 shuffles of NV doubles for 4 sums instead of 24.
+template <int NV>
+__device__ __forceinline__ void reduce4(double (&v)[4][NV], double (&out)[NV],
+                                        int lane) {
+  const bool h1 = (lane >> 5) & 1, h2 = (lane >> 4) & 1;
+  double u[2][NV];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const double keep = h1 ? v[i + 2][c] : v[i][c];
+      const double send = h1 ? v[i][c] : v[i + 2][c];
+      u[i][c] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    const double keep = h2 ? u[1][c] : u[0][c];
+    const double send = h2 ? u[0][c] : u[1][c];
+    double z = keep + __shfl_xor(send, 16);
+    z += __shfl_xor(z, 8);
+    z += __shfl_xor(z, 4);
+    z += __shfl_xor(z, 2);
+    z += __shfl_xor(z, 1);
+    out[c] = z;
+  }
+}
+
+// The scaled recursions are rescaled once per eight steps: one step
+// multiplies max(|lambda_l|, |lambda_{l-1}|) by at most 1.5 / eps_{l+1} <
+// 1.5 sqrt(2 l + 3) < 2^12 for l < 2^20, so eight steps stay below 2^96 of
+// growth, far inside the 2^768 left above SHT_BIG
+__device__ __forceinline__ void rescale8(double &lam, double &lamp, int &k) {
+  if (k < 0 && fabs(lam) > SHT_BIG) {
+    lam *= SHT_DOWN;
+    lamp *= SHT_DOWN;
+    ++k;
+  }
+}
+
 // LDS chunk of one m: a_lm (complex FP64) and the recursion coefficients
 // A_l = 1 / eps_{l+1}, B_l = eps_l / eps_{l+1}
 struct ShtChunk {
@@ -105,7 +184,13 @@ __device__ __forceinline__ double eps_lm(int l, int m) {
 
 __device__ void load_chunk(ShtChunk &c, int l0, int l1, int m,
                            const ShtArgs &g, bool with_alm) {
-  for (int i = threadIdx.x; i < l1 - l0; i += blockDim.x) {
+  // entries up to the next multiple of 8 zero
+  const int n = l1 - l0, npad = (n + 7) & ~7;
+  for (int i = n + threadIdx.x; i < npad; i += blockDim.x) {
+    c.ab[i] = make_double2(0.0, 0.0);
+    c.a[i] = make_double2(0.0, 0.0);
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int l = l0 + i;
     const double e1 = eps_lm(l + 1, m);
     c.ab[i] = make_double2(1.0 / e1, eps_lm(l, m) / e1);
@@ -134,7 +219,7 @@ __global__ __launch_bounds__(SHT_THREADS) void k_alm2leg(ShtArgs g,
   __shared__ ShtChunk ch;
   const int m = blockIdx.y;
   const int N = g.N, nphi = g.nphi, npairs = (N + 1) / 2;
-  const int q0 = (blockIdx.x * SHT_THREADS + threadIdx.x) * SHT_R;
+  const int q0 = (blockIdx.x * blockDim.x + threadIdx.x) * SHT_R;
   double x[SHT_R], lam[SHT_R], lamp[SHT_R];
   int k[SHT_R];
   double2 se[SHT_R], so[SHT_R];
@@ -153,29 +238,26 @@ __global__ __launch_bounds__(SHT_THREADS) void k_alm2leg(ShtArgs g,
     __syncthreads();
     load_chunk(ch, l0, l1, m, g, true);
     __syncthreads();
-    for (int i = 0; i < l1 - l0; ++i) {
-      const double2 a = ch.a[i];
-      const double2 ab = ch.ab[i];
-      const bool odd = ((l0 + i - m) & 1) != 0;
+    // eight l at a time (zero-padded; l0 - m is even, so u's parity is
+    // that of l - m)
+    for (int i0 = 0; i0 < l1 - l0; i0 += 8) {
+#pragma unroll 2
+      for (int u = 0; u < 8; ++u) {
+        const double2 a = ch.a[i0 + u];
+        const double2 ab = ch.ab[i0 + u];
 #pragma unroll
-      for (int r = 0; r < SHT_R; ++r) {
-        const double w = k[r] == 0 ? lam[r] : 0.0;
-        if (odd) {
-          so[r].x = fma(a.x, w, so[r].x);
-          so[r].y = fma(a.y, w, so[r].y);
-        } else {
-          se[r].x = fma(a.x, w, se[r].x);
-          se[r].y = fma(a.y, w, se[r].y);
-        }
-        const double nl = fma(ab.x * x[r], lam[r], -ab.y * lamp[r]);
-        lamp[r] = lam[r];
-        lam[r] = nl;
-        if (k[r] < 0 && fabs(nl) > SHT_BIG) {
-          lam[r] *= SHT_DOWN;
-          lamp[r] *= SHT_DOWN;
-          ++k[r];
+        for (int r = 0; r < SHT_R; ++r) {
+          const double w = k[r] == 0 ? lam[r] : 0.0;
+          double2 &acc = (u & 1) ? so[r] : se[r];
+          acc.x = fma(a.x, w, acc.x);
+          acc.y = fma(a.y, w, acc.y);
+          const double nl = fma(ab.x * x[r], lam[r], -ab.y * lamp[r]);
+          lamp[r] = lam[r];
+          lam[r] = nl;
         }
       }
+#pragma unroll
+      for (int r = 0; r < SHT_R; ++r) rescale8(lam[r], lamp[r], k[r]);
     }
   }
   const double sg = (m & 1) ? -1.0 : 1.0;
@@ -210,8 +292,9 @@ __global__ __launch_bounds__(SHT_THREADS) void k_leg2alm(ShtArgs g,
   __shared__ double2 red[SHT_THREADS / 64][SHT_CH];
   const int m = blockIdx.y;
   const int N = g.N, nphi = g.nphi, npairs = (N + 1) / 2;
-  const int q0 = (blockIdx.x * SHT_THREADS + threadIdx.x) * SHT_R;
+  const int q0 = (blockIdx.x * blockDim.x + threadIdx.x) * SHT_R;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
   const double sg = (m & 1) ? -1.0 : 1.0;
   double x[SHT_R], lam[SHT_R], lamp[SHT_R];
   int k[SHT_R];
@@ -251,38 +334,41 @@ __global__ __launch_bounds__(SHT_THREADS) void k_leg2alm(ShtArgs g,
     __syncthreads();
     load_chunk(ch, l0, l1, m, g, false);
     __syncthreads();
-    for (int i = 0; i < l1 - l0; ++i) {
-      const double2 ab = ch.ab[i];
-      const bool odd = ((l0 + i - m) & 1) != 0;
-      double sr = 0.0, si = 0.0;
+    // eight l at a time (l0 - m is even: the parities alternate from even),
+    // summed over the wave's rings with reduce8
+    for (int i0 = 0; i0 < l1 - l0; i0 += 8) {
+      double v[8][2];
 #pragma unroll
-      for (int r = 0; r < SHT_R; ++r) {
-        const double w = k[r] == 0 ? lam[r] : 0.0;
-        const double2 v = odd ? go[r] : ge[r];
-        sr = fma(w, v.x, sr);
-        si = fma(w, v.y, si);
-        const double nl = fma(ab.x * x[r], lam[r], -ab.y * lamp[r]);
-        lamp[r] = lam[r];
-        lam[r] = nl;
-        if (k[r] < 0 && fabs(nl) > SHT_BIG) {
-          lam[r] *= SHT_DOWN;
-          lamp[r] *= SHT_DOWN;
-          ++k[r];
+      for (int u = 0; u < 8; ++u) {
+        const double2 ab = ch.ab[i0 + u];
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int r = 0; r < SHT_R; ++r) {
+          const double w = k[r] == 0 ? lam[r] : 0.0;
+          const double2 gv = (u & 1) ? go[r] : ge[r];
+          sr = fma(w, gv.x, sr);
+          si = fma(w, gv.y, si);
+          const double nl = fma(ab.x * x[r], lam[r], -ab.y * lamp[r]);
+          lamp[r] = lam[r];
+          lam[r] = nl;
         }
+        v[u][0] = sr;
+        v[u][1] = si;
       }
-      // sum over the wave's rings
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        sr += __shfl_xor(sr, o);
-        si += __shfl_xor(si, o);
+      for (int r = 0; r < SHT_R; ++r) rescale8(lam[r], lamp[r], k[r]);
+      double o[2];
+      reduce8<2>(v, o, lane);
+      if ((lane & 7) == 0) {
+        const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 +
+                        ((lane >> 3) & 1);
+        red[wave][i0 + idx] = make_double2(o[0], o[1]);
       }
-      if (lane == 0) red[wave][i] = make_double2(sr, si);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < l1 - l0; i += blockDim.x) {
       double2 s = red[0][i];
-#pragma unroll
-      for (int w = 1; w < SHT_THREADS / 64; ++w) {
+      for (int w = 1; w < nwaves; ++w) {
         s.x += red[w][i].x;
         s.y += red[w][i].y;
       }
@@ -415,16 +501,16 @@ __device__ __forceinline__ void spin_start(double mant, int e, int sgn,
   v = ldexp(sgn < 0 ? -pm : pm, E - SHT_SB * k);
 }
 
-__device__ __forceinline__ void spin_step(double &lam, double &lamp, int &k,
+// one step of a spin recursion; rescaled by rescale8 every eight steps: for
+// l >= max(m, s), R_{l+1} >= 2l + 1, so |A_l| <= 1.3 (l + 1), |B_l| <= 1.3 l
+// and |C_l| < 2.6, and one step grows max(|lambda_l|, |lambda_{l-1}|) by
+// less than 2.6 l + 4 < 2^23 for l < 2^20: eight steps stay below 2^184,
+// far inside the 2^768 left above SHT_BIG
+__device__ __forceinline__ void spin_step(double &lam, double &lamp,
                                           double axb, double C) {
   const double nl = fma(axb, lam, -C * lamp);
   lamp = lam;
   lam = nl;
-  if (k < 0 && fabs(nl) > SHT_BIG) {
-    lam *= SHT_DOWN;
-    lamp *= SHT_DOWN;
-    ++k;
-  }
 }
 
 // recursion coefficients of chunk [lb, le) and, for the synthesis, the
@@ -516,26 +602,38 @@ __global__ __launch_bounds__(SHT_THREADS) void k_alm2leg_spin(ShtArgs g,
     __syncthreads();
     load_spin_chunk(ch, lb, le, m, g, true);
     __syncthreads();
-    for (int i = 0; i < le - lb; ++i) {
-      const double2 a = ch.a[i], ab = ch.ab[i];
-      const double A = ch.A[i], B = ch.B[i], C = ch.C[i];
-      const bool odd = ((lb + i + m) & 1) != 0;
+    // eight l at a time (zero-padded); the parity of l + m alternates from
+    // that of lb + m
+    const int par0 = (lb + m) & 1;
+    for (int i0 = 0; i0 < le - lb; i0 += 8) {
+#pragma unroll 2
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u;
+        const double2 a = ch.a[i], ab = ch.ab[i];
+        const double A = ch.A[i], B = ch.B[i], C = ch.C[i];
+        const bool odd = ((u & 1) ^ par0) != 0;
+#pragma unroll
+        for (int r = 0; r < SP_R; ++r) {
+          const double wp = kp[r] == 0 ? lp[r] : 0.0;
+          const double wm = km[r] == 0 ? lm[r] : 0.0;
+          const double wps = odd ? -wp : wp, wms = odd ? -wm : wm;
+          nA[r].x = fma(a.x, wp, nA[r].x);
+          nA[r].y = fma(a.y, wp, nA[r].y);
+          sA[r].x = fma(a.x, wms, sA[r].x);
+          sA[r].y = fma(a.y, wms, sA[r].y);
+          nB[r].x = fma(ab.x, wm, nB[r].x);
+          nB[r].y = fma(ab.y, wm, nB[r].y);
+          sB[r].x = fma(ab.x, wps, sB[r].x);
+          sB[r].y = fma(ab.y, wps, sB[r].y);
+          const double ax = A * x[r];
+          spin_step(lp[r], lpp[r], ax + B, C);
+          spin_step(lm[r], lmp[r], ax - B, C);
+        }
+      }
 #pragma unroll
       for (int r = 0; r < SP_R; ++r) {
-        const double wp = kp[r] == 0 ? lp[r] : 0.0;
-        const double wm = km[r] == 0 ? lm[r] : 0.0;
-        const double wps = odd ? -wp : wp, wms = odd ? -wm : wm;
-        nA[r].x = fma(a.x, wp, nA[r].x);
-        nA[r].y = fma(a.y, wp, nA[r].y);
-        sA[r].x = fma(a.x, wms, sA[r].x);
-        sA[r].y = fma(a.y, wms, sA[r].y);
-        nB[r].x = fma(ab.x, wm, nB[r].x);
-        nB[r].y = fma(ab.y, wm, nB[r].y);
-        sB[r].x = fma(ab.x, wps, sB[r].x);
-        sB[r].y = fma(ab.y, wps, sB[r].y);
-        const double ax = A * x[r];
-        spin_step(lp[r], lpp[r], kp[r], ax + B, C);
-        spin_step(lm[r], lmp[r], km[r], ax - B, C);
+        rescale8(lp[r], lpp[r], kp[r]);
+        rescale8(lm[r], lmp[r], km[r]);
       }
     }
   }
@@ -558,44 +656,6 @@ __global__ __launch_bounds__(SHT_THREADS) void k_alm2leg_spin(ShtArgs g,
       put(j, nphi - m, make_double2(sgs * nB[r].x, sgs * nB[r].y));
       if (jm != j) put(jm, nphi - m, make_double2(sgs * sB[r].x, sgs * sB[r].y));
     }
-  }
-}
-
-// Sum over the 64 lanes of eight values per lane: on return lane L holds,
-// in out, the total of v[idx] with idx = ((L >> 5) & 1) * 4 + ((L >> 4) & 1)
-// * 2 + ((L >> 3) & 1), in every lane of its group of 8.  Three halving
-// exchanges (32, 16, 8) then a butterfly over 4, 2, 1: 10 shuffles of NV
-// doubles for 8 sums instead of 48.
-template <int NV>
-__device__ __forceinline__ void reduce8(double (&v)[8][NV], double (&out)[NV],
-                                        int lane) {
-  const bool h1 = (lane >> 5) & 1, h2 = (lane >> 4) & 1, h3 = (lane >> 3) & 1;
-  double u[4][NV], w[2][NV];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int c = 0; c < NV; ++c) {
-      const double keep = h1 ? v[i + 4][c] : v[i][c];
-      const double send = h1 ? v[i][c] : v[i + 4][c];
-      u[i][c] = keep + __shfl_xor(send, 32);
-    }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int c = 0; c < NV; ++c) {
-      const double keep = h2 ? u[i + 2][c] : u[i][c];
-      const double send = h2 ? u[i][c] : u[i + 2][c];
-      w[i][c] = keep + __shfl_xor(send, 16);
-    }
-#pragma unroll
-  for (int c = 0; c < NV; ++c) {
-    const double keep = h3 ? w[1][c] : w[0][c];
-    const double send = h3 ? w[0][c] : w[1][c];
-    double z = keep + __shfl_xor(send, 8);
-    z += __shfl_xor(z, 4);
-    z += __shfl_xor(z, 2);
-    z += __shfl_xor(z, 1);
-    out[c] = z;
   }
 }
 
@@ -651,13 +711,14 @@ __global__ __launch_bounds__(SHT_THREADS) void k_leg2alm_spin(
     __syncthreads();
     load_spin_chunk(ch, lb, le, m, g, false);
     __syncthreads();
-    for (int i0 = 0; i0 < le - lb; i0 += 8) {
-      double v[8][4];
+    const int par0 = (lb + m) & 1;
+    for (int i0 = 0; i0 < le - lb; i0 += 4) {
+      double v[4][4];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 4; ++u) {
         const int i = i0 + u;
         const double A = ch.A[i], B = ch.B[i], C = ch.C[i];
-        const bool odd = ((lb + i + m) & 1) != 0;
+        const bool odd = ((u & 1) ^ par0) != 0;
         double hpx = 0.0, hpy = 0.0, hmx = 0.0, hmy = 0.0;
 #pragma unroll
         for (int r = 0; r < SP_R; ++r) {
@@ -669,19 +730,25 @@ __global__ __launch_bounds__(SHT_THREADS) void k_leg2alm_spin(
           hmx = fma(wm, nn_[r].x, fma(wps, sn_[r].x, hmx));
           hmy = fma(wm, nn_[r].y, fma(wps, sn_[r].y, hmy));
           const double ax = A * x[r];
-          spin_step(lp[r], lpp[r], kp[r], ax + B, C);
-          spin_step(lm[r], lmp[r], km[r], ax - B, C);
+          spin_step(lp[r], lpp[r], ax + B, C);
+          spin_step(lm[r], lmp[r], ax - B, C);
         }
         v[u][0] = hpx;
         v[u][1] = hpy;
         v[u][2] = hmx;
         v[u][3] = hmy;
       }
+      if (i0 & 4) {  // every eight steps
+#pragma unroll
+        for (int r = 0; r < SP_R; ++r) {
+          rescale8(lp[r], lpp[r], kp[r]);
+          rescale8(lm[r], lmp[r], km[r]);
+        }
+      }
       double o[4];
-      reduce8<4>(v, o, lane);
-      if ((lane & 7) == 0) {
-        const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 +
-                        ((lane >> 3) & 1);
+      reduce4<4>(v, o, lane);
+      if ((lane & 15) == 0) {
+        const int idx = ((lane >> 5) & 1) * 2 + ((lane >> 4) & 1);
         red[wave][i0 + idx] = make_double4(o[0], o[1], o[2], o[3]);
       }
     }
@@ -744,10 +811,10 @@ __global__ void k_store_alm_spin(const double2 *acc, ShtArgs g, TO *alm) {
 
 unsigned nblk(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-// threads per block of the spin kernels: the ring pairs over SP_R per
-// thread, whole waves, at most SHT_THREADS
-unsigned spin_threads(int npairs) {
-  const unsigned t = nblk((size_t)npairs, SP_R);
+// threads per block of the Legendre kernels: the ring pairs over R per
+// thread, in whole waves, at most SHT_THREADS
+unsigned ring_threads(int npairs, int R) {
+  const unsigned t = nblk((size_t)npairs, (unsigned)R);
   return std::min<unsigned>(SHT_THREADS, (t + 63) / 64 * 64);
 }
 
@@ -777,50 +844,74 @@ void sht_mfac(size_t mmax, std::vector<double> &mf) {
 
 void launch_alm2leg(const ShtArgs &g, double2 *G, hipStream_t s) {
   const int npairs = (g.N + 1) / 2;
-  const dim3 grid(nblk((size_t)npairs, SHT_THREADS * SHT_R),
-                  (unsigned)(g.mmax + 1));
-  hipLaunchKernelGGL(k_alm2leg, grid, dim3(SHT_THREADS), 0, s, g, G);
+  const unsigned th = ring_threads(npairs, SHT_R);
+  const dim3 grid(nblk((size_t)npairs, th * SHT_R), (unsigned)(g.mmax + 1));
+  hipLaunchKernelGGL(k_alm2leg, grid, dim3(th), 0, s, g, G);
   DMX_HIP(hipGetLastError());
 }
 
 void launch_leg2alm(const ShtArgs &g, const double2 *Gt, double2 *acc,
                     hipStream_t s) {
   const int npairs = (g.N + 1) / 2;
-  const dim3 grid(nblk((size_t)npairs, SHT_THREADS * SHT_R),
-                  (unsigned)(g.mmax + 1));
-  hipLaunchKernelGGL(k_leg2alm, grid, dim3(SHT_THREADS), 0, s, g, Gt, acc);
+  const unsigned th = ring_threads(npairs, SHT_R);
+  const dim3 grid(nblk((size_t)npairs, th * SHT_R), (unsigned)(g.mmax + 1));
+  hipLaunchKernelGGL(k_leg2alm, grid, dim3(th), 0, s, g, Gt, acc);
   DMX_HIP(hipGetLastError());
 }
 
 bool leg2alm_needs_zero(const ShtArgs &g) {
   const int npairs = (g.N + 1) / 2;
-  if (g.spin > 0) return nblk((size_t)npairs, spin_threads(npairs) * SP_R) > 1;
-  return nblk((size_t)npairs, SHT_THREADS * SHT_R) > 1;
+  const int R = g.spin > 0 ? SP_R : SHT_R;
+  return nblk((size_t)npairs, ring_threads(npairs, R) * R) > 1;
+}
+
+// x * 2^e -> the same with x in [0.5, 1)
+static void normalize_ld(long double &x, long &e) {
+  int t;
+  x = frexpl(x, &t);
+  e += t;
 }
 
 void sht_spin_start(size_t mmax, int spin, std::vector<SpinStart> &out) {
   out.resize(mmax + 1);
+  const int s = spin;
+  // binomial(2 l0, l0 - min(m, s)) as bm * 2^e in long double, by its ratio
+  // from one m to the next: for m < s (l0 = s) binomial(2s, s - m) from
+  // binomial(2s, s) down, for m >= s (l0 = m) binomial(2m, m - s) up from
+  // binomial(2s, 0) = 1
+  long double bm = 1.0L;
+  long e = 0;
+  auto norm = [&] { normalize_ld(bm, e); };
+  for (int i = 1; i <= s; ++i) {  // binomial(2s, s): the entry of m = 0
+    bm *= (long double)(s + i) / (long double)i;
+    norm();
+  }
   for (size_t mm = 0; mm <= mmax; ++mm) {
-    const int m = (int)mm, s = spin;
-    const int l0 = std::max(m, s), lo = std::min(m, s);
-    // binomial(2 l0, l0 - lo) as bm * 2^e, in long double
-    long double bm = 1.0L;
-    long e = 0;
-    const int n = 2 * l0, k = l0 - lo;
-    for (int i = 1; i <= k; ++i) {
-      int t;
-      bm = std::frexp(bm * (long double)(n - k + i) / (long double)i, &t);
-      e += t;
+    const int m = (int)mm;
+    const int l0 = std::max(m, s);
+    if (m > 0) {
+      if (m < s) {  // binomial(2s, s-m) = binomial(2s, s-m+1) (s-m+1)/(s+m)
+        bm *= (long double)(s - m + 1) / (long double)(s + m);
+      } else if (m == s) {  // binomial(2s, 0)
+        bm = 1.0L;
+        e = 0;
+      } else {  // binomial(2m, m-s) from binomial(2m-2, m-1-s)
+        bm *= (long double)(2 * m - 1) * (long double)(2 * m) /
+              ((long double)(m - s) * (long double)(m + s));
+      }
+      norm();
     }
-    if (e & 1) {
-      bm *= 2.0L;
-      e -= 1;
+    long double b = bm;
+    long ee = e;
+    if (ee & 1) {
+      b *= 2.0L;
+      ee -= 1;
     }
-    bm = std::sqrt(bm) * std::sqrt((2.0L * l0 + 1.0L) / (4.0L * (long double)M_PI));
+    b = std::sqrt(b) * std::sqrt((2.0L * l0 + 1.0L) / (4.0L * (long double)M_PI));
     int t;
     SpinStart st;
-    st.mant = std::frexp((double)bm, &t);
-    st.exp = (int)(e / 2) + t;
+    st.mant = std::frexp((double)b, &t);
+    st.exp = (int)(ee / 2) + t;
     st.a = std::abs(m - s);
     st.b = m + s;
     st.sgnp = (m & 1) ? -1 : 1;
@@ -832,7 +923,7 @@ void sht_spin_start(size_t mmax, int spin, std::vector<SpinStart> &out) {
 
 void launch_alm2leg_spin(const ShtArgs &g, double2 *G, hipStream_t s) {
   const int npairs = (g.N + 1) / 2;
-  const unsigned th = spin_threads(npairs);
+  const unsigned th = ring_threads(npairs, SP_R);
   const dim3 grid(nblk((size_t)npairs, th * SP_R), (unsigned)(g.mmax + 1));
   hipLaunchKernelGGL(k_alm2leg_spin, grid, dim3(th), 0, s, g, G);
   DMX_HIP(hipGetLastError());
@@ -841,7 +932,7 @@ void launch_alm2leg_spin(const ShtArgs &g, double2 *G, hipStream_t s) {
 void launch_leg2alm_spin(const ShtArgs &g, const double2 *Gt, double2 *acc,
                          hipStream_t s) {
   const int npairs = (g.N + 1) / 2;
-  const unsigned th = spin_threads(npairs);
+  const unsigned th = ring_threads(npairs, SP_R);
   const dim3 grid(nblk((size_t)npairs, th * SP_R), (unsigned)(g.mmax + 1));
   hipLaunchKernelGGL(k_leg2alm_spin, grid, dim3(th), 0, s, g, Gt, acc);
   DMX_HIP(hipGetLastError());
