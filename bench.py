@@ -497,6 +497,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--chooser", default="auto",
                     choices=["auto", "learned", "analytic"])
+    ap.add_argument("--point-records", action="store_true",
+                    help="nufft workload: sort point records instead of a "
+                         "permutation (DMX_POINT_RECORDS; off by default)")
     ap.add_argument("--calib", action="store_true",
                     help="run one 256 MiB device copy first (calibrates HBM "
                          "byte counters of a profiler run)")
@@ -542,12 +545,14 @@ def main():
     sync = torch.cuda.synchronize
 
     info1, info2 = {}, {}
+    # --point-records: the opt-in record sort instead of the index sort
+    rec = dict(point_records=True) if args.point_records else {}
 
     def step_fast():
         nufft.nu2u(points=p32, coord=c, forward=True, epsilon=eps, out=og32,
-                   chooser=chooser, info=info1)
+                   chooser=chooser, info=info1, **rec)
         nufft.u2nu(grid=g32, coord=c, forward=False, epsilon=eps, out=op32,
-                   chooser=chooser, info=info2)
+                   chooser=chooser, info=info2, **rec)
 
     # timed with the phase profiler off; the phase times come from a second
     # pass of the same steps with HIP events around each phase
@@ -716,6 +721,7 @@ def main():
                              "(sigma 0.3 rad), eps=1e-5, complex64 values, "
                              "float64 coords",
                     M=M, grid=[n, n], eps=eps, chooser=chooser,
+                    sort="point records" if args.point_records else "index",
                     nu2u=dict(W=info1.get("W"), nover=info1.get("nover"),
                               ofactor=info1.get("ofactor"),
                               log2tile=info1.get("log2tile"),

@@ -7,7 +7,8 @@ GPU (used in place, ordered on torch's current stream).  The precision of a
 call is that of `vis` (vis2dirty) or `dirty` (dirty2vis): complex64 /
 float32 or complex128 / float64.  `gpu`, `nthreads` and
 `double_precision_accumulation` are accepted for compatibility (the work
-always runs on the GPU, and accumulation is FP64 throughout)."""
+always runs on the GPU; each tile's sums are accumulated in FP64 in LDS,
+and the w-plane sums of the image in FP64)."""
 import ctypes as ct
 
 import numpy as np

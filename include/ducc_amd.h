@@ -268,20 +268,29 @@ int dmx_nufft_noalloc(int type, const dmx_array *uniform,
                       int fft_order, const dmx_options *opt,
                       dmx_plan_info *info);
 
-/* ---- spherical harmonic synthesis at arbitrary points (spin 0) ----
+/* ---- spherical harmonic synthesis at arbitrary points ----
    ducc0.sht.synthesis_general (src/ducc0/sht/sht.cc:2926-2960, bound at
-   python/sht_pymod.cc:805-845, 2299): map[0, i] = sum_{l, m} a_lm
-   Y_lm(loc[i, 0], loc[i, 1]) for the real field whose a_lm (m >= 0) are
-   given, i.e. theta = loc[i, 0] in [0, pi], phi = loc[i, 1].
-   alm: (1, nalm) complex64 / complex128, a_lm at index mstart[m] +
-   l * lstride (m = 0 .. nm-1 = mmax, l = m .. lmax); map: (1, M) float32 /
-   float64 matching the a_lm precision (any stride); loc: (M, 2) float64.
-   The points are interpolated from the doubled sphere's Fourier
-   coefficients with the NUFFT to accuracy epsilon (sigma_min / sigma_max
-   as for the NUFFT; ducc0's defaults are 1.1 / 2.6).  spin must be 0 and
-   mode 0 (STANDARD): anything else fails with "not yet supported".
-   nthreads is accepted and ignored; verbosity > 0 prints the NUFFT
-   parameters. */
+   python/sht_pymod.cc:805-845, 2299), at the points theta = loc[i, 0] in
+   [0, pi], phi = loc[i, 1]:
+     spin 0:  map[0, i] = sum_{l, m} a_lm Y_lm  for the real field whose
+              a_lm (m >= 0) are given;
+     spin s > 0: map[0, i] + i map[1, i] = -sum_{l, |m| <= l} (E_lm + i B_lm)
+              sY_lm, E_{l,-m} = (-1)^m conj(E_lm) (B likewise), with
+              alm[0] = E and alm[1] = B (mode 0, STANDARD), alm[0] = E and
+              B = 0 (mode 1, GRAD_ONLY), or alm[0] = a_lm of a spin-0 field
+              with E_lm = sqrt(l (l + 1)) a_lm, B = 0 and s = 1 (mode 2,
+              DERIV1: the map is its gradient (d/dtheta, d/dphi / sin
+              theta)).
+   alm: (ncomp, nalm) complex64 / complex128 (ncomp 1 for spin 0 or modes 1
+   and 2, else 2), a_lm at index mstart[m] + l * lstride (m = 0 .. nm-1 =
+   mmax, l = m .. lmax; entries with l < spin are ignored); map: (1, M) for
+   spin 0, else (2, M), float32 / float64 matching the a_lm precision (any
+   strides); loc: (M, 2) float64.  The points are interpolated from the
+   doubled sphere's Fourier coefficients with the NUFFT to accuracy epsilon
+   (sigma_min / sigma_max as for the NUFFT; ducc0's defaults are 1.1 /
+   2.6).  Modes 1 and 2 need spin > 0, mode 2 spin 1 (ducc0 computes mode 2
+   as spin 1 whatever spin it is given).  nthreads is accepted and ignored;
+   verbosity > 0 prints the NUFFT parameters. */
 int dmx_sht_synthesis_general(const dmx_array *alm, dmx_array *map,
                               size_t spin, size_t lmax,
                               const int64_t *mstart, size_t nm,
@@ -292,9 +301,9 @@ int dmx_sht_synthesis_general(const dmx_array *alm, dmx_array *map,
 /* ducc0.sht.adjoint_synthesis_general (src/ducc0/sht/sht.cc:2962-2993,
    python/sht_pymod.cc:847-870, 2300): the adjoint of the above with respect
    to the real inner products of the map and of the a_lm (m > 0 entries
-   counted twice), written into alm at the (l, m) positions of the layout;
-   for host arrays the other entries come back zero, device arrays keep
-   them.  Same arguments. */
+   counted twice), written into alm at the (l, m) positions of the layout
+   (zero for l < spin; the m = 0 entries real); for host arrays the other
+   entries come back zero, device arrays keep them.  Same arguments. */
 int dmx_sht_adjoint_synthesis_general(dmx_array *alm, const dmx_array *map,
                                       size_t spin, size_t lmax,
                                       const int64_t *mstart, size_t nm,
