@@ -33,7 +33,7 @@ class DmxOptions(ct.Structure):
                 ("flags", ct.c_int32), ("deterministic", ct.c_int32),
                 ("reserved", ct.c_int32 * 5)]
 
-STREAM_GIVEN, ASYNC, POINT_RECORDS = 1, 2, 4
+STREAM_GIVEN, ASYNC, POINT_RECORDS, CHECK_OFFSETS = 1, 2, 4, 8
 
 
 class DmxPlanInfo(ct.Structure):
@@ -239,7 +239,7 @@ def profile_read():
 
 def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
             fp64_below=1e-6, stream=None, sigma=0.0, spot_check=True,
-            deterministic=False, point_records=False):
+            deterministic=False, point_records=False, check_offsets=False):
     o = DmxOptions()
     lib().dmx_default_options(ct.byref(o))
     o.spot_check = 2 if spot_check == "fail" else int(bool(spot_check))
@@ -261,4 +261,6 @@ def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
     o.deterministic = int(bool(deterministic))
     if point_records:
         o.flags |= POINT_RECORDS
+    if check_offsets:
+        o.flags |= CHECK_OFFSETS
     return o

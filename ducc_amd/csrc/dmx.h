@@ -304,6 +304,9 @@ struct SpreadArgs {
   // the sorted position, instead of idx / coords / values
   int rec;
   RecBufs recs;
+  // non-NULL: incremented for every point whose fold lands outside its
+  // tile (the offset is clamped either way)
+  uint32_t *offcheck;
 };
 void launch_spread(bool fp64, const SpreadArgs &a, hipStream_t s);
 void launch_interp(bool fp64, const SpreadArgs &a, void *out, int out_f64,

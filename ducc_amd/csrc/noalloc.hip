@@ -76,6 +76,8 @@ extern "C" int dmx_nufft_noalloc(int type, const dmx_array *uniform,
                "the learned chooser is not available without allocation");
     DMX_ASSERT(opt.deterministic == 0,
                "deterministic mode is not available without allocation");
+    DMX_ASSERT(!(opt.flags & DMX_CHECK_OFFSETS),
+               "offset checks are not available without allocation");
     // validation (same rules and messages as dmx_nu2u / dmx_u2nu)
     DMX_ASSERT(coord->ndim == 2, "coord must be a 2D array");
     DMX_ASSERT(coord->typecode == DMX_F32 || coord->typecode == DMX_F64,

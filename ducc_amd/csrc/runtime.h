@@ -137,6 +137,7 @@ struct Params {
   int chooser_used = 0;
   int chooser_verified = 0;
   bool deterministic = false;  // type 1 without floating-point atomics
+  bool check_offsets = false;  // DMX_CHECK_OFFSETS
 };
 
 // device constants shared by all plans of one parameter set
@@ -264,6 +265,10 @@ class Plan {
   // returns true when it did
   bool run_fft(bool forward, bool type1, const GridCopyArgs *crop = nullptr);
   void fill_spread_args(SpreadArgs &a) const;
+  // DMX_CHECK_OFFSETS: waits for the kernels and fails if any point's
+  // offset was clamped
+  void check_offsets();
+  Buf offcheck_;  // the clamp counter (4 bytes, zeroed at creation)
 
   Params p_;
   hipStream_t s_;

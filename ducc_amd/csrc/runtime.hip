@@ -364,6 +364,7 @@ void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt) {
   const int ndim = p.ndim;
   const KRow &k = kernel_db()[kidx];
   p.kidx = kidx;
+  p.check_offsets = (opt.flags & DMX_CHECK_OFFSETS) != 0;
   p.W = k.W;
   p.beta = k.beta;
   p.e0 = k.e0;
@@ -436,6 +437,13 @@ Plan::Plan(const Params &p, hipStream_t s) : p_(p), s_(s) {
   lds_side_ = (1 << p.log2tile) + p.W - 1;
   ntiles_ = (size_t)g_.ntiles[0] * g_.ntiles[1] * g_.ntiles[2];
   su_ = current_arena() ? arena_setup(p, s) : setup_for(p);
+  if (p.check_offsets) {
+    DMX_ASSERT(!current_arena(),
+               "offset checks are not available without allocation");
+    offcheck_.stream = s;
+    offcheck_.reset(4);
+    launch_zero(offcheck_.p, 4, s);
+  }
 }
 
 Plan::~Plan() {
@@ -767,6 +775,19 @@ void Plan::fill_spread_args(SpreadArgs &a) const {
   a.tile_sub0 = tile_sub0_;
   a.rec = rec_ ? 1 : 0;
   a.recs = recs_;
+  a.offcheck = p_.check_offsets ? offcheck_.as<uint32_t>() : nullptr;
+}
+
+void Plan::check_offsets() {
+  if (!p_.check_offsets) return;
+  uint32_t n = 0;
+  DMX_HIP(hipMemcpyAsync(&n, offcheck_.p, 4, hipMemcpyDeviceToHost, s_));
+  DMX_HIP(hipStreamSynchronize(s_));
+  if (n) {
+    launch_zero(offcheck_.p, 4, s_);
+    throw Error("internal: " + std::to_string(n) +
+                " point offsets fell outside their tile");
+  }
 }
 
 void Plan::prepare_grid_side() {
@@ -955,6 +976,7 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
     } else {
       launch_spread(p_.fp64, a, s_);
     }
+    check_offsets();
   }
   GridCopyArgs c = copy_args(uni, uni_f64, ust);
   if (run_fft(forward, true, &c)) return;  // the last FFT pass cropped
@@ -984,6 +1006,7 @@ void Plan::u2nu(bool forward, const void *uni, bool uni_f64,
     a.grid = cur_;
     Phase ph(this, "interpolation");
     launch_interp(p_.fp64, a, out, out_f64 ? 1 : 0, s_);
+    check_offsets();
   }
 }
 
@@ -999,6 +1022,7 @@ void Plan::u2nu_grad(bool forward, const void *uni, bool uni_f64,
     a.grid = cur_;
     Phase ph(this, "interpolation (derivative)");
     launch_interp_grad(p_.fp64, a, out, out_f64 ? 1 : 0, s_);
+    check_offsets();
   }
 }
 

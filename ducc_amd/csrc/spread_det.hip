@@ -68,7 +68,12 @@ __global__ __launch_bounds__(64) void k_spread_det(SpreadArgs a) {
         const int i0 = fold_axis(a.g, d, xc[d], frac);
         xk[d] = (T)((double)(W - 1) - 2.0 * frac);
         const int od = i0 - b0[d];
-        o[d] = od < 0 ? 0 : (od >= tile ? tile - 1 : od);
+        if (od < 0 || od >= tile) {
+          if (a.offcheck && lane == 0) atomicAdd(a.offcheck, 1u);
+          o[d] = od < 0 ? 0 : tile - 1;
+        } else {
+          o[d] = od;
+        }
       }
       T vr, vi;
       {

@@ -228,7 +228,10 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
                 const int i0 = fold_axis(a.g, d, cc[k][d], frac);
                 xs[k][d] = (T)((double)(W - 1) - 2.0 * frac);
                 int od = i0 - b0[d];
-                od = od < 0 ? 0 : (od >= tile ? tile - 1 : od);
+                if (od < 0 || od >= tile) {
+                  if (a.offcheck) atomicAdd(a.offcheck, 1u);
+                  od = od < 0 ? 0 : tile - 1;
+                }
                 o |= (uint32_t)od << (10 * d);
               }
               os[k] = o;
@@ -552,7 +555,10 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
           i0[d] = fold_axis(a.g, d, x_[d], frac);
           xk[d] = (T)((double)(W - 1) - 2.0 * frac);
           o[d] = i0[d] - b0[d];
-          o[d] = o[d] < 0 ? 0 : (o[d] >= tile ? tile - 1 : o[d]);
+          if (o[d] < 0 || o[d] >= tile) {
+            if (a.offcheck) atomicAdd(a.offcheck, 1u);
+            o[d] = o[d] < 0 ? 0 : tile - 1;
+          }
         }
       }
 #pragma unroll

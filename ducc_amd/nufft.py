@@ -16,6 +16,13 @@ behaviour apart from the compute precision, see `u2nu`):
              True: type-1 results identical bit for bit from run to run
              (spreading without floating-point atomics; slower).  Type 2
              is always deterministic.
+  point_records
+             True: unplanned calls sort the points' records instead of a
+             permutation (same results; slower on MI355X).
+  check_offsets
+             True: spreading / interpolation count points whose fold lands
+             outside their tile and the call fails if any did (a debugging
+             aid: one counter and a wait per call).
   info       a dict that receives the parameters actually used.
 
 Arrays may be numpy arrays (host memory: copied to the GPU and back) or
@@ -72,19 +79,22 @@ def _check_dtypes(vals, coord):
 
 
 def _opts(precision, chooser, log2tile, stream, kernel_idx=-1, sigma=0.0,
-          spot_check=True, deterministic=False, point_records=False):
+          spot_check=True, deterministic=False, point_records=False,
+          check_offsets=False):
     return _lib.options(precision=precision, chooser=chooser,
                         log2tile=log2tile, stream=stream,
                         kernel_idx=kernel_idx, sigma=sigma,
                         spot_check=spot_check, deterministic=deterministic,
-                        point_records=point_records)
+                        point_records=point_records,
+                        check_offsets=check_offsets)
 
 
 def u2nu(*, grid, coord, forward, epsilon, nthreads=1, out=None,
          verbosity=0, sigma_min=1.2, sigma_max=2.51,
          periodicity=_TWO_PI, fft_order=False, precision="auto",
          chooser="analytic", log2tile=0, info=None, kernel_idx=-1,
-         spot_check=True, deterministic=False, point_records=False):
+         spot_check=True, deterministic=False, point_records=False,
+         check_offsets=False):
     """Type 2 NUFFT: uniform grid -> values at nonuniform points.
 
     out_j = sum_k grid_k exp(isign i 2pi/P k.x_j), isign = -1 if forward,
@@ -106,7 +116,8 @@ def u2nu(*, grid, coord, forward, epsilon, nthreads=1, out=None,
     o, ko = _lib.describe(out)
     opt = _opts(precision, chooser, log2tile, _stream_for(grid, coord, out),
                 kernel_idx=kernel_idx, spot_check=spot_check,
-                deterministic=deterministic, point_records=point_records)
+                deterministic=deterministic, point_records=point_records,
+                check_offsets=check_offsets)
     inf = _lib.DmxPlanInfo()
     _lib.check(_lib.lib().dmx_u2nu(
         ct.byref(g), ct.byref(c), int(bool(forward)), float(epsilon),
@@ -124,7 +135,8 @@ def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
          verbosity=0, sigma_min=1.2, sigma_max=2.51,
          periodicity=_TWO_PI, fft_order=False, precision="auto",
          chooser="analytic", log2tile=0, info=None, kernel_idx=-1,
-         spot_check=True, deterministic=False, point_records=False):
+         spot_check=True, deterministic=False, point_records=False,
+         check_offsets=False):
     """Type 1 NUFFT: values at nonuniform points -> uniform grid.
 
     out_k = sum_j points_j exp(isign i 2pi/P k.x_j).  `out` is required: its
@@ -141,7 +153,8 @@ def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
     o, ko = _lib.describe(out)
     opt = _opts(precision, chooser, log2tile, _stream_for(points, coord, out),
                 kernel_idx=kernel_idx, spot_check=spot_check,
-                deterministic=deterministic, point_records=point_records)
+                deterministic=deterministic, point_records=point_records,
+                check_offsets=check_offsets)
     inf = _lib.DmxPlanInfo()
     _lib.check(_lib.lib().dmx_nu2u(
         ct.byref(p), ct.byref(c), int(bool(forward)), float(epsilon),
@@ -201,7 +214,7 @@ class plan:
     def __init__(self, *, nu2u, coord, grid_shape, epsilon, nthreads=0,
                  sigma_min=1.1, sigma_max=2.6, periodicity=_TWO_PI,
                  fft_order=False, precision="auto", chooser="analytic",
-                 log2tile=0, deterministic=False):
+                 log2tile=0, deterministic=False, check_offsets=False):
         _lib.require_gpu()
         self._h = None
         if _is_torch(coord):
@@ -220,7 +233,7 @@ class plan:
         shp = np.asarray(self._shape, dtype=np.int64)
         self._stream = _stream_for(coord)
         opt = _opts(precision, chooser, log2tile, self._stream,
-                    deterministic=deterministic)
+                    deterministic=deterministic, check_offsets=check_offsets)
         h = ct.c_void_p()
         _lib.check(_lib.lib().dmx_plan_create(
             ct.byref(h), int(bool(nu2u)), ct.byref(c), shp.ctypes.data, nd,

@@ -84,6 +84,12 @@ typedef struct dmx_options {
                               MI355X the record sort moves 2.5x the bytes
                               of the index sort and costs more than the
                               gathers it saves */
+#define DMX_CHECK_OFFSETS 8 /* debugging aid: spreading and interpolation
+                              count the points whose fold lands outside the
+                              tile the index build put them in (the kernels
+                              clamp such offsets), and the call fails with
+                              an error when any did; one device counter and
+                              a wait per call */
 
 typedef struct dmx_plan dmx_plan;
 

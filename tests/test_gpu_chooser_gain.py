@@ -5,7 +5,7 @@ both choices in alternation on the GPU.  3-D requests in FP64 (eps < 1e-6),
 where ducc0's CPU cost model picks large supports with small tiles that
 the GPU runs slowly, must gain at least 1.1x (geometric mean; 1.19x over 35
 such requests of the 200-request evaluation, up to 2.7x), and the whole set
-must not be slower than 0.95x.  Both choices meet eps (the learned one
+must not be slower than the analytic choice (>= 1.0x).  Both choices meet eps (the learned one
 passed its FP64 spot check, which the call records as chooser_used)."""
 import math
 
@@ -65,4 +65,4 @@ def test_learned_choice_gains_over_analytic(gpu):
     gall = gm([x[4] for x in rows])
     print(f"geo-mean gain: 3-D FP64 {g3:.3f}, all {gall:.3f}")
     assert g3 >= 1.1
-    assert gall >= 0.95
+    assert gall >= 1.0
