@@ -81,9 +81,39 @@ __global__ __launch_bounds__(RT) void k_rec_scatter(RecPassArgs a) {
   const size_t blk0 = (size_t)blockIdx.x * RTILE;
   const size_t c0 = blk0 + (size_t)wave * (RTILE / NW);
   const int dsh = a.offbits + a.shift;
+  // every field of this thread's items is loaded before the ranking, so
+  // that all of the block's global loads are in flight together
   uint32_t word[RITEMS], dig[RITEMS];
   bool valid[RITEMS];
-  T xk[FIRST ? NDIM : 1][RITEMS];
+  T xk[NDIM][RITEMS];
+  T vr[TYPE1 ? RITEMS : 1], vi[TYPE1 ? RITEMS : 1];
+  uint32_t jv[TYPE1 ? 1 : RITEMS];
+  auto load_value = [&](int r, size_t j) {
+    if constexpr (TYPE1) {
+      vr[r] = vi[r] = T(0);
+      if (!valid[r]) return;
+      if constexpr (FIRST) {
+        if (a.val_f64) {
+          const double2 z = ((const double2 *)a.values)[j];
+          vr[r] = (T)z.x;
+          vi[r] = (T)z.y;
+        } else {
+          const float2 z = ((const float2 *)a.values)[j];
+          vr[r] = (T)z.x;
+          vi[r] = (T)z.y;
+        }
+      } else {
+        const T2 z = ((const T2 *)a.in.v)[j];
+        vr[r] = z.x;
+        vi[r] = z.y;
+      }
+    } else {
+      if constexpr (FIRST)
+        jv[r] = (uint32_t)j;
+      else
+        jv[r] = valid[r] ? a.in.j[j] : 0u;
+    }
+  };
   if constexpr (FIRST) {
     const int L = a.g.log2tile;
     const int tmask = (1 << L) - 1;
@@ -94,6 +124,9 @@ __global__ __launch_bounds__(RT) void k_rec_scatter(RecPassArgs a) {
       valid[r] = j < M;
       if (valid[r]) load_point(a.g, a.coords, j, NDIM, xc[r]);
     }
+#pragma unroll
+    for (int r = 0; r < RITEMS; ++r)
+      load_value(r, c0 + (size_t)r * 64 + lane);
 #pragma unroll
     for (int r = 0; r < RITEMS; ++r) {
       uint32_t key = 0, off = 0;
@@ -116,6 +149,16 @@ __global__ __launch_bounds__(RT) void k_rec_scatter(RecPassArgs a) {
       valid[r] = j < M;
       word[r] = valid[r] ? a.in.word[j] : 0xFFFFFFFFu;
     }
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d)
+#pragma unroll
+      for (int r = 0; r < RITEMS; ++r) {
+        const size_t j = c0 + (size_t)r * 64 + lane;
+        xk[d][r] = valid[r] ? ((const T *)a.in.x[d])[j] : T(0);
+      }
+#pragma unroll
+    for (int r = 0; r < RITEMS; ++r)
+      load_value(r, c0 + (size_t)r * 64 + lane);
   }
 #pragma unroll
   for (int r = 0; r < RITEMS; ++r) dig[r] = (word[r] >> dsh) & 0xFFu;
@@ -155,45 +198,12 @@ __global__ __launch_bounds__(RT) void k_rec_scatter(RecPassArgs a) {
 
   // kernel arguments, one axis at a time
 #pragma unroll
-  for (int d = 0; d < NDIM; ++d) {
-    T v[RITEMS];
-#pragma unroll
-    for (int r = 0; r < RITEMS; ++r) {
-      if constexpr (FIRST) {
-        v[r] = xk[d][r];
-      } else {
-        const size_t j = c0 + (size_t)r * 64 + lane;
-        v[r] = valid[r] ? ((const T *)a.in.x[d])[j] : T(0);
-      }
-    }
-    stage_move<T>((T *)stage8, v, lpos, valid, gpos, nvalid,
+  for (int d = 0; d < NDIM; ++d)
+    stage_move<T>((T *)stage8, xk[d], lpos, valid, gpos, nvalid,
                   (T *)a.out.x[d], 1);
-  }
 
   if constexpr (TYPE1) {
-    // the value, cast to the compute type; 16-byte values move as halves
-    T vr[RITEMS], vi[RITEMS];
-#pragma unroll
-    for (int r = 0; r < RITEMS; ++r) {
-      const size_t j = c0 + (size_t)r * 64 + lane;
-      vr[r] = vi[r] = T(0);
-      if (!valid[r]) continue;
-      if constexpr (FIRST) {
-        if (a.val_f64) {
-          const double2 z = ((const double2 *)a.values)[j];
-          vr[r] = (T)z.x;
-          vi[r] = (T)z.y;
-        } else {
-          const float2 z = ((const float2 *)a.values)[j];
-          vr[r] = (T)z.x;
-          vi[r] = (T)z.y;
-        }
-      } else {
-        const T2 z = ((const T2 *)a.in.v)[j];
-        vr[r] = z.x;
-        vi[r] = z.y;
-      }
-    }
+    // the value, in the compute type; 16-byte values move as halves
     if constexpr (sizeof(T) == 4) {
       uint2 v[RITEMS];
 #pragma unroll
@@ -209,16 +219,7 @@ __global__ __launch_bounds__(RT) void k_rec_scatter(RecPassArgs a) {
                     (T *)a.out.v + 1, 2);
     }
   } else {
-    uint32_t v[RITEMS];
-#pragma unroll
-    for (int r = 0; r < RITEMS; ++r) {
-      const size_t j = c0 + (size_t)r * 64 + lane;
-      if constexpr (FIRST)
-        v[r] = (uint32_t)j;
-      else
-        v[r] = valid[r] ? a.in.j[j] : 0u;
-    }
-    stage_move<uint32_t>(stage4, v, lpos, valid, gpos, nvalid, a.out.j, 1);
+    stage_move<uint32_t>(stage4, jv, lpos, valid, gpos, nvalid, a.out.j, 1);
   }
 }
 
