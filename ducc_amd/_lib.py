@@ -34,6 +34,7 @@ class DmxOptions(ct.Structure):
                 ("reserved", ct.c_int32 * 5)]
 
 STREAM_GIVEN, ASYNC, POINT_RECORDS, CHECK_OFFSETS = 1, 2, 4, 8
+SPREAD_FP64_ACC = 16
 
 
 class DmxPlanInfo(ct.Structure):
@@ -239,7 +240,8 @@ def profile_read():
 
 def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
             fp64_below=1e-6, stream=None, sigma=0.0, spot_check=True,
-            deterministic=False, point_records=False, check_offsets=False):
+            deterministic=False, point_records=False, check_offsets=False,
+            fp64_accumulation=False):
     o = DmxOptions()
     lib().dmx_default_options(ct.byref(o))
     o.spot_check = 2 if spot_check == "fail" else int(bool(spot_check))
@@ -263,4 +265,6 @@ def options(precision="auto", chooser="analytic", log2tile=0, kernel_idx=-1,
         o.flags |= POINT_RECORDS
     if check_offsets:
         o.flags |= CHECK_OFFSETS
+    if fp64_accumulation:
+        o.flags |= SPREAD_FP64_ACC
     return o

@@ -307,6 +307,9 @@ struct SpreadArgs {
   // non-NULL: incremented for every point whose fold lands outside its
   // tile (the offset is clamped either way)
   uint32_t *offcheck;
+  // 1: FP32 spreading accumulates the region in packed fixed point
+  // (spread_impl.h, SpreadStash<..., FIX>)
+  int fixacc;
 };
 void launch_spread(bool fp64, const SpreadArgs &a, hipStream_t s);
 void launch_interp(bool fp64, const SpreadArgs &a, void *out, int out_f64,

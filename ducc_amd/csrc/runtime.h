@@ -138,7 +138,12 @@ struct Params {
   int chooser_verified = 0;
   bool deterministic = false;  // type 1 without floating-point atomics
   bool check_offsets = false;  // DMX_CHECK_OFFSETS
+  // FP32 spreading in packed fixed point (eps >= FIX_MIN_EPS, no
+  // DMX_SPREAD_FP64_ACC)
+  bool spread_fix = false;
 };
+// the smallest eps for which FP32 spreading accumulates in fixed point
+constexpr double FIX_MIN_EPS = 1e-5;
 
 // device constants shared by all plans of one parameter set
 struct Setup {

@@ -365,6 +365,8 @@ void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt) {
   const KRow &k = kernel_db()[kidx];
   p.kidx = kidx;
   p.check_offsets = (opt.flags & DMX_CHECK_OFFSETS) != 0;
+  p.spread_fix = !p.fp64 && p.eps >= FIX_MIN_EPS &&
+                 !(opt.flags & DMX_SPREAD_FP64_ACC);
   p.W = k.W;
   p.beta = k.beta;
   p.e0 = k.e0;
@@ -776,6 +778,7 @@ void Plan::fill_spread_args(SpreadArgs &a) const {
   a.rec = rec_ ? 1 : 0;
   a.recs = recs_;
   a.offcheck = p_.check_offsets ? offcheck_.as<uint32_t>() : nullptr;
+  a.fixacc = p_.spread_fix ? 1 : 0;
 }
 
 void Plan::check_offsets() {

@@ -19,6 +19,18 @@ void go(const SpreadArgs &a, hipStream_t s) {
   if (a.use_lds) {
     size_t cells = a.lds_side;
     for (int d = 1; d < NDIM; ++d) cells *= a.lds_side;
+    if constexpr (sizeof(T) == 4) {
+      // packed fixed point: one 8-byte cell, the whole subproblem staged
+      const size_t shf =
+          fix_region_bytes(cells) + sizeof(SpreadStash<T, NDIM, true>);
+      if (a.fixacc && shf <= LDS_MAX_BYTES) {
+        allow_big_lds(k_spread<T, NDIM, W, true, true>, shf);
+        hipLaunchKernelGGL((k_spread<T, NDIM, W, true, true>), dim3(nb),
+                           dim3(256), shf, s, a);
+        DMX_HIP(hipGetLastError());
+        return;
+      }
+    }
     size_t sh = spread_lds_bytes(cells, NDIM, sizeof(T) == 8);
     allow_big_lds(k_spread<T, NDIM, W, true>, sh);
     hipLaunchKernelGGL((k_spread<T, NDIM, W, true>), dim3(nb), dim3(256), sh,

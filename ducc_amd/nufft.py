@@ -19,6 +19,11 @@ behaviour apart from the compute precision, see `u2nu`):
   point_records
              True: unplanned calls sort the points' records instead of a
              permutation (same results; slower on MI355X).
+  fp64_accumulation
+             True: type-1 spreading of the FP32 path accumulates each tile
+             in FP64 (two LDS float atomics per tap) instead of packed
+             fixed point (one 64-bit integer atomic per tap, used at
+             eps >= 1e-5).
   check_offsets
              True: spreading / interpolation count points whose fold lands
              outside their tile and the call fails if any did (a debugging
@@ -80,13 +85,14 @@ def _check_dtypes(vals, coord):
 
 def _opts(precision, chooser, log2tile, stream, kernel_idx=-1, sigma=0.0,
           spot_check=True, deterministic=False, point_records=False,
-          check_offsets=False):
+          check_offsets=False, fp64_accumulation=False):
     return _lib.options(precision=precision, chooser=chooser,
                         log2tile=log2tile, stream=stream,
                         kernel_idx=kernel_idx, sigma=sigma,
                         spot_check=spot_check, deterministic=deterministic,
                         point_records=point_records,
-                        check_offsets=check_offsets)
+                        check_offsets=check_offsets,
+                        fp64_accumulation=fp64_accumulation)
 
 
 def u2nu(*, grid, coord, forward, epsilon, nthreads=1, out=None,
@@ -136,7 +142,7 @@ def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
          periodicity=_TWO_PI, fft_order=False, precision="auto",
          chooser="analytic", log2tile=0, info=None, kernel_idx=-1,
          spot_check=True, deterministic=False, point_records=False,
-         check_offsets=False):
+         check_offsets=False, fp64_accumulation=False):
     """Type 1 NUFFT: values at nonuniform points -> uniform grid.
 
     out_k = sum_j points_j exp(isign i 2pi/P k.x_j).  `out` is required: its
@@ -154,7 +160,8 @@ def nu2u(*, points, coord, forward, epsilon, nthreads=1, out=None,
     opt = _opts(precision, chooser, log2tile, _stream_for(points, coord, out),
                 kernel_idx=kernel_idx, spot_check=spot_check,
                 deterministic=deterministic, point_records=point_records,
-                check_offsets=check_offsets)
+                check_offsets=check_offsets,
+                fp64_accumulation=fp64_accumulation)
     inf = _lib.DmxPlanInfo()
     _lib.check(_lib.lib().dmx_nu2u(
         ct.byref(p), ct.byref(c), int(bool(forward)), float(epsilon),

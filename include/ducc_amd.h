@@ -84,6 +84,11 @@ typedef struct dmx_options {
                               MI355X the record sort moves 2.5x the bytes
                               of the index sort and costs more than the
                               gathers it saves */
+#define DMX_SPREAD_FP64_ACC 16 /* type-1 spreading of the FP32 path
+                              accumulates each tile region in FP64 (two LDS
+                              float atomics per tap) also where it would
+                              use packed 32.32 fixed point (one 64-bit
+                              integer atomic per tap; eps >= 1e-5) */
 #define DMX_CHECK_OFFSETS 8 /* debugging aid: spreading and interpolation
                               count the points whose fold lands outside the
                               tile the index build put them in (the kernels
