@@ -31,6 +31,10 @@ PEAK_HBM_GBS = 8000.0
 # a rate measured on MI355X with a microbenchmark (profiles/lds_atomics.md),
 # not a published figure; random addresses reach a third of it
 PEAK_LDS_F64_ATOMICS = 4.66e12
+# ds_add_u64 on distinct cells, the same microbenchmark: the FP32 path's
+# fixed-point spreading (eps >= 1e-5) adds one per tap
+PEAK_LDS_U64_ATOMICS = 5.06e12
+FIX_MIN_EPS = 1e-5
 
 
 def clustered_points(rng, M, d=2, ncl=8, sig=0.3):
@@ -672,14 +676,17 @@ def main():
         roofline["traffic_GBps"] = round(hb, 1)
         roofline["traffic_frac"] = round(hb / PEAK_HBM_GBS, 4)
     if name == "spreading":
-        # LDS atomics: W^d taps x (re, im) per point, plus one ranking
-        # atomic per point
-        na = M * (2 * inf["W"] ** d + 1)
+        # LDS atomics per point: the FP32 path at eps >= 1e-5 adds each tap
+        # as one packed fixed-point ds_add_u64, otherwise as two ds_add_f64
+        # (re, im); one ranking atomic per point besides
+        fix = not inf.get("fp64") and eps >= FIX_MIN_EPS
+        na = M * ((1 if fix else 2) * inf["W"] ** d + 1)
         ra = na / (per_launch[dom] * 1e-3)
+        pk = PEAK_LDS_U64_ATOMICS if fix else PEAK_LDS_F64_ATOMICS
         roofline["lds_atomics"] = dict(
+            kind="ds_add_u64 (fixed point)" if fix else "ds_add_f64",
             per_launch=int(na), achieved=float(f"{ra:.4g}"),
-            peak=PEAK_LDS_F64_ATOMICS, unit="atomics/s",
-            frac=round(ra / PEAK_LDS_F64_ATOMICS, 4),
+            peak=pk, unit="atomics/s", frac=round(ra / pk, 4),
             peak_source="self-measured microbenchmark rate "
                         "(profiles/lds_atomics.md), not a published figure")
 
