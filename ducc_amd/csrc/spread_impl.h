@@ -221,6 +221,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
         }
       }
       float fscale = 1.f, finv = 1.f;  // FIX: this subproblem's scale
+      bool fbad = false;  // FIX: a value of the subproblem is not finite
       constexpr int SC = SpreadStash<T, NDIM, FIX>::SC, PT = SC / 256;
       for (uint32_t c0 = sb.start; c0 < sb.end; c0 += SC) {
         const int nc = (int)min((uint32_t)SC, sb.end - c0);
@@ -326,8 +327,12 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
 #pragma unroll
           for (int k = 0; k < PT; ++k) {
             amax[k] = 0.f;
-            if (tid + 256 * k < nc)
-              amax[k] = fmaxf(fabsf((float)vr_[k]), fabsf((float)vi_[k]));
+            if (tid + 256 * k < nc) {
+              const float ar = fabsf((float)vr_[k]), ai = fabsf((float)vi_[k]);
+              // fmaxf ignores NaN: a NaN part counts as infinite
+              amax[k] = (ar != ar || ai != ai) ? __builtin_inff()
+                                               : fmaxf(ar, ai);
+            }
             vs += amax[k];
             vm = fmaxf(vm, amax[k]);
           }
@@ -415,6 +420,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
         if constexpr (FIX) {
           fscale = fx.scale;
           finv = fx.inv;
+          fbad = !(__uint_as_float(fx.vmaxb) <= 3.4028235e38f);
         }
         const int run = ((lane & 15) << 2) | (lane >> 4);
         // 3-D tiles of few points (runs shorter than the four waves): every
@@ -519,7 +525,18 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
       __syncthreads();
       const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
                 n2 = NDIM > 2 ? a.g.nover[2] : 1;
-      for (int c = tid; c < ncell; c += blockDim.x) {
+      if (FIX && fbad) {
+        // a NaN or infinite value: the integer sums mean nothing, and the
+        // FFT turns one NaN grid cell into NaN everywhere, as FP64
+        // accumulation of such a value would
+        if (tid == 0) {
+          size_t gi = (size_t)wrapi(b0[0], n0);
+          if (NDIM > 1) gi = gi * n1 + wrapi(b0[1], n1);
+          if (NDIM > 2) gi = gi * n2 + wrapi(b0[2], n2);
+          atomicAdd(&grid[2 * gi], (T)__builtin_nanf(""));
+        }
+      }
+      for (int c = tid; c < ncell && !(FIX && fbad); c += blockDim.x) {
         T re, im;
         if constexpr (FIX) {
           const unsigned long long t = sfx[c];

@@ -90,3 +90,22 @@ def test_fixed_point_zero_values(gpu):
     out = np.full((256, 256), 1 + 1j, np.complex64)
     nufft.nu2u(points=z, coord=c, forward=True, epsilon=1e-4, out=out)
     assert np.all(out == 0)
+
+
+def test_fixed_point_non_finite_values(gpu):
+    """a NaN or infinite value turns the result into NaN, as FP64
+    accumulation does (after the FFT every mode sees the bad cell)"""
+    from ducc_amd import nufft
+    rng = np.random.default_rng(6)
+    c = rng.uniform(-np.pi, np.pi, (5000, 2))
+    for bad in (np.nan, np.inf):
+        v = (rng.normal(size=5000) + 1j * rng.normal(size=5000)) \
+            .astype(np.complex64)
+        v[1234] = bad
+        out = np.empty((64, 64), np.complex64)
+        nufft.nu2u(points=v, coord=c, forward=True, epsilon=1e-4, out=out)
+        ref = np.empty((64, 64), np.complex64)
+        nufft.nu2u(points=v, coord=c, forward=True, epsilon=1e-4, out=ref,
+                   fp64_accumulation=True)
+        assert not np.all(np.isfinite(ref))
+        assert np.all(np.isnan(out))
