@@ -272,6 +272,85 @@ def bench_sht(args):
     print(json.dumps(line))
 
 
+def bench_sht_pseudo(args):
+    """ducc0.sht.pseudo_analysis_general as the reference demo runs it
+    (python/demos/sht_general.py:29-45): a random spin-2 sky at lmax = mmax
+    = 1187 synthesized at eps 1e-12 on 2,000,000 points of a Fibonacci
+    lattice, then analysed at eps 1e-7 with at most 20 LSMR iterations.
+    One step is one pseudo_analysis_general with the map and the points on
+    the device; ducc0 itself (oracle/_ref) on all host cores, on the same
+    map, is the CPU baseline (one call after a warm-up: about 2 s)."""
+    import torch
+    from ducc_amd import _lib, sht
+    from oracle import sht as S
+    dev = torch.device("cuda:0")
+    lmax, npt, eps, spin, maxiter = 1187, 2_000_000, 1e-7, 2, 20
+    rng = np.random.default_rng(42)
+    a_np = np.stack([S.random_alm(lmax, lmax, rng) for _ in range(2)])
+    ofs = 0
+    for s_ in range(spin):  # l < spin: zero (sht_general.py:8-17)
+        a_np[:, ofs:ofs + spin - s_] = 0
+        ofs += lmax + 1 - s_
+    loc_np = fibonacci_lattice(npt)
+    a = torch.from_numpy(a_np).to(dev)
+    loc = torch.from_numpy(loc_np).to(dev)
+    mp = torch.empty((2, npt), dtype=torch.float64, device=dev)
+    sht.synthesis_general(alm=a, spin=spin, lmax=lmax, loc=loc,
+                          epsilon=1e-12, map=mp)
+    at = torch.empty_like(a)
+    res = {}
+    sync = torch.cuda.synchronize
+
+    def step():
+        res["r"] = sht.pseudo_analysis_general(
+            map=mp, loc=loc, lmax=lmax, spin=spin, nthreads=1, epsilon=eps,
+            maxiter=maxiter, alm=at)
+    steps = max(1, min(args.steps, 3))
+    warm = 1
+    t = time_steps(step, steps, warm, sync)
+    _, istop, itn, rnorm, _ = res["r"]
+    err_truth = l2rel(at.cpu().numpy(), a_np)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            from oracle import ref as R
+            if R.available():
+                nth = os.cpu_count() or 1
+                m_np = mp.cpu().numpy()
+                R.sht_pseudo_analysis_general(m_np, lmax, loc_np, spin, 2,
+                                              eps, a_np.shape[1],
+                                              nthreads=nth)
+                t0 = time.perf_counter()
+                ref = R.sht_pseudo_analysis_general(m_np, lmax, loc_np, spin,
+                                                    maxiter, eps,
+                                                    a_np.shape[1],
+                                                    nthreads=nth)
+                tc = time.perf_counter() - t0
+                cpu = dict(value=1. / tc, unit="analyses/s", cores=nth,
+                           kind="reference", seconds=round(tc, 3),
+                           istop=int(ref[1]), itn=int(ref[2]),
+                           rnorm=float(ref[3]),
+                           err_vs_ducc0=l2rel(at.cpu().numpy(), ref[0]),
+                           sample="ducc0 pseudo_analysis_general, same map "
+                                  "and points, one call after a 2-iteration "
+                                  "warm-up")
+        except Exception as e:
+            cpu = dict(error=str(e))
+    line = dict(
+        metric="analyses/s, spherical harmonic pseudo-analysis at arbitrary "
+               "points (spin 2, LSMR)",
+        value=round(steps / t, 3), unit="analyses/s", n_gpus=1,
+        steps=steps, warmup=warm, ms_per_step=round(1e3 * t / steps, 3),
+        higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="fp64", data="synthetic",
+        config=dict(workload="ducc0.sht.pseudo_analysis_general, spin 2, "
+                             "lmax = mmax = 1187, 2,000,000 Fibonacci-lattice "
+                             "points, eps 1e-7, maxiter 20"),
+        istop=int(istop), itn=int(itn), rnorm=float(rnorm),
+        err_vs_input_alm=err_truth, cpu_baseline=cpu)
+    print(json.dumps(line))
+
+
 def bench_wgridder(args):
     """ducc0.wgridder vis2dirty + dirty2vis with w-gridding on a synthetic
     observation shaped like the reference's benchmark
@@ -492,6 +571,9 @@ def main():
     ap.add_argument("--spin", type=int, default=0,
                     help="sht workload: spin of the field (0, or > 0 for "
                          "E and B)")
+    ap.add_argument("--pseudo", action="store_true",
+                    help="sht workload: pseudo_analysis_general as the "
+                         "reference demo runs it (spin 2, 20 iterations)")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--M", type=int, default=10_000_000)
@@ -526,7 +608,7 @@ def main():
         bench_c2c(args)
         return
     if args.workload == "sht":
-        bench_sht(args)
+        (bench_sht_pseudo if args.pseudo else bench_sht)(args)
         return
     if args.workload == "wgridder":
         bench_wgridder(args)

@@ -121,6 +121,13 @@ _SIG = {
                    ct.c_int64, _AP, ct.c_double, ct.c_double, ct.c_double,
                    ct.c_size_t, ct.c_int, ct.c_size_t,
                    ct.POINTER(DmxOptions)]),
+    "dmx_sht_pseudo_analysis_general": (
+        ct.c_int, [_AP, _AP, ct.c_size_t, ct.c_size_t, _P, ct.c_size_t,
+                   ct.c_int64, _AP, ct.c_double, ct.c_double, ct.c_size_t,
+                   ct.c_size_t, ct.c_double, ct.c_size_t,
+                   ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t),
+                   ct.POINTER(ct.c_double), ct.POINTER(ct.c_double),
+                   ct.POINTER(DmxOptions)]),
     "dmx_vis2dirty": (ct.c_int, [_AP, _AP, _AP, _AP, _AP, ct.c_double,
                                  ct.c_double, ct.c_double, ct.c_int,
                                  ct.c_int, ct.c_int, ct.c_int, ct.c_int,

@@ -946,6 +946,65 @@ struct ShtJob {
 };
 }  // namespace
 
+namespace {
+// The synthesis and its adjoint on device data, with the points sorted
+// once for any number of applications (the index of one NUFFT plan serves
+// both directions).  alm: complex, component c at c * acomp + index; map:
+// real, component c / point i at c * ms0 + i * ms1 (ms0 unused for spin 0).
+struct ShtOp {
+  ShtJob &job;
+  hipStream_t s;
+  std::unique_ptr<Plan> pl;
+  Buf G, z, acc;
+  ShtOp(ShtJob &j, const void *loc_dev, hipStream_t s_) : job(j), s(s_) {
+    G.stream = z.stream = acc.stream = s;
+    G.reset(job.rows() * job.geo.nphi * 16);
+    z.reset(std::max<size_t>(job.M, 1) * 16);
+    acc.reset(job.nacc * 16 * (job.spin > 0 ? 2 : 1));
+    if (job.M) pl = job.plan(loc_dev, s);
+  }
+  ~ShtOp() {
+    if (pl) pl->mark_done();
+  }
+  void synth(const void *alm, int64_t acomp, void *map, int64_t ms0,
+             int64_t ms1) {
+    job.args.alm = alm;
+    job.args.acomp = acomp;
+    if (job.spin > 0)
+      launch_alm2leg_spin(job.args, G.as<double2>(), s);
+    else
+      launch_alm2leg(job.args, G.as<double2>(), s);
+    auto keep = job.theta_fft(G.p, true, s);
+    if (!job.M) return;
+    const int64_t ust[2] = {job.geo.nphi, 1};
+    pl->u2nu(false, G.p, true, ust, z.p, true);
+    launch_real_part(z.as<double2>(), job.M, map, job.vf64, ms1,
+                     job.spin > 0 ? ms0 : 0, s);
+  }
+  void adjoint(const void *map, int64_t ms0, int64_t ms1, void *alm,
+               int64_t acomp) {
+    job.args.acomp = acomp;
+    if (job.M) {
+      launch_to_complex(map, job.vf64, ms1, job.spin > 0 ? ms0 : 0, job.M,
+                        z.as<double2>(), s);
+      const int64_t ust[2] = {job.geo.nphi, 1};
+      pl->nu2u(true, z.p, true, G.p, true, ust);
+    } else {
+      launch_zero(G.p, G.bytes, s);
+    }
+    auto keep = job.theta_fft(G.p, false, s);
+    if (leg2alm_needs_zero(job.args)) launch_zero(acc.p, acc.bytes, s);
+    if (job.spin > 0) {
+      launch_leg2alm_spin(job.args, G.as<double2>(), acc.as<double2>(), s);
+      launch_store_alm_spin(acc.as<double2>(), job.args, alm, job.vf64, s);
+    } else {
+      launch_leg2alm(job.args, G.as<double2>(), acc.as<double2>(), s);
+      launch_store_alm(acc.as<double2>(), job.args, alm, job.vf64, s);
+    }
+  }
+};
+}  // namespace
+
 int dmx_sht_synthesis_general(const dmx_array *alm, dmx_array *map,
                               size_t spin, size_t lmax,
                               const int64_t *mstart, size_t nm,
@@ -963,27 +1022,10 @@ int dmx_sht_synthesis_general(const dmx_array *alm, dmx_array *map,
     hipStream_t s = pick_stream(opt);
     DevIn ain(*alm, s, false), lin(*loc, s, false);
     DevOut mo(*map, s, true);
-    job.args.alm = ain.p;
-    job.args.acomp = ain.st[0];
-    Buf G(job.rows() * job.geo.nphi * 16, s);
-    if (job.spin > 0)
-      launch_alm2leg_spin(job.args, G.as<double2>(), s);
-    else
-      launch_alm2leg(job.args, G.as<double2>(), s);
-    auto keep = job.theta_fft(G.p, true, s);
-    Buf z(job.M * 16, s);
-    if (job.M) {
-      auto pl = job.plan(lin.p, s);
-      const int64_t ust[2] = {job.geo.nphi, 1};
-      pl->u2nu(false, G.p, true, ust, z.p, true);
-      launch_real_part(z.as<double2>(), job.M, mo.p, job.vf64, mo.st[1],
-                       job.spin > 0 ? mo.st[0] : 0, s);
-      if (verbosity > 0) pl->report_params(false);
-      mo.finish(s);
-      pl->mark_done();
-    } else {
-      mo.finish(s);
-    }
+    ShtOp op(job, lin.p, s);
+    op.synth(ain.p, ain.st[0], mo.p, mo.st[0], mo.st[1]);
+    if (verbosity > 0 && op.pl) op.pl->report_params(false);
+    mo.finish(s);
   });
 }
 
@@ -1005,36 +1047,92 @@ int dmx_sht_adjoint_synthesis_general(dmx_array *alm, const dmx_array *map,
     hipStream_t s = pick_stream(opt);
     DevIn min_(*map, s, true), lin(*loc, s, false);
     DevOut ao(*alm, s, false);
-    job.args.acomp = ao.st[0];
     // entries outside the (l, m) layout of a host array come back zero
     if (!alm->device) launch_zero(ao.p, (size_t)nelem(*alm) * elsize(alm->typecode), s);
-    Buf Gt(job.rows() * job.geo.nphi * 16, s);
-    Buf z(std::max<size_t>(job.M, 1) * 16, s);
-    std::unique_ptr<Plan> pl;
-    if (job.M) {
-      launch_to_complex(min_.p, job.vf64, min_.st[1],
-                        job.spin > 0 ? min_.st[0] : 0, job.M,
-                        z.as<double2>(), s);
-      pl = job.plan(lin.p, s);
-      const int64_t ust[2] = {job.geo.nphi, 1};
-      pl->nu2u(true, z.p, true, Gt.p, true, ust);
-      if (verbosity > 0) pl->report_params(true);
-    } else {
-      launch_zero(Gt.p, Gt.bytes, s);
-    }
-    auto keep = job.theta_fft(Gt.p, false, s);
-    const size_t nacc_c = job.spin > 0 ? 2 : 1;
-    Buf acc(job.nacc * 16 * nacc_c, s);
-    if (leg2alm_needs_zero(job.args)) launch_zero(acc.p, acc.bytes, s);
-    if (job.spin > 0) {
-      launch_leg2alm_spin(job.args, Gt.as<double2>(), acc.as<double2>(), s);
-      launch_store_alm_spin(acc.as<double2>(), job.args, ao.p, job.vf64, s);
-    } else {
-      launch_leg2alm(job.args, Gt.as<double2>(), acc.as<double2>(), s);
-      launch_store_alm(acc.as<double2>(), job.args, ao.p, job.vf64, s);
-    }
+    ShtOp op(job, lin.p, s);
+    op.adjoint(min_.p, min_.st[0], min_.st[1], ao.p, ao.st[0]);
+    if (verbosity > 0 && op.pl) op.pl->report_params(true);
     ao.finish(s);
-    if (pl) pl->mark_done();
+  });
+}
+
+int dmx_sht_pseudo_analysis_general(dmx_array *alm, const dmx_array *map,
+                                    size_t spin, size_t lmax,
+                                    const int64_t *mstart, size_t nm,
+                                    int64_t lstride, const dmx_array *loc,
+                                    double sigma_min, double sigma_max,
+                                    size_t nthreads, size_t maxiter,
+                                    double epsilon, size_t verbosity,
+                                    size_t *istop_out, size_t *itn_out,
+                                    double *rnorm_out, double *arnorm_out,
+                                    const dmx_options *opt_in) {
+  (void)nthreads;
+  return guard([&] {
+    DMX_ASSERT(alm && map && loc, "null argument");
+    DMX_ASSERT(epsilon > 0, "epsilon must be positive");
+    reap(false);
+    dmx_options opt = opt_in ? *opt_in : defaults();
+    // sht.cc:3003-3058: LSMR on the synthesis (STANDARD mode) with its
+    // adjoint, both at 0.1 epsilon
+    ShtJob job(*alm, *map, spin, lmax, mstart, nm, lstride, *loc,
+               0.1 * epsilon, sigma_min, sigma_max, 0, opt);
+    hipStream_t s = pick_stream(opt);
+    DevIn min_(*map, s, true), lin(*loc, s, false);
+    DevOut ao(*alm, s, false);
+    const bool f64 = job.vf64;
+    const int ncomp = job.spin > 0 ? 2 : 1;
+    const size_t es = f64 ? 8 : 4;
+    const int64_t nalm = alm->shape[1];
+    const size_t na = (size_t)ncomp * nalm * 2, nmp = (size_t)ncomp * job.M;
+    // LSMR vectors: x, v, h, hbar (a_lm, contiguous, entries outside the
+    // layout stay zero), u, w (map, contiguous); b: the map
+    Buf x(na * es, s), v(na * es, s), h(na * es, s), hb(na * es, s),
+        u(std::max<size_t>(nmp, 1) * es, s), w(std::max<size_t>(nmp, 1) * es, s),
+        nrm(64, s);
+    for (Buf *b : {&x, &v, &h, &hb}) launch_zero(b->p, b->bytes, s);
+    // u = b: the map, made contiguous
+    lsmr_copy_map(f64, ncomp, job.M, min_.p, min_.st[0], min_.st[1], u.p, s);
+    ShtOp op(job, lin.p, s);
+    const int64_t acomp = nalm;
+    auto alm_norm = [&](const void *a) {
+      launch_zero(nrm.p, 8, s);
+      lsmr_alm_sumsq(job.args, f64, a, ncomp, acomp, nrm.as<double>(), s);
+      double r = 0;
+      DMX_HIP(hipMemcpyAsync(&r, nrm.p, 8, hipMemcpyDeviceToHost, s));
+      DMX_HIP(hipStreamSynchronize(s));
+      return std::sqrt(r);
+    };
+    auto map_norm = [&](const void *m) {
+      launch_zero(nrm.p, 8, s);
+      lsmr_sumsq(f64, nmp, m, nrm.as<double>(), s);
+      double r = 0;
+      DMX_HIP(hipMemcpyAsync(&r, nrm.p, 8, hipMemcpyDeviceToHost, s));
+      DMX_HIP(hipStreamSynchronize(s));
+      return std::sqrt(r);
+    };
+    // A^T u lands in a scratch a_lm vector (the adjoint writes only the
+    // layout's entries; the others stay zero)
+    Buf at(na * es, s);
+    launch_zero(at.p, at.bytes, s);
+    auto applyA = [&](const void *vin, void *mout) {
+      op.synth(vin, acomp, mout, (int64_t)job.M, 1);
+    };
+    auto applyAT = [&](const void *min, void *aout) {
+      op.adjoint(min, (int64_t)job.M, 1, aout, acomp);
+    };
+    LsmrResult res = lsmr_solve(
+        na, nmp, f64, applyA, applyAT, alm_norm, map_norm, x.p, v.p, h.p,
+        hb.p, u.p, w.p, at.p, 1e-14 * std::sqrt((double)job.M), epsilon, 1e8,
+        maxiter, s);
+    DMX_HIP(hipMemcpyAsync(ao.p, x.p, na * es, hipMemcpyDeviceToDevice, s));
+    ao.finish(s);
+    if (verbosity > 0)
+      std::printf("pseudo_analysis_general: istop %zu, %zu iterations, "
+                  "|r|/|b| %.3e\n", res.istop, res.itn, res.rnorm);
+    if (istop_out) *istop_out = res.istop;
+    if (itn_out) *itn_out = res.itn;
+    if (rnorm_out) *rnorm_out = res.rnorm;
+    if (arnorm_out) *arnorm_out = res.arnorm;
   });
 }
 

@@ -2,6 +2,7 @@
 // on the NUFFT (sht.hip).
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "dmx.h"
@@ -72,5 +73,34 @@ void launch_leg2alm_spin(const ShtArgs &g, const double2 *Gt, double2 *acc,
                          hipStream_t s);
 void launch_store_alm_spin(const double2 *acc, const ShtArgs &g, void *alm,
                            bool alm_f64, hipStream_t s);
+
+// vector steps of the LSMR solver of pseudo_analysis_general: y = a x + b y
+// over n reals; *out += the sum of squares of n reals; *out += the a_lm
+// inner product of alm with itself (m > 0 entries twice) over ncomp
+// components at stride acomp
+void lsmr_axpby(bool f64, size_t n, double a, const void *x, double b,
+                void *y, hipStream_t s);
+void lsmr_sumsq(bool f64, size_t n, const void *x, double *out,
+                hipStream_t s);
+void lsmr_alm_sumsq(const ShtArgs &g, bool f64, const void *alm, int ncomp,
+                    int64_t acomp, double *out, hipStream_t s);
+// ncomp components of M points at strides (s0, s1) -> contiguous
+void lsmr_copy_map(bool f64, int ncomp, size_t M, const void *src,
+                   int64_t s0, int64_t s1, void *dst, hipStream_t s);
+// LSMR from x0 = 0 for the operator A (x-space -> b-space) and its adjoint
+// AT, with b in u on entry; nx / nb reals per vector; v, h, hbar, t
+// (x-space) and w (b-space) are scratch; x holds the solution on return
+struct LsmrResult {
+  size_t istop, itn;
+  double rnorm, arnorm;  // |r| / |b|, |A^T r| / (|A| |r|)
+};
+LsmrResult lsmr_solve(size_t nx, size_t nb, bool f64,
+                      const std::function<void(const void *, void *)> &A,
+                      const std::function<void(const void *, void *)> &AT,
+                      const std::function<double(const void *)> &normx,
+                      const std::function<double(const void *)> &normb,
+                      void *x, void *v, void *h, void *hbar, void *u,
+                      void *w, void *t, double atol, double btol,
+                      double conlim, size_t maxiter, hipStream_t s);
 
 }  // namespace dmx

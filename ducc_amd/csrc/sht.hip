@@ -37,6 +37,8 @@
 // (role of the scaled recursion of sht.cc:77-82).  All in FP64.
 #include <algorithm>
 #include <cmath>
+#include <functional>
+#include <limits>
 #include <vector>
 
 #include "c2c.h"
@@ -809,6 +811,73 @@ __global__ void k_store_alm_spin(const double2 *acc, ShtArgs g, TO *alm) {
   }
 }
 
+// ---------------------------------------------------------- LSMR helpers
+// y = a x + b y over n reals of type T (x may alias y)
+template <typename T>
+__global__ void k_axpby(size_t n, double a, const T *x, double b, T *y) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+}
+
+// block sum of v into *out (FP64 atomic per block)
+__device__ __forceinline__ void block_sum_to(double v, double *out) {
+  __shared__ double part[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+    atomicAdd(out, t);
+  }
+}
+
+// sum of squares of n reals
+template <typename T>
+__global__ __launch_bounds__(256) void k_sumsq(size_t n, const T *x,
+                                               double *out) {
+  double v = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double t = (double)x[i];
+    v += t * t;
+  }
+  block_sum_to(v, out);
+}
+
+// sum over the layout's entries of w_m |a_lm|^2, w_0 = 1, w_m = 2, over
+// ncomp components at stride acomp (complex elements)
+template <typename T>
+__global__ __launch_bounds__(256) void k_alm_sumsq(ShtArgs g, const T *alm,
+                                                   int ncomp, int64_t acomp,
+                                                   double *out) {
+  const int m = blockIdx.y;
+  const int l = m + blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (l <= g.lmax) {
+    const int64_t ix = g.mstart[m] + (int64_t)l * g.lstride;
+    for (int c = 0; c < ncomp; ++c) {
+      const double re = (double)alm[2 * (c * acomp + ix)],
+                   im = (double)alm[2 * (c * acomp + ix) + 1];
+      v += re * re + im * im;
+    }
+    if (m > 0) v *= 2.0;
+  }
+  block_sum_to(v, out);
+}
+
+// component c, point i: src[c * s0 + i * s1] -> dst[c * M + i]
+template <typename T>
+__global__ void k_copy_map(int ncomp, size_t M, const T *src, int64_t s0,
+                           int64_t s1, T *dst) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  for (int c = 0; c < ncomp; ++c)
+    dst[c * M + i] = src[(int64_t)c * s0 + (int64_t)i * s1];
+}
+
 unsigned nblk(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 // threads per block of the Legendre kernels: the ring pairs over R per
@@ -984,6 +1053,178 @@ void launch_store_alm(const double2 *acc, const ShtArgs &g, void *alm,
     hipLaunchKernelGGL(k_store_alm<float>, grid, dim3(256), 0, s, acc, g,
                        (float *)alm);
   DMX_HIP(hipGetLastError());
+}
+
+
+void lsmr_axpby(bool f64, size_t n, double a, const void *x, double b,
+                void *y, hipStream_t s) {
+  if (!n) return;
+  const unsigned nb = (unsigned)std::min<size_t>(nblk(n, 256), 8192);
+  if (f64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3(nb), dim3(256), 0, s, n, a,
+                       (const double *)x, b, (double *)y);
+  else
+    hipLaunchKernelGGL(k_axpby<float>, dim3(nb), dim3(256), 0, s, n, a,
+                       (const float *)x, b, (float *)y);
+  DMX_HIP(hipGetLastError());
+}
+
+void lsmr_sumsq(bool f64, size_t n, const void *x, double *out,
+                hipStream_t s) {
+  if (!n) return;
+  const unsigned nb = (unsigned)std::min<size_t>(nblk(n, 256), 2048);
+  if (f64)
+    hipLaunchKernelGGL(k_sumsq<double>, dim3(nb), dim3(256), 0, s, n,
+                       (const double *)x, out);
+  else
+    hipLaunchKernelGGL(k_sumsq<float>, dim3(nb), dim3(256), 0, s, n,
+                       (const float *)x, out);
+  DMX_HIP(hipGetLastError());
+}
+
+void lsmr_alm_sumsq(const ShtArgs &g, bool f64, const void *alm, int ncomp,
+                    int64_t acomp, double *out, hipStream_t s) {
+  const dim3 grid(nblk((size_t)g.lmax + 1, 256), (unsigned)(g.mmax + 1));
+  if (f64)
+    hipLaunchKernelGGL(k_alm_sumsq<double>, grid, dim3(256), 0, s, g,
+                       (const double *)alm, ncomp, acomp, out);
+  else
+    hipLaunchKernelGGL(k_alm_sumsq<float>, grid, dim3(256), 0, s, g,
+                       (const float *)alm, ncomp, acomp, out);
+  DMX_HIP(hipGetLastError());
+}
+
+
+void lsmr_copy_map(bool f64, int ncomp, size_t M, const void *src,
+                   int64_t s0, int64_t s1, void *dst, hipStream_t s) {
+  if (!M) return;
+  if (f64)
+    hipLaunchKernelGGL(k_copy_map<double>, dim3(nblk(M, 256)), dim3(256), 0,
+                       s, ncomp, M, (const double *)src, s0, s1,
+                       (double *)dst);
+  else
+    hipLaunchKernelGGL(k_copy_map<float>, dim3(nblk(M, 256)), dim3(256), 0,
+                       s, ncomp, M, (const float *)src, s0, s1, (float *)dst);
+  DMX_HIP(hipGetLastError());
+}
+
+// LSMR (D. C.-L. Fong and M. A. Saunders, "LSMR: an iterative algorithm
+// for sparse least-squares problems", SIAM J. Sci. Comput. 33 (2011)
+// 2950-2971), without damping, from x0 = 0: the solver of
+// pseudo_analysis_general (sht.cc:3050-3056 calls ducc0's with atol =
+// 1e-14 sqrt(npix), btol = epsilon, conlim = 1e8).  The vectors live on the
+// device; the scalar recurrences run on the host in FP64, and the norms
+// (nx / nb: the x- and b-space norms) return to the host each iteration.
+LsmrResult lsmr_solve(size_t nx, size_t nb, bool f64,
+                      const std::function<void(const void *, void *)> &A,
+                      const std::function<void(const void *, void *)> &AT,
+                      const std::function<double(const void *)> &normx,
+                      const std::function<double(const void *)> &normb,
+                      void *x, void *v, void *h, void *hbar, void *u,
+                      void *w, void *t, double atol, double btol,
+                      double conlim, size_t maxiter, hipStream_t s) {
+  LsmrResult r{0, 0, 0.0, 0.0};
+  auto copy = [&](size_t n, const void *from, void *to) {
+    lsmr_axpby(f64, n, 1.0, from, 0.0, to, s);
+  };
+  auto scale = [&](size_t n, double a, void *y) {
+    lsmr_axpby(f64, n, 0.0, y, a, y, s);
+  };
+  // beta u = b, alpha v = A^T u
+  double beta = normb(u);
+  const double bnorm = beta;
+  if (beta > 0) scale(nb, 1.0 / beta, u);
+  AT(u, t);
+  copy(nx, t, v);
+  double alpha = normx(v);
+  if (alpha > 0) scale(nx, 1.0 / alpha, v);
+  double zetabar = alpha * beta, alphabar = alpha, rho = 1.0, rhobar = 1.0,
+         cbar = 1.0, sbar = 0.0;
+  copy(nx, v, h);
+  double betadd = beta, betad = 0.0, rhodold = 1.0, tautildeold = 0.0,
+         thetatilde = 0.0, zeta = 0.0, d = 0.0;
+  double normA2 = alpha * alpha, maxrbar = 0.0, minrbar = 1e100;
+  double normA = std::sqrt(normA2), normr = beta, normar = alpha * beta;
+  const double ctol = conlim > 0 ? 1.0 / conlim : 0.0;
+  if (normar == 0.0) {
+    r.rnorm = bnorm > 0 ? normr / bnorm : 0.0;
+    return r;
+  }
+  size_t itn = 0, istop = 0;
+  while (itn < maxiter) {
+    ++itn;
+    // beta u = A v - alpha u, alpha v = A^T u - beta v
+    A(v, w);
+    lsmr_axpby(f64, nb, 1.0, w, -alpha, u, s);
+    beta = normb(u);
+    if (beta > 0) scale(nb, 1.0 / beta, u);
+    AT(u, t);
+    lsmr_axpby(f64, nx, 1.0, t, -beta, v, s);
+    alpha = normx(v);
+    if (alpha > 0) scale(nx, 1.0 / alpha, v);
+    // rotations (no damping: alphahat = alphabar)
+    const double rhoold = rho;
+    rho = std::hypot(alphabar, beta);
+    const double c = alphabar / rho, sn = beta / rho;
+    const double thetanew = sn * alpha;
+    alphabar = c * alpha;
+    const double rhobarold = rhobar, zetaold = zeta;
+    const double thetabar = sbar * rho, rhotemp = cbar * rho;
+    rhobar = std::hypot(cbar * rho, thetanew);
+    cbar = cbar * rho / rhobar;
+    sbar = thetanew / rhobar;
+    zeta = cbar * zetabar;
+    zetabar = -sbar * zetabar;
+    // hbar, x, h
+    lsmr_axpby(f64, nx, 1.0, h, -(thetabar * rho / (rhoold * rhobarold)),
+               hbar, s);
+    lsmr_axpby(f64, nx, zeta / (rho * rhobar), hbar, 1.0, x, s);
+    lsmr_axpby(f64, nx, 1.0, v, -(thetanew / rho), h, s);
+    // the residual norm estimate
+    const double betaacute = betadd, betahat = c * betaacute;
+    betadd = -sn * betaacute;
+    const double thetatildeold = thetatilde;
+    const double rhotildeold = std::hypot(rhodold, thetabar);
+    const double ctildeold = rhodold / rhotildeold,
+                 stildeold = thetabar / rhotildeold;
+    thetatilde = stildeold * rhobar;
+    rhodold = ctildeold * rhobar;
+    betad = -stildeold * betad + ctildeold * betahat;
+    tautildeold = (zetaold - thetatildeold * tautildeold) / rhotildeold;
+    const double taud = (zeta - thetatilde * tautildeold) / rhodold;
+    normr = std::sqrt(d + (betad - taud) * (betad - taud) + betadd * betadd);
+    // norm and condition estimates of A
+    normA2 += beta * beta;
+    normA = std::sqrt(normA2);
+    normA2 += alpha * alpha;
+    maxrbar = std::max(maxrbar, rhobarold);
+    if (itn > 1) minrbar = std::min(minrbar, rhobarold);
+    const double condA =
+        std::max(maxrbar, rhotemp) / std::min(minrbar, rhotemp);
+    // stopping rules
+    normar = std::fabs(zetabar);
+    const double xnorm = normx(x);
+    const double test1 = normr / bnorm;
+    const double test2 = normA * normr != 0.0
+                             ? normar / (normA * normr)
+                             : std::numeric_limits<double>::infinity();
+    const double test3 = 1.0 / condA;
+    const double t1 = test1 / (1.0 + normA * xnorm / bnorm);
+    const double rtol = btol + atol * normA * xnorm / bnorm;
+    if (itn >= maxiter) istop = 7;
+    if (1.0 + test3 <= 1.0) istop = 6;
+    if (1.0 + test2 <= 1.0) istop = 5;
+    if (1.0 + t1 <= 1.0) istop = 4;
+    if (test3 <= ctol) istop = 3;
+    if (test2 <= atol) istop = 2;
+    if (test1 <= rtol) istop = 1;
+    if (istop > 0) break;
+  }
+  r.istop = istop;
+  r.itn = itn;
+  r.rnorm = normr / bnorm;
+  r.arnorm = normA * normr != 0.0 ? normar / (normA * normr) : 0.0;
+  return r;
 }
 
 }  // namespace dmx

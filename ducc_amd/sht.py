@@ -1,7 +1,7 @@
 """ducc0.sht general synthesis on the GPU: the Python surface of
-ducc0.sht.synthesis_general / adjoint_synthesis_general
-(python/sht_pymod.cc:805-870, 2299-2300), keyword-only, with ducc0's
-defaults.
+ducc0.sht.synthesis_general / adjoint_synthesis_general /
+pseudo_analysis_general (python/sht_pymod.cc:805-927, 2299-2301),
+keyword-only, with ducc0's defaults.
 
 Arrays may be numpy (copied to and from the device) or torch tensors on the
 GPU (used in place, ordered on torch's current stream).  a_lm are complex
@@ -140,3 +140,43 @@ def adjoint_synthesis_general(*, map, spin, lmax, loc, epsilon, mstart=None,
         float(sigma_min), float(sigma_max), int(nthreads), mo,
         int(bool(verbose)), ct.byref(opt)))
     return alm
+
+
+def pseudo_analysis_general(*, lmax, map, loc, spin, nthreads, maxiter,
+                            epsilon=1e-5, sigma_min=1.1, sigma_max=2.6,
+                            mstart=None, lstride=1, mmax=None, alm=None,
+                            verbose=False):
+    """a_lm whose synthesis_general best fits `map` at loc (least squares),
+    by LSMR (sht.cc:3003-3058, sht_pymod.cc:885-927: atol 1e-14
+    sqrt(npoints), btol epsilon, at most maxiter iterations, the NUFFTs at
+    0.1 epsilon).  Returns (alm, istop, itn, |r| / |b|,
+    |A^T r| / (|A| |r|)) with ducc0's meaning of istop (1: solution found,
+    2: least-squares solution found, 3: condition too large, 7: maxiter
+    reached); `alm` is the same object when given."""
+    _lib.require_gpu()
+    ncomp, _ = _ncomp(spin, "STANDARD")
+    ms = _mstart(lmax, mmax, mstart)
+    nalm = _min_almdim(lmax, ms, lstride)
+    if alm is None:
+        if _is_torch(map):
+            import torch
+            cdt = torch.complex128 if map.dtype == torch.float64 \
+                else torch.complex64
+            alm = torch.zeros((ncomp, nalm), dtype=cdt, device=map.device)
+        else:
+            cdt = np.complex128 if np.asarray(map).dtype == np.float64 \
+                else np.complex64
+            alm = np.zeros((ncomp, nalm), dtype=cdt)
+    a, ka = _lib.describe(alm)
+    m_, km = _lib.describe(map)
+    lc, kl = _lib.describe(loc)
+    opt = _stream_opts(alm, map, loc)
+    istop, itn = ct.c_size_t(0), ct.c_size_t(0)
+    rn, arn = ct.c_double(0), ct.c_double(0)
+    _lib.check(_lib.lib().dmx_sht_pseudo_analysis_general(
+        ct.byref(a), ct.byref(m_), int(spin), int(lmax), ms.ctypes.data,
+        len(ms), int(lstride), ct.byref(lc), float(sigma_min),
+        float(sigma_max), int(nthreads), int(maxiter), float(epsilon),
+        int(bool(verbose)), ct.byref(istop), ct.byref(itn), ct.byref(rn),
+        ct.byref(arn), ct.byref(opt)))
+    return alm, istop.value, itn.value, rn.value, arn.value

@@ -324,6 +324,29 @@ int dmx_sht_adjoint_synthesis_general(dmx_array *alm, const dmx_array *map,
                                       int mode, size_t verbosity,
                                       const dmx_options *opt);
 
+/* ducc0.sht.pseudo_analysis_general (src/ducc0/sht/sht.cc:3003-3058,
+   python/sht_pymod.cc:885-927, 2301): a_lm whose synthesis_general best
+   fits the map values at loc in the least-squares sense, by LSMR from zero
+   (atol 1e-14 sqrt(npoints), btol epsilon, conlim 1e8, at most maxiter
+   iterations), the synthesis and its adjoint in mode STANDARD at accuracy
+   0.1 epsilon; the a_lm norm counts m > 0 entries twice.  alm (output):
+   (ncomp, nalm) complex as for the synthesis, every entry written (zero
+   outside the layout); map: (nmaps, npoints) real of the same precision.
+   istop (1: solution found, 2: least-squares solution found, 3: condition
+   too large, 7: maxiter reached, as in ducc0), itn, |r| / |b| and
+   |A^T r| / (|A| |r|) are returned through the pointers (each may be
+   NULL).  The points are sorted once for all iterations. */
+int dmx_sht_pseudo_analysis_general(dmx_array *alm, const dmx_array *map,
+                                    size_t spin, size_t lmax,
+                                    const int64_t *mstart, size_t nm,
+                                    int64_t lstride, const dmx_array *loc,
+                                    double sigma_min, double sigma_max,
+                                    size_t nthreads, size_t maxiter,
+                                    double epsilon, size_t verbosity,
+                                    size_t *istop, size_t *itn,
+                                    double *rnorm, double *arnorm,
+                                    const dmx_options *opt);
+
 /* ---- radio-interferometric gridding with the w-term ----
    ducc0.wgridder.vis2dirty (src/ducc0/wgridder/wgridder.h:1626-1714, bound
    at python/wgridder_pymod.cc:561-640; ms2dirty :483-493 is the same call

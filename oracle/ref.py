@@ -46,6 +46,9 @@ def lib():
                                       S, D, D, D, S, S, I]
         L.ref_wgridder.argtypes = [I, I, P, S, P, S, P, P, P, P, S, S, D, D,
                                    D, I, I, I, I, I, D, D, D, D, S]
+        L.ref_sht_pseudo_analysis.argtypes = [I, P, S, P, S, P, S,
+                                              ct.c_ssize_t, P, S, D, D, S, S,
+                                              D, S, P]
         _lib = L
     return _lib
 
@@ -222,6 +225,30 @@ def sht_adjoint_synthesis_general(mapv, lmax, loc, epsilon, nalm,
                                  epsilon, sigma_min, sigma_max, nthreads,
                                  spin, _MODES[mode]))
     return out[0] if spin == 0 else out
+
+
+def sht_pseudo_analysis_general(mapv, lmax, loc, spin, maxiter, epsilon,
+                                nalm, mstart=None, lstride=1, sigma_min=1.1,
+                                sigma_max=2.6, nthreads=1):
+    """ducc0.sht.pseudo_analysis_general (sht.cc:3003-3058): map (nmaps,
+    npoints) -> (alm (ncomp, nalm), istop, itn, |r|/|b|,
+    |A^T r|/(|A||r|))"""
+    m = np.ascontiguousarray(mapv)
+    single = m.dtype == np.float32
+    if not single:
+        m = m.astype(np.float64, copy=False)
+    m2 = m.reshape(-1, m.shape[-1])
+    ms = _mstart(lmax, mstart)
+    lc = np.ascontiguousarray(loc, np.float64)
+    na = 1 if spin == 0 else 2
+    out = np.zeros((na, nalm), np.complex64 if single else np.complex128)
+    res = np.zeros(4)
+    _check(lib().ref_sht_pseudo_analysis(int(single), _p(out), nalm, _p(m2),
+                                         lmax, _p(ms), len(ms), lstride,
+                                         _p(lc), len(lc), sigma_min,
+                                         sigma_max, nthreads, maxiter,
+                                         epsilon, spin, _p(res)))
+    return out, int(res[0]), int(res[1]), res[2], res[3]
 
 
 def wgridder(adjoint, uvw, freq, vis_or_dirty, wgt, mask, npix_x, npix_y,

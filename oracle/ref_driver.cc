@@ -345,6 +345,37 @@ int ref_sht_general(int single, int adjoint, void *alm, size_t nalm,
   });
 }
 
+// ducc0.sht.pseudo_analysis_general: map (nmap, npt) real -> alm (na,
+// nalm) complex (na = nmap = 1 for spin 0, else 2); res[0..3] = istop,
+// itn, |r| / |b|, |A^T r| / (|A| |r|)
+int ref_sht_pseudo_analysis(int single, void *alm, size_t nalm,
+                            const void *map, size_t lmax, const size_t *mstart,
+                            size_t nm, ptrdiff_t lstride, const double *loc,
+                            size_t npt, double smin, double smax,
+                            size_t nthreads, size_t maxiter, double eps,
+                            size_t spin, double *res) {
+  return guard([&] {
+    cmav<size_t, 1> ms(mstart, {nm});
+    cmav<double, 2> lc(loc, {npt, 2});
+    const size_t na = spin == 0 ? 1 : 2;
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      auto r = pseudo_analysis_general(
+          vmav<complex<T>, 2>((complex<T> *)alm, {na, nalm}),
+          cmav<T, 2>((const T *)map, {na, npt}), spin, lmax, ms, lstride,
+          lc, smin, smax, nthreads, maxiter, eps, false);
+      res[0] = (double)std::get<0>(r);
+      res[1] = (double)std::get<1>(r);
+      res[2] = std::get<2>(r);
+      res[3] = std::get<3>(r);
+    };
+    if (single)
+      run(float(0));
+    else
+      run(double(0));
+  });
+}
+
 // the w-gridder (vis2dirty when adjoint == 0, dirty2vis when 1): uvw
 // (nrow, 3), freq (nchan), vis (nrow, nchan) complex, wgt (nrow, nchan)
 // real or NULL, mask (nrow, nchan) uint8 or NULL, dirty (nx, ny) real;

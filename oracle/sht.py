@@ -225,3 +225,87 @@ def adjoint_synthesis_general_spin(mapv, lmax, loc, spin, ncomp=2,
         if ncomp > 1:
             out[1, ms[m] + ls * lstride] = bB
     return out
+
+
+# ------------------------------------------------- pseudo_analysis_general
+# ducc0 (sht.cc:3003-3058) solves min |synthesis(a) - map| by LSMR with the
+# adjoint above, the a_lm norm counting m > 0 twice.  Here the a_lm are
+# mapped to real coordinates (m = 0: the real part; m > 0: sqrt(2) times
+# the real and imaginary parts), in which that norm is the Euclidean one
+# and the adjoint is the transpose, and scipy's LSMR (the same published
+# algorithm, Fong & Saunders 2011) runs on the direct sums.
+
+def _layout(lmax, ms, lstride, spin):
+    idx = []
+    for m in range(len(ms)):
+        for l in range(m, lmax + 1):
+            idx.append((m, int(ms[m]) + l * lstride))
+    return idx
+
+
+def _alm_to_real(a, idx):
+    out = []
+    for c in range(a.shape[0]):
+        for m, ix in idx:
+            if m == 0:
+                out.append(a[c, ix].real)
+            else:
+                out.append(np.sqrt(2.) * a[c, ix].real)
+                out.append(np.sqrt(2.) * a[c, ix].imag)
+    return np.asarray(out)
+
+
+def _real_to_alm(x, idx, ncomp, size):
+    a = np.zeros((ncomp, size), np.complex128)
+    k = 0
+    for c in range(ncomp):
+        for m, ix in idx:
+            if m == 0:
+                a[c, ix] = x[k]
+                k += 1
+            else:
+                a[c, ix] = (x[k] + 1j * x[k + 1]) / np.sqrt(2.)
+                k += 2
+    return a
+
+
+def pseudo_analysis_general(mapv, lmax, loc, spin, maxiter, epsilon,
+                            mstart=None, lstride=1, size=None):
+    """(alm (ncomp, size), istop, itn, |r| / |b|, |A^T r| / (|A| |r|))"""
+    from scipy.sparse.linalg import LinearOperator, lsmr
+    mapv = np.atleast_2d(np.asarray(mapv, np.float64))
+    ncomp = 1 if spin == 0 else 2
+    ms = default_mstart(lmax) if mstart is None else np.asarray(mstart)
+    n = size
+    if n is None:
+        n = max(int(ms[m]) + lmax * lstride for m in range(len(ms))) + 1
+    idx = _layout(lmax, ms, lstride, spin)
+    npt = loc.shape[0]
+
+    def mv(x):
+        a = _real_to_alm(np.asarray(x).ravel(), idx, ncomp, n)
+        if spin == 0:
+            return synthesis_general(a[0], lmax, loc, ms, lstride)
+        return synthesis_general_spin(a, lmax, loc, spin, ms,
+                                      lstride).ravel()
+
+    def rmv(y):
+        y = np.asarray(y).ravel().reshape(ncomp, npt)
+        if spin == 0:
+            a = adjoint_synthesis_general(y[0], lmax, loc, ms, lstride,
+                                          size=n)[None, :]
+        else:
+            a = adjoint_synthesis_general_spin(y, lmax, loc, spin, ncomp,
+                                               ms, lstride, size=n)
+        return _alm_to_real(a, idx)
+
+    nx = len(_alm_to_real(np.zeros((ncomp, n), complex), idx))
+    A = LinearOperator((ncomp * npt, nx), matvec=mv, rmatvec=rmv,
+                       dtype=np.float64)
+    b = mapv.ravel()
+    x, istop, itn, normr, normar, normA, condA, normx = lsmr(
+        A, b, damp=0.0, atol=1e-14 * np.sqrt(npt), btol=epsilon,
+        conlim=1e8, maxiter=maxiter)
+    nb = np.linalg.norm(b)
+    return (_real_to_alm(x, idx, ncomp, n), istop, itn, normr / nb,
+            normar / (normA * normr) if normA * normr else 0.0)

@@ -262,3 +262,68 @@ def test_spin_matches_ducc0_itself(mode):
                                         spin=s, mode=mode)
     bo = S.adjoint_synthesis_general_spin(r, lmax, loc, s, nc, ms, size=n) * w
     assert O.l2error(b.ravel(), bo.ravel()) <= 1e-10
+
+
+# ------------------------------------------------- pseudo_analysis_general
+
+def _noisy_problem(rng, lmax, spin, npt):
+    n = S.nalm(lmax, lmax)
+    nc = 1 if spin == 0 else 2
+    a = rng.uniform(-1, 1, (nc, n)) + 1j * rng.uniform(-1, 1, (nc, n))
+    a[:, :lmax + 1].imag = 0
+    ofs = 0
+    for s in range(spin):
+        a[:, ofs:ofs + spin - s] = 0
+        ofs += lmax + 1 - s
+    loc = _loc(rng, npt)
+    m = S.synthesis_general(a[0], lmax, loc)[None, :] if spin == 0 \
+        else S.synthesis_general_spin(a, lmax, loc, spin)
+    return a, loc, m + 1e-3 * rng.normal(size=m.shape)
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_pseudo_analysis_matches_ducc0_itself(spin):
+    """scipy's LSMR on the direct sums against ducc0's
+    pseudo_analysis_general (oracle/_ref): the same stopping reason,
+    iteration count and residual, and the same a_lm"""
+    from oracle import ref as R
+    from oracle import core as O
+    if not R.available():
+        pytest.skip("reference library not built")
+    rng = np.random.default_rng(1)
+    lmax = 12
+    # 2000 points for 169 coefficients: well conditioned, so that rounding
+    # differences do not grow over the iterations (on 600 points a
+    # perturbation of the map at 1e-15 moves the 25th iterate by 1e-5)
+    a, loc, m = _noisy_problem(rng, lmax, spin, 2000)
+    n = a.shape[1]
+    for it in (5, 30):
+        o = S.pseudo_analysis_general(m, lmax, loc, spin, it, 1e-10, size=n)
+        r = R.sht_pseudo_analysis_general(m, lmax, loc, spin, it, 1e-10, n)
+        assert o[1:3] == r[1:3]
+        assert abs(o[3] - r[3]) <= 1e-6 * r[3]
+        assert O.l2error(o[0].ravel(), r[0].ravel()) <= 1e-6
+
+
+def test_pseudo_analysis_solves_the_normal_equations():
+    """run to convergence on a full-rank problem, the solution satisfies
+    A^T (A x - b) = 0 (numpy's dense least squares gives the same x)"""
+    rng = np.random.default_rng(2)
+    lmax = 6
+    a, loc, m = _noisy_problem(rng, lmax, 0, 200)
+    n = a.shape[1]
+    x, istop, itn, rr, ar = S.pseudo_analysis_general(m, lmax, loc, 0, 200,
+                                                      1e-13, size=n)
+    assert istop in (1, 2)
+    # the dense problem in the real coordinates of the oracle
+    idx = S._layout(lmax, S.default_mstart(lmax), 1, 0)
+    nx = len(S._alm_to_real(np.zeros((1, n), complex), idx))
+    cols = []
+    for k in range(nx):
+        e = np.zeros(nx)
+        e[k] = 1
+        cols.append(S.synthesis_general(S._real_to_alm(e, idx, 1, n)[0], lmax,
+                                        loc))
+    Am = np.stack(cols, 1)
+    xr = np.linalg.lstsq(Am, m.ravel(), rcond=None)[0]
+    assert np.allclose(S._alm_to_real(x, idx), xr, atol=1e-9)
