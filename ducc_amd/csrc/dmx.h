@@ -39,9 +39,10 @@ struct Geo {
   int coord_v2;   // 1: 2-D float64 coordinates 16-byte aligned (one load)
 };
 
-// One subproblem: a run of at most MAXSUB sorted points of one tile.
+// One subproblem: a run of at most MAXSUB sorted points of one tile; rank:
+// its place among the tile's subproblems (0 for the first).
 struct Sub {
-  uint32_t tile, start, end, pad_;
+  uint32_t tile, start, end, rank;
 };
 
 constexpr int MAXSUB = 1024;
@@ -310,6 +311,12 @@ struct SpreadArgs {
   // 1: FP32 spreading accumulates the region in packed fixed point
   // (spread_impl.h, SpreadStash<..., FIX>)
   int fixacc;
+  // 1: the finished region of a tile, complex of the compute type, goes to
+  // a.slab + tile * cells (plain stores; one workgroup takes all of the
+  // tile's subproblems and adds the later ones to the stored region)
+  // instead of into the grid with atomics; launch_fold_regions then writes
+  // every grid cell
+  int slab_out;
 };
 void launch_spread(bool fp64, const SpreadArgs &a, hipStream_t s);
 void launch_interp(bool fp64, const SpreadArgs &a, void *out, int out_f64,
@@ -320,5 +327,10 @@ void launch_interp_grad(bool fp64, const SpreadArgs &a, void *out,
 // a.grid (no zeroing needed); a.slab holds spread_det_slab_bytes(a)
 void launch_spread_det(bool fp64, const SpreadArgs &a, hipStream_t s);
 size_t spread_det_slab_bytes(const SpreadArgs &a);
+// after launch_spread with a.slab_out: every grid cell = the sum of the
+// region cells covering it (spread_det.hip); writes all of a.grid
+void launch_fold_regions(bool fp64, const SpreadArgs &a, hipStream_t s);
+// bytes of a.slab for launch_spread with a.slab_out (one region per tile)
+size_t spread_region_slab_bytes(const SpreadArgs &a, bool fp64);
 
 }  // namespace dmx

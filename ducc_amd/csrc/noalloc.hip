@@ -105,7 +105,9 @@ extern "C" int dmx_nufft_noalloc(int type, const dmx_array *uniform,
     Params p = choose_params(ndim, nuni, M, type == 1, epsilon, sigma_min,
                              sigma_max, periodicity, fft_order != 0, vf64,
                              opt);
-    // the whole requirement is checked before anything is queued
+    // regions flushed with atomics (storing them would need far more
+    // scratch); the whole requirement is checked before anything is queued
+    p.slab_flush = false;
     const size_t need = Plan::scratch_bytes(p, M);
     if (need > (size_t)DMX_NOALLOC_SCRATCH_BYTES)
       throw Error("dmx_nufft_noalloc: this request needs " +

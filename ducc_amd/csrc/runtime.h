@@ -141,7 +141,13 @@ struct Params {
   // FP32 spreading in packed fixed point (eps >= FIX_MIN_EPS, no
   // DMX_SPREAD_FP64_ACC)
   bool spread_fix = false;
+  // type 1 stores each subproblem's region and folds the overlaps into the
+  // grid (no grid zeroing, no global atomics); 3-D
+  bool slab_flush = false;
 };
+// the largest region store the default spreader takes; larger requests
+// flush with atomics
+constexpr size_t SLAB_MAX_BYTES = (size_t)16 << 30;
 // the smallest eps for which FP32 spreading accumulates in fixed point
 constexpr double FIX_MIN_EPS = 1e-5;
 
@@ -255,6 +261,7 @@ class Plan {
   std::vector<std::pair<std::string, double>> timed_;  // phases, this call
   bool done_ = false;
   bool grid_ready_ = false;  // a *_grid call enqueued the grid side
+  bool grid_zeroed_ = true;  // nu2u_grid zeroed the grid
   std::string tag_ = "plan";  // "index", "nu2u" or "u2nu": phase prefix
   void prepare_grid_side();
   // how the FFT along real axis d runs: in LDS, as the two-pass 2-D column

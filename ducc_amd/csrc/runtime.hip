@@ -391,6 +391,11 @@ void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt) {
   size_t ntot = 1;
   for (int d = 0; d < ndim; ++d) ntot *= ((p.nover[d] >> l2t) + 3);
   DMX_ASSERT(ntot < 0xFFFFFFFFull, "oversampled grid too large");
+  // 3-D regions are (tile + W - 1)^3 cells around tile^3: flushing them
+  // with atomics costs about half of the spreading, storing them and
+  // folding the overlaps costs a fraction of that
+  p.slab_flush = ndim == 3 && p.use_lds && !p.deterministic &&
+                 !opt.deterministic;
 }
 
 Params kernel_params(int ndim, const size_t *nuni, const double *period,
@@ -952,8 +957,12 @@ bool Plan::run_fft(bool forward, bool type1, const GridCopyArgs *crop) {
 void Plan::nu2u_grid() {
   tag_ = "nu2u";
   prepare_grid_side();
-  Phase ph(this, "zeroing grid");
-  launch_zero(grid_.p, grid_.bytes, s_);
+  // with stored regions the fold writes every grid cell
+  grid_zeroed_ = !p_.slab_flush;
+  if (grid_zeroed_) {
+    Phase ph(this, "zeroing grid");
+    launch_zero(grid_.p, grid_.bytes, s_);
+  }
   grid_ready_ = true;
 }
 
@@ -970,16 +979,28 @@ void Plan::nu2u(bool forward, const void *points, bool pts_f64, void *uni,
     a.values = points;
     a.val_f64 = pts_f64 ? 1 : 0;
     a.grid = grid_.p;
+    const size_t rb =
+        p_.slab_flush ? spread_region_slab_bytes(a, p_.fp64) : 0;
     if (p_.deterministic) {
       const size_t sb = spread_det_slab_bytes(a);
       slab_.stream = s_;
       if (slab_.bytes != sb) slab_.reset(sb);
       a.slab = slab_.p;
       launch_spread_det(p_.fp64, a, s_);  // writes every grid cell
+    } else if (p_.slab_flush && rb <= SLAB_MAX_BYTES) {
+      slab_.stream = s_;
+      if (slab_.bytes != rb) slab_.reset(rb);
+      a.slab = slab_.p;
+      a.slab_out = 1;
+      launch_spread(p_.fp64, a, s_);
+      launch_fold_regions(p_.fp64, a, s_);  // writes every grid cell
     } else {
+      if (!grid_zeroed_) launch_zero(grid_.p, grid_.bytes, s_);
       launch_spread(p_.fp64, a, s_);
     }
     check_offsets();
+  } else if (!grid_zeroed_) {
+    launch_zero(grid_.p, grid_.bytes, s_);
   }
   GridCopyArgs c = copy_args(uni, uni_f64, ust);
   if (run_fft(forward, true, &c)) return;  // the last FFT pass cropped

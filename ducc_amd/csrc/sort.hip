@@ -254,7 +254,7 @@ __global__ void k_fill_subs(const uint32_t *counts, const uint32_t *tstart,
     sb.tile = (uint32_t)t;
     sb.start = ts + s * MAXSUB;
     sb.end = min(te, sb.start + (uint32_t)MAXSUB);
-    sb.pad_ = 0;
+    sb.rank = s;
     subs[s0 + s] = sb;
   }
 }
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(ST) void k_tile_subs_one(
         sb.tile = (uint32_t)t;
         sb.start = st[k] + q * MAXSUB;
         sb.end = min(st[k] + cn[k], sb.start + (uint32_t)MAXSUB);
-        sb.pad_ = 0;
+        sb.rank = q;
         subs[ex + q] = sb;
       }
       ex += ns[k];

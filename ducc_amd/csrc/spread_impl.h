@@ -207,400 +207,445 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   T *grid = (T *)a.grid;
   const T *cf = (const T *)a.coef;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (uint32_t sidx = blockIdx.x; sidx < total; sidx += gridDim.x) {
-    const Sub sb = a.subs[sidx];
-    int b0[3];
-    tile_origin(a.g, sb.tile, b0);
-    if constexpr (LDS) {
-      for (int c = tid; c < ncell; c += blockDim.x) {
-        if constexpr (FIX) {
-          sfx[c] = 0ull;
-        } else {
-          sre[c] = Acc(0);
-          sim[c] = Acc(0);
-        }
-      }
-      float fscale = 1.f, finv = 1.f;  // FIX: this subproblem's scale
-      bool fbad = false;  // FIX: a value of the subproblem is not finite
-      constexpr int SC = SpreadStash<T, NDIM, FIX>::SC, PT = SC / 256;
-      for (uint32_t c0 = sb.start; c0 < sb.end; c0 += SC) {
-        const int nc = (int)min((uint32_t)SC, sb.end - c0);
-        const int K = (nc + 63) >> 6;            // run length
-        const uint32_t kmul = (65536u + K - 1) / K;  // pos / K == pos*kmul >> 16
-        if (tid < 16) st.cnt[tid] = 0;
-        if constexpr (FIX) {
-          if (tid == 0) {
-            fx.vmaxb = 0u;
-            fx.vsum = 0.f;
+  for (uint32_t sidx0 = blockIdx.x; sidx0 < total; sidx0 += gridDim.x) {
+    // stored regions: a tile's first subproblem brings all of the tile's
+    // subproblems to this workgroup, the others are skipped
+    uint32_t nq = 1;
+    if (LDS && a.slab_out) {
+      const Sub s0 = a.subs[sidx0];
+      if (s0.rank != 0) continue;
+      nq = a.tile_nsub[s0.tile];
+    }
+    for (uint32_t q = 0; q < nq; ++q) {
+      const uint32_t sidx = sidx0 + q;
+      const Sub sb = a.subs[sidx];
+      int b0[3];
+      tile_origin(a.g, sb.tile, b0);
+      if constexpr (LDS) {
+        for (int c = tid; c < ncell; c += blockDim.x) {
+          if constexpr (FIX) {
+            sfx[c] = 0ull;
+          } else {
+            sre[c] = Acc(0);
+            sim[c] = Acc(0);
           }
-          if (tid < FIX_MAXBLK) fx.blk[tid] = 0u;
         }
-        // the previous chunk's phase 2 is done with the stash (and the
-        // accumulators are zero before the first chunk)
-        __syncthreads();
-        T xs[PT][NDIM];
-        uint32_t os[PT], rnk[PT];
-        int cls[PT];
-        T vr_[PT], vi_[PT];
-        if (a.rec) {
-          // records in tile order: offsets, kernel arguments and values
-          // read in sequence
-          const int L = a.g.log2tile;
-          const uint32_t tm = (1u << L) - 1u;
-#pragma unroll
-          for (int k = 0; k < PT; ++k) {
-            const int i = tid + 256 * k;
-            os[k] = 0;
-            if (i < nc) {
-              const uint32_t p = c0 + i;
-              const uint32_t w = a.recs.word[p];
-              uint32_t o = 0;
-#pragma unroll
-              for (int d = 0; d < NDIM; ++d) {
-                o |= ((w >> (d * L)) & tm) << (10 * d);
-                xs[k][d] = ((const T *)a.recs.x[d])[p];
-              }
-              os[k] = o;
-              const typename cplx2<T>::type v =
-                  ((const typename cplx2<T>::type *)a.recs.v)[p];
-              vr_[k] = v.x;
-              vi_[k] = v.y;
+        float fscale = 1.f, finv = 1.f;  // FIX: this subproblem's scale
+        bool fbad = false;  // FIX: a value of the subproblem is not finite
+        constexpr int SC = SpreadStash<T, NDIM, FIX>::SC, PT = SC / 256;
+        for (uint32_t c0 = sb.start; c0 < sb.end; c0 += SC) {
+          const int nc = (int)min((uint32_t)SC, sb.end - c0);
+          const int K = (nc + 63) >> 6;            // run length
+          // pos / K == pos * kmul >> 16
+          const uint32_t kmul = (65536u + K - 1) / K;
+          if (tid < 16) st.cnt[tid] = 0;
+          if constexpr (FIX) {
+            if (tid == 0) {
+              fx.vmaxb = 0u;
+              fx.vsum = 0.f;
             }
+            if (tid < FIX_MAXBLK) fx.blk[tid] = 0u;
           }
-        } else {
-          // loads of all this thread's points first (memory-level
-          // parallelism)
-          uint32_t jj[PT];
-          double cc[PT][NDIM];
-#pragma unroll
-          for (int k = 0; k < PT; ++k) {
-            const int i = tid + 256 * k;
-            jj[k] = i < nc ? a.idx[c0 + i] : 0u;
-          }
-#pragma unroll
-          for (int k = 0; k < PT; ++k) {
-            const int i = tid + 256 * k;
-            if (i < nc) {
-              const uint32_t p = c0 + i;
-              const size_t cj = a.coords_sorted ? (size_t)p : (size_t)jj[k];
-              load_point(a.g, a.coords, cj, NDIM, cc[k]);
-              load_value<T>(a.values,
-                            a.values_sorted ? (size_t)p : (size_t)jj[k],
-                            a.val_f64, vr_[k], vi_[k]);
-            }
-          }
-#pragma unroll
-          for (int k = 0; k < PT; ++k) {
-            const int i = tid + 256 * k;
-            os[k] = 0;
-            if (i < nc) {
-              uint32_t o = 0;
-#pragma unroll
-              for (int d = 0; d < NDIM; ++d) {
-                double frac;
-                const int i0 = fold_axis(a.g, d, cc[k][d], frac);
-                xs[k][d] = (T)((double)(W - 1) - 2.0 * frac);
-                int od = i0 - b0[d];
-                if (od < 0 || od >= tile) {
-                  if (a.offcheck) atomicAdd(a.offcheck, 1u);
-                  od = od < 0 ? 0 : tile - 1;
-                }
-                o |= (uint32_t)od << (10 * d);
-              }
-              os[k] = o;
-            }
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < PT; ++k) {
-          const int i = tid + 256 * k;
-          cls[k] = 0;
-          rnk[k] = 0;
-          if (i < nc) {
-            cls[k] = region_cell<T, NDIM>(os[k], S) & 15;
-            rnk[k] = atomicAdd(&st.cnt[cls[k]], 1u);
-          }
-        }
-        float amax[FIX ? PT : 1];
-        if constexpr (FIX) {
-          float vs = 0.f, vm = 0.f;
-#pragma unroll
-          for (int k = 0; k < PT; ++k) {
-            amax[k] = 0.f;
-            if (tid + 256 * k < nc) {
-              const float ar = fabsf((float)vr_[k]), ai = fabsf((float)vi_[k]);
-              // fmaxf ignores NaN: a NaN part counts as infinite
-              amax[k] = (ar != ar || ai != ai) ? __builtin_inff()
-                                               : fmaxf(ar, ai);
-            }
-            vs += amax[k];
-            vm = fmaxf(vm, amax[k]);
-          }
-#pragma unroll
-          for (int o = 32; o > 0; o >>= 1) {
-            vs += __shfl_xor(vs, o);
-            vm = fmaxf(vm, __shfl_xor(vm, o));
-          }
-          if (lane == 0) {
-            atomicAdd(&fx.vsum, vs);
-            atomicMax(&fx.vmaxb, __float_as_uint(vm));
-          }
-        }
-        __syncthreads();
-        if constexpr (FIX) {
-          // each point's quantized magnitude into the blocks it touches
-          const float vmax = __uint_as_float(fx.vmaxb);
-          if (nblk <= FIX_MAXBLK && vmax > 0.f) {
+          // the previous chunk's phase 2 is done with the stash (and the
+          // accumulators are zero before the first chunk)
+          __syncthreads();
+          T xs[PT][NDIM];
+          uint32_t os[PT], rnk[PT];
+          int cls[PT];
+          T vr_[PT], vi_[PT];
+          if (a.rec) {
+            // records in tile order: offsets, kernel arguments and values
+            // read in sequence
+            const int L = a.g.log2tile;
+            const uint32_t tm = (1u << L) - 1u;
 #pragma unroll
             for (int k = 0; k < PT; ++k) {
-              if (tid + 256 * k >= nc) continue;
-              const uint32_t q =
-                  (uint32_t)ceilf(amax[k] / vmax * 1048576.f);
-              int blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+              const int i = tid + 256 * k;
+              os[k] = 0;
+              if (i < nc) {
+                const uint32_t p = c0 + i;
+                const uint32_t w = a.recs.word[p];
+                uint32_t o = 0;
 #pragma unroll
-              for (int d = 0; d < NDIM; ++d) {
-                const int od = (int)((os[k] >> (10 * d)) & 1023u);
-                blo[d] = od / FIX_BLK;
-                bhi[d] = (od + W - 1) / FIX_BLK;
+                for (int d = 0; d < NDIM; ++d) {
+                  o |= ((w >> (d * L)) & tm) << (10 * d);
+                  xs[k][d] = ((const T *)a.recs.x[d])[p];
+                }
+                os[k] = o;
+                const typename cplx2<T>::type v =
+                    ((const typename cplx2<T>::type *)a.recs.v)[p];
+                vr_[k] = v.x;
+                vi_[k] = v.y;
               }
-              for (int b0_ = blo[0]; b0_ <= bhi[0]; ++b0_)
-                for (int b1_ = blo[1]; b1_ <= bhi[1]; ++b1_)
-                  for (int b2_ = blo[2]; b2_ <= bhi[2]; ++b2_) {
-                    int bi = b0_;
-                    if (NDIM > 1) bi = bi * nbx + b1_;
-                    if (NDIM > 2) bi = bi * nbx + b2_;
-                    atomicAdd(&fx.blk[bi], q);
+            }
+          } else {
+            // loads of all this thread's points first (memory-level
+            // parallelism)
+            uint32_t jj[PT];
+            double cc[PT][NDIM];
+#pragma unroll
+            for (int k = 0; k < PT; ++k) {
+              const int i = tid + 256 * k;
+              jj[k] = i < nc ? a.idx[c0 + i] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < PT; ++k) {
+              const int i = tid + 256 * k;
+              if (i < nc) {
+                const uint32_t p = c0 + i;
+                const size_t cj = a.coords_sorted ? (size_t)p : (size_t)jj[k];
+                load_point(a.g, a.coords, cj, NDIM, cc[k]);
+                load_value<T>(a.values,
+                              a.values_sorted ? (size_t)p : (size_t)jj[k],
+                              a.val_f64, vr_[k], vi_[k]);
+              }
+            }
+#pragma unroll
+            for (int k = 0; k < PT; ++k) {
+              const int i = tid + 256 * k;
+              os[k] = 0;
+              if (i < nc) {
+                uint32_t o = 0;
+#pragma unroll
+                for (int d = 0; d < NDIM; ++d) {
+                  double frac;
+                  const int i0 = fold_axis(a.g, d, cc[k][d], frac);
+                  xs[k][d] = (T)((double)(W - 1) - 2.0 * frac);
+                  int od = i0 - b0[d];
+                  if (od < 0 || od >= tile) {
+                    if (a.offcheck) atomicAdd(a.offcheck, 1u);
+                    od = od < 0 ? 0 : tile - 1;
                   }
+                  o |= (uint32_t)od << (10 * d);
+                }
+                os[k] = o;
+              }
             }
           }
-        }
-        if (tid < 16) {
-          uint32_t run = 0;
-          for (int k = 0; k < tid; ++k) run += st.cnt[k];
-          st.off[tid] = run;
-        }
-        __syncthreads();
-        if constexpr (FIX) {
-          // wave 0: the bound, and the scale that maps it to 0.999 * 2^31
-          // (published by the barrier before phase 2)
-          if (wave == 0) {
-            const float vmax = __uint_as_float(fx.vmaxb);
-            double bound = (double)fx.vsum;
-            if (nblk <= FIX_MAXBLK) {
-              uint32_t m = 0u;
-              for (int b = lane; b < nblk; b += 64) m = max(m, fx.blk[b]);
 #pragma unroll
-              for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
-              bound = (double)m * (double)vmax * (1.0 / 1048576.0);
-            }
-            bound *= (double)FIX_WMAX;
-            if (lane == 0) {
-              const double sc = bound > 0.0 ? 0.999 * 2147483648.0 / bound
-                                            : 1.0;
-              fx.scale = (float)fmin(sc, 1e30);
-              fx.inv = (float)(1.0 / (double)fx.scale);
+          for (int k = 0; k < PT; ++k) {
+            const int i = tid + 256 * k;
+            cls[k] = 0;
+            rnk[k] = 0;
+            if (i < nc) {
+              cls[k] = region_cell<T, NDIM>(os[k], S) & 15;
+              rnk[k] = atomicAdd(&st.cnt[cls[k]], 1u);
             }
           }
-        }
-#pragma unroll
-        for (int k = 0; k < PT; ++k) {
-          const int i = tid + 256 * k;
-          if (i < nc) {
-            const uint32_t pos = st.off[cls[k]] + rnk[k];
-            const uint32_t r = (pos * kmul) >> 16, jr = pos - r * K;
-            const int slot = (int)(jr * 64 + r);
-#pragma unroll
-            for (int d = 0; d < NDIM; ++d) st.x[d][slot] = xs[k][d];
-            st.o[slot] = os[k];
-            st.v[slot].x = vr_[k];
-            st.v[slot].y = vi_[k];
-          }
-        }
-        __syncthreads();
-        if constexpr (FIX) {
-          fscale = fx.scale;
-          finv = fx.inv;
-          fbad = !(__uint_as_float(fx.vmaxb) <= 3.4028235e38f);
-        }
-        const int run = ((lane & 15) << 2) | (lane >> 4);
-        // 3-D tiles of few points (runs shorter than the four waves): every
-        // wave takes every run and a quarter of each point's W planes, so
-        // all waves work; the planes shift all lanes alike, which keeps
-        // the lanes of a quarter-wave on distinct classes
-        const bool split = NDIM == 3 && K < 4;
-        const int jr0 = split ? 0 : wave, jstep = split ? 1 : 4;
-        const int q00 = split ? wave : 0, qstep = split ? 4 : 1;
-        for (int jr = jr0; jr < K; jr += jstep) {
-          if (run * K + jr >= nc) continue;
-          const int i = jr * 64 + run;
-          const uint32_t o = st.o[i];
-          T vr = st.v[i].x, vi = st.v[i].y;
-          T w[NDIM][W];
-#pragma unroll
-          for (int d = 0; d < NDIM; ++d)
-            kernel_weights<T, W>(st.x[d][i], cf, w[d]);
-          const int cell0 = region_cell<T, NDIM>(o, S);
+          float amax[FIX ? PT : 1];
           if constexpr (FIX) {
-            vr *= fscale;
-            vi *= fscale;
-            if constexpr (NDIM == 1) {
+            float vs = 0.f, vm = 0.f;
 #pragma unroll
-              for (int q0 = 0; q0 < W; ++q0)
-                atomicAdd(&sfx[cell0 + q0],
-                          fix_pack(vr * w[0][q0], vi * w[0][q0]));
+            for (int k = 0; k < PT; ++k) {
+              amax[k] = 0.f;
+              if (tid + 256 * k < nc) {
+                const float ar = fabsf((float)vr_[k]),
+                            ai = fabsf((float)vi_[k]);
+                // fmaxf ignores NaN: a NaN part counts as infinite
+                amax[k] = (ar != ar || ai != ai) ? __builtin_inff()
+                                                 : fmaxf(ar, ai);
+              }
+              vs += amax[k];
+              vm = fmaxf(vm, amax[k]);
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+              vs += __shfl_xor(vs, o);
+              vm = fmaxf(vm, __shfl_xor(vm, o));
+            }
+            if (lane == 0) {
+              atomicAdd(&fx.vsum, vs);
+              atomicMax(&fx.vmaxb, __float_as_uint(vm));
+            }
+          }
+          __syncthreads();
+          if constexpr (FIX) {
+            // each point's quantized magnitude into the blocks it touches
+            const float vmax = __uint_as_float(fx.vmaxb);
+            if (nblk <= FIX_MAXBLK && vmax > 0.f) {
+#pragma unroll
+              for (int k = 0; k < PT; ++k) {
+                if (tid + 256 * k >= nc) continue;
+                const uint32_t q =
+                    (uint32_t)ceilf(amax[k] / vmax * 1048576.f);
+                int blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+#pragma unroll
+                for (int d = 0; d < NDIM; ++d) {
+                  const int od = (int)((os[k] >> (10 * d)) & 1023u);
+                  blo[d] = od / FIX_BLK;
+                  bhi[d] = (od + W - 1) / FIX_BLK;
+                }
+                for (int b0_ = blo[0]; b0_ <= bhi[0]; ++b0_)
+                  for (int b1_ = blo[1]; b1_ <= bhi[1]; ++b1_)
+                    for (int b2_ = blo[2]; b2_ <= bhi[2]; ++b2_) {
+                      int bi = b0_;
+                      if (NDIM > 1) bi = bi * nbx + b1_;
+                      if (NDIM > 2) bi = bi * nbx + b2_;
+                      atomicAdd(&fx.blk[bi], q);
+                    }
+              }
+            }
+          }
+          if (tid < 16) {
+            uint32_t run = 0;
+            for (int k = 0; k < tid; ++k) run += st.cnt[k];
+            st.off[tid] = run;
+          }
+          __syncthreads();
+          if constexpr (FIX) {
+            // wave 0: the bound, and the scale that maps it to 0.999 * 2^31
+            // (published by the barrier before phase 2)
+            if (wave == 0) {
+              const float vmax = __uint_as_float(fx.vmaxb);
+              double bound = (double)fx.vsum;
+              if (nblk <= FIX_MAXBLK) {
+                uint32_t m = 0u;
+                for (int b = lane; b < nblk; b += 64) m = max(m, fx.blk[b]);
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+                bound = (double)m * (double)vmax * (1.0 / 1048576.0);
+              }
+              bound *= (double)FIX_WMAX;
+              if (lane == 0) {
+                const double sc = bound > 0.0 ? 0.999 * 2147483648.0 / bound
+                                              : 1.0;
+                fx.scale = (float)fmin(sc, 1e30);
+                fx.inv = (float)(1.0 / (double)fx.scale);
+              }
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < PT; ++k) {
+            const int i = tid + 256 * k;
+            if (i < nc) {
+              const uint32_t pos = st.off[cls[k]] + rnk[k];
+              const uint32_t r = (pos * kmul) >> 16, jr = pos - r * K;
+              const int slot = (int)(jr * 64 + r);
+#pragma unroll
+              for (int d = 0; d < NDIM; ++d) st.x[d][slot] = xs[k][d];
+              st.o[slot] = os[k];
+              st.v[slot].x = vr_[k];
+              st.v[slot].y = vi_[k];
+            }
+          }
+          __syncthreads();
+          if constexpr (FIX) {
+            fscale = fx.scale;
+            finv = fx.inv;
+            fbad = !(__uint_as_float(fx.vmaxb) <= 3.4028235e38f);
+          }
+          const int run = ((lane & 15) << 2) | (lane >> 4);
+          // 3-D tiles of few points (runs shorter than the four waves): every
+          // wave takes every run and a quarter of each point's W planes, so
+          // all waves work; the planes shift all lanes alike, which keeps
+          // the lanes of a quarter-wave on distinct classes
+          const bool split = NDIM == 3 && K < 4;
+          const int jr0 = split ? 0 : wave, jstep = split ? 1 : 4;
+          const int q00 = split ? wave : 0, qstep = split ? 4 : 1;
+          for (int jr = jr0; jr < K; jr += jstep) {
+            if (run * K + jr >= nc) continue;
+            const int i = jr * 64 + run;
+            const uint32_t o = st.o[i];
+            T vr = st.v[i].x, vi = st.v[i].y;
+            T w[NDIM][W];
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d)
+              kernel_weights<T, W>(st.x[d][i], cf, w[d]);
+            const int cell0 = region_cell<T, NDIM>(o, S);
+            if constexpr (FIX) {
+              vr *= fscale;
+              vi *= fscale;
+              if constexpr (NDIM == 1) {
+#pragma unroll
+                for (int q0 = 0; q0 < W; ++q0)
+                  atomicAdd(&sfx[cell0 + q0],
+                            fix_pack(vr * w[0][q0], vi * w[0][q0]));
+              } else if constexpr (NDIM == 2) {
+#pragma unroll
+                for (int q0 = 0; q0 < W; ++q0) {
+                  const T fr = vr * w[0][q0], fi = vi * w[0][q0];
+                  const int row = cell0 + q0 * S;
+#pragma unroll
+                  for (int q1 = 0; q1 < W; ++q1)
+                    atomicAdd(&sfx[row + q1],
+                              fix_pack(fr * w[1][q1], fi * w[1][q1]));
+                }
+              } else {
+#pragma unroll 1
+                for (int q0 = q00; q0 < W; q0 += qstep) {
+                  T w0 = w[0][0];
+#pragma unroll
+                  for (int k = 1; k < W; ++k)
+                    if (k == q0) w0 = w[0][k];
+#pragma unroll
+                  for (int q1 = 0; q1 < W; ++q1) {
+                    const T f01 = w0 * w[1][q1];
+                    const T fr = vr * f01, fi = vi * f01;
+                    const int row = cell0 + (q0 * S + q1) * S;
+#pragma unroll
+                    for (int q2 = 0; q2 < W; ++q2)
+                      atomicAdd(&sfx[row + q2],
+                                fix_pack(fr * w[2][q2], fi * w[2][q2]));
+                  }
+                }
+              }
+            } else if constexpr (NDIM == 1) {
+#pragma unroll
+              for (int q0 = 0; q0 < W; ++q0) {
+                atomicAdd(&sre[cell0 + q0], (Acc)(vr * w[0][q0]));
+                atomicAdd(&sim[cell0 + q0], (Acc)(vi * w[0][q0]));
+              }
             } else if constexpr (NDIM == 2) {
 #pragma unroll
               for (int q0 = 0; q0 < W; ++q0) {
                 const T fr = vr * w[0][q0], fi = vi * w[0][q0];
                 const int row = cell0 + q0 * S;
 #pragma unroll
-                for (int q1 = 0; q1 < W; ++q1)
-                  atomicAdd(&sfx[row + q1],
-                            fix_pack(fr * w[1][q1], fi * w[1][q1]));
-              }
-            } else {
-#pragma unroll 1
-              for (int q0 = q00; q0 < W; q0 += qstep) {
-                T w0 = w[0][0];
-#pragma unroll
-                for (int k = 1; k < W; ++k)
-                  if (k == q0) w0 = w[0][k];
-#pragma unroll
                 for (int q1 = 0; q1 < W; ++q1) {
-                  const T f01 = w0 * w[1][q1];
-                  const T fr = vr * f01, fi = vi * f01;
-                  const int row = cell0 + (q0 * S + q1) * S;
-#pragma unroll
-                  for (int q2 = 0; q2 < W; ++q2)
-                    atomicAdd(&sfx[row + q2],
-                              fix_pack(fr * w[2][q2], fi * w[2][q2]));
+                  atomicAdd(&sre[row + q1], (Acc)(fr * w[1][q1]));
+                  atomicAdd(&sim[row + q1], (Acc)(fi * w[1][q1]));
                 }
               }
-            }
-          } else if constexpr (NDIM == 1) {
-#pragma unroll
-            for (int q0 = 0; q0 < W; ++q0) {
-              atomicAdd(&sre[cell0 + q0], (Acc)(vr * w[0][q0]));
-              atomicAdd(&sim[cell0 + q0], (Acc)(vi * w[0][q0]));
-            }
-          } else if constexpr (NDIM == 2) {
-#pragma unroll
-            for (int q0 = 0; q0 < W; ++q0) {
-              const T fr = vr * w[0][q0], fi = vi * w[0][q0];
-              const int row = cell0 + q0 * S;
-#pragma unroll
-              for (int q1 = 0; q1 < W; ++q1) {
-                atomicAdd(&sre[row + q1], (Acc)(fr * w[1][q1]));
-                atomicAdd(&sim[row + q1], (Acc)(fi * w[1][q1]));
-              }
-            }
-          } else {
-            if (W <= 8 && qstep == 1) {
-#pragma unroll
-              for (int q0 = 0; q0 < W; ++q0)
-#pragma unroll
-                for (int q1 = 0; q1 < W; ++q1) {
-                  const T f01 = w[0][q0] * w[1][q1];
-                  spread_row3<T, W>(sre, sim, cell0 + (q0 * S + q1) * S,
-                                    vr * f01, vi * f01, w[2]);
-                }
             } else {
+              if (W <= 8 && qstep == 1) {
+#pragma unroll
+                for (int q0 = 0; q0 < W; ++q0)
+#pragma unroll
+                  for (int q1 = 0; q1 < W; ++q1) {
+                    const T f01 = w[0][q0] * w[1][q1];
+                    spread_row3<T, W>(sre, sim, cell0 + (q0 * S + q1) * S,
+                                      vr * f01, vi * f01, w[2]);
+                  }
+              } else {
 #pragma unroll 1
-              for (int q0 = q00; q0 < W; q0 += qstep) {
-                T w0 = w[0][0];
+                for (int q0 = q00; q0 < W; q0 += qstep) {
+                  T w0 = w[0][0];
 #pragma unroll
-                for (int k = 1; k < W; ++k)
-                  if (k == q0) w0 = w[0][k];
+                  for (int k = 1; k < W; ++k)
+                    if (k == q0) w0 = w[0][k];
 #pragma unroll
-                for (int q1 = 0; q1 < W; ++q1) {
-                  const T f01 = w0 * w[1][q1];
-                  spread_row3<T, W>(sre, sim, cell0 + (q0 * S + q1) * S,
-                                    vr * f01, vi * f01, w[2]);
+                  for (int q1 = 0; q1 < W; ++q1) {
+                    const T f01 = w0 * w[1][q1];
+                    spread_row3<T, W>(sre, sim, cell0 + (q0 * S + q1) * S,
+                                      vr * f01, vi * f01, w[2]);
+                  }
                 }
               }
             }
           }
         }
-      }
-      __syncthreads();
-      const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
-                n2 = NDIM > 2 ? a.g.nover[2] : 1;
-      if (FIX && fbad) {
-        // a NaN or infinite value: the integer sums mean nothing, and the
-        // FFT turns one NaN grid cell into NaN everywhere, as FP64
-        // accumulation of such a value would
-        if (tid == 0) {
-          size_t gi = (size_t)wrapi(b0[0], n0);
-          if (NDIM > 1) gi = gi * n1 + wrapi(b0[1], n1);
-          if (NDIM > 2) gi = gi * n2 + wrapi(b0[2], n2);
-          atomicAdd(&grid[2 * gi], (T)__builtin_nanf(""));
+        __syncthreads();
+        if (a.slab_out) {
+          // the whole region to the tile's slab (the first subproblem stores
+          // it, later ones add to it: each thread revisits the cells it
+          // wrote); the fold adds the overlaps of neighbouring tiles.  A
+          // non-finite value makes every cell NaN.
+          typename cplx2<T>::type *dst =
+              (typename cplx2<T>::type *)a.slab + (size_t)sb.tile * ncell;
+          for (int c = tid; c < ncell; c += blockDim.x) {
+            T re, im;
+            if constexpr (FIX) {
+              int ri, ii;
+              fix_unpack(sfx[c], ri, ii);
+              re = fbad ? (T)__builtin_nanf("") : (float)ri * finv;
+              im = fbad ? (T)__builtin_nanf("") : (float)ii * finv;
+            } else {
+              re = (T)sre[c];
+              im = (T)sim[c];
+            }
+            typename cplx2<T>::type v;
+            if (sb.rank == 0) {
+              v.x = re;
+              v.y = im;
+            } else {
+              v = dst[c];
+              v.x += re;
+              v.y += im;
+            }
+            dst[c] = v;
+          }
+          __syncthreads();
+          continue;
         }
-      }
-      for (int c = tid; c < ncell && !(FIX && fbad); c += blockDim.x) {
-        T re, im;
-        if constexpr (FIX) {
-          const unsigned long long t = sfx[c];
-          if (t == 0ull) continue;
-          int ri, ii;
-          fix_unpack(t, ri, ii);
-          re = (float)ri * finv;
-          im = (float)ii * finv;
-        } else {
-          re = (T)sre[c];
-          im = (T)sim[c];
+        const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
+                  n2 = NDIM > 2 ? a.g.nover[2] : 1;
+        if (FIX && fbad) {
+          // a NaN or infinite value: the integer sums mean nothing, and the
+          // FFT turns one NaN grid cell into NaN everywhere, as FP64
+          // accumulation of such a value would
+          if (tid == 0) {
+            size_t gi = (size_t)wrapi(b0[0], n0);
+            if (NDIM > 1) gi = gi * n1 + wrapi(b0[1], n1);
+            if (NDIM > 2) gi = gi * n2 + wrapi(b0[2], n2);
+            atomicAdd(&grid[2 * gi], (T)__builtin_nanf(""));
+          }
         }
-        if (re == T(0) && im == T(0)) continue;
-        size_t gi;
-        if constexpr (NDIM == 1) {
-          gi = (size_t)wrapi(b0[0] + c, n0);
-        } else if constexpr (NDIM == 2) {
-          const int cu = c / S, cv = c - cu * S;
-          gi = (size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1);
-        } else {
-          const int cu = c / (S * S), r = c - cu * S * S, cv = r / S,
-                    cw = r - cv * S;
-          gi = ((size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1)) *
-                   n2 +
-               wrapi(b0[2] + cw, n2);
-        }
-        atomicAdd(&grid[2 * gi], re);
-        atomicAdd(&grid[2 * gi + 1], im);
-      }
-      __syncthreads();
-    } else {
-      // direct HBM accumulation with periodic wrap (regions beyond LDS)
-      const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
-                n2 = NDIM > 2 ? a.g.nover[2] : 1;
-      for (uint32_t p = sb.start + tid; p < sb.end; p += blockDim.x) {
-        uint32_t j;
-        int i0[3];
-        T w[NDIM][W];
-        point_setup<T, NDIM, W>(a, p, j, i0, w);
-        T vr, vi;
-        load_value<T>(a.values, a.values_sorted ? (size_t)p : (size_t)j,
-                      a.val_f64, vr, vi);
-#pragma unroll 1
-        for (int c0 = 0; c0 < W; ++c0) {
-          const int g0 = wrapi(i0[0] + c0, n0);
-          if constexpr (NDIM == 1) {
-            atomicAdd(&grid[2 * (size_t)g0], vr * w[0][c0]);
-            atomicAdd(&grid[2 * (size_t)g0 + 1], vi * w[0][c0]);
+        for (int c = tid; c < ncell && !(FIX && fbad); c += blockDim.x) {
+          T re, im;
+          if constexpr (FIX) {
+            const unsigned long long t = sfx[c];
+            if (t == 0ull) continue;
+            int ri, ii;
+            fix_unpack(t, ri, ii);
+            re = (float)ri * finv;
+            im = (float)ii * finv;
           } else {
+            re = (T)sre[c];
+            im = (T)sim[c];
+          }
+          if (re == T(0) && im == T(0)) continue;
+          size_t gi;
+          if constexpr (NDIM == 1) {
+            gi = (size_t)wrapi(b0[0] + c, n0);
+          } else if constexpr (NDIM == 2) {
+            const int cu = c / S, cv = c - cu * S;
+            gi = (size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1);
+          } else {
+            const int cu = c / (S * S), r = c - cu * S * S, cv = r / S,
+                      cw = r - cv * S;
+            gi = ((size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1)) *
+                     n2 +
+                 wrapi(b0[2] + cw, n2);
+          }
+          atomicAdd(&grid[2 * gi], re);
+          atomicAdd(&grid[2 * gi + 1], im);
+        }
+        __syncthreads();
+      } else {
+        // direct HBM accumulation with periodic wrap (regions beyond LDS)
+        const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
+                  n2 = NDIM > 2 ? a.g.nover[2] : 1;
+        for (uint32_t p = sb.start + tid; p < sb.end; p += blockDim.x) {
+          uint32_t j;
+          int i0[3];
+          T w[NDIM][W];
+          point_setup<T, NDIM, W>(a, p, j, i0, w);
+          T vr, vi;
+          load_value<T>(a.values, a.values_sorted ? (size_t)p : (size_t)j,
+                        a.val_f64, vr, vi);
 #pragma unroll 1
-            for (int c1 = 0; c1 < W; ++c1) {
-              const int g1 = wrapi(i0[1] + c1, n1);
-              const T f01 = w[0][c0] * w[1][c1];
-              if constexpr (NDIM == 2) {
-                size_t gi = (size_t)g0 * n1 + g1;
-                atomicAdd(&grid[2 * gi], vr * f01);
-                atomicAdd(&grid[2 * gi + 1], vi * f01);
-              } else {
-                const size_t rb = ((size_t)g0 * n1 + g1) * n2;
+          for (int c0 = 0; c0 < W; ++c0) {
+            const int g0 = wrapi(i0[0] + c0, n0);
+            if constexpr (NDIM == 1) {
+              atomicAdd(&grid[2 * (size_t)g0], vr * w[0][c0]);
+              atomicAdd(&grid[2 * (size_t)g0 + 1], vi * w[0][c0]);
+            } else {
+#pragma unroll 1
+              for (int c1 = 0; c1 < W; ++c1) {
+                const int g1 = wrapi(i0[1] + c1, n1);
+                const T f01 = w[0][c0] * w[1][c1];
+                if constexpr (NDIM == 2) {
+                  size_t gi = (size_t)g0 * n1 + g1;
+                  atomicAdd(&grid[2 * gi], vr * f01);
+                  atomicAdd(&grid[2 * gi + 1], vi * f01);
+                } else {
+                  const size_t rb = ((size_t)g0 * n1 + g1) * n2;
 #pragma unroll
-                for (int c2 = 0; c2 < W; ++c2) {
-                  const size_t gi = rb + wrapi(i0[2] + c2, n2);
-                  atomicAdd(&grid[2 * gi], vr * f01 * w[2][c2]);
-                  atomicAdd(&grid[2 * gi + 1], vi * f01 * w[2][c2]);
+                  for (int c2 = 0; c2 < W; ++c2) {
+                    const size_t gi = rb + wrapi(i0[2] + c2, n2);
+                    atomicAdd(&grid[2 * gi], vr * f01 * w[2][c2]);
+                    atomicAdd(&grid[2 * gi + 1], vi * f01 * w[2][c2]);
+                  }
                 }
               }
             }
