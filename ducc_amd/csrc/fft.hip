@@ -269,36 +269,62 @@ k_fft_lds(FftLaunch p) {
   const int nel = nlines * L;
   const float inv_L = 1.0f / (float)L, inv_nl = 1.0f / (float)nlines;
   // load (unrolled: several loads of a lane in flight before their LDS
-  // writes)
+  // writes); one line per workgroup needs no element -> (line, index)
+  // division
+  const int64_t lbase =
+      ubase + map_range(v0, p.vn1, p.va1, p.va2) * p.st_v;
+  if (nlines == 1) {
 #pragma unroll 4
-  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
-    int b, k;
-    if (p.st_ax == 1) b = fdivq(e, L, inv_L, k);
-    else k = fdivq(e, nlines, inv_nl, b);
-    C x;
-    if (p.kprune && k >= p.kn1 && k < p.ka2) {
-      x.x = T(0);
-      x.y = T(0);
-    } else {
-      const int64_t addr = ubase +
-                           map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v +
-                           k * p.st_ax;
-      x = data[addr];
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+      C x;
+      if (p.kprune && k >= p.kn1 && k < p.ka2) {
+        x.x = T(0);
+        x.y = T(0);
+      } else {
+        x = data[lbase + (int64_t)k * p.st_ax];
+      }
+      buf[k] = x;
     }
-    buf[b * LP + k] = x;
+  } else {
+#pragma unroll 4
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+      int b, k;
+      if (p.st_ax == 1) b = fdivq(e, L, inv_L, k);
+      else k = fdivq(e, nlines, inv_nl, b);
+      C x;
+      if (p.kprune && k >= p.kn1 && k < p.ka2) {
+        x.x = T(0);
+        x.y = T(0);
+      } else {
+        const int64_t addr = ubase +
+                             map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v +
+                             k * p.st_ax;
+        x = data[addr];
+      }
+      buf[b * LP + k] = x;
+    }
   }
   __syncthreads();
   lds_passes<T, MAXEL, SIGN>(buf, LP, nlines, p.npass, p.pass, tw);
   // store
+  if (nlines == 1) {
 #pragma unroll 4
-  for (int e = threadIdx.x; e < nel; e += blockDim.x) {
-    int b, k;
-    if (p.st_ax == 1) b = fdivq(e, L, inv_L, k);
-    else k = fdivq(e, nlines, inv_nl, b);
-    if (p.sprune && k >= p.sn1 && k < p.sa2) continue;
-    const int64_t addr =
-        ubase + map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v + k * p.st_ax;
-    data[addr] = buf[b * LP + k];
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+      if (p.sprune && k >= p.sn1 && k < p.sa2) continue;
+      data[lbase + (int64_t)k * p.st_ax] = buf[k];
+    }
+  } else {
+#pragma unroll 4
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+      int b, k;
+      if (p.st_ax == 1) b = fdivq(e, L, inv_L, k);
+      else k = fdivq(e, nlines, inv_nl, b);
+      if (p.sprune && k >= p.sn1 && k < p.sa2) continue;
+      const int64_t addr = ubase +
+                           map_range(v0 + b, p.vn1, p.va1, p.va2) * p.st_v +
+                           k * p.st_ax;
+      data[addr] = buf[b * LP + k];
+    }
   }
 }
 
