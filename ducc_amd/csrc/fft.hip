@@ -354,18 +354,23 @@ k_fft_cols(FftCols p) {
       map_range(blockIdx.z, p.on1, p.oa1, p.oa2) * p.st_o;
   const int nel = nb * Ls;
   const float inv_nb = 1.0f / (float)nb, inv_L = 1.0f / (float)p.L;
+  // element i = e * nb + b (row e, column b): a lane's first (e, b) from one
+  // division, the next ones (i + FFT_THREADS) by a constant step
+  int sb_;
+  const int se = fdivq(FFT_THREADS, nb, inv_nb, sb_);
+  int b_first;
+  const int e_first = fdivq((int)threadIdx.x, nb, inv_nb, b_first);
   // a lane's global loads are all issued before its LDS writes (nel < MAXEL)
   constexpr int NPER = (MAXEL + FFT_THREADS - 1) / FFT_THREADS;
   {
     C tmp[NPER];
     int slot[NPER];
+    int e = e_first, b = b_first;
 #pragma unroll
     for (int j = 0; j < NPER; ++j) {
       const int i = (int)threadIdx.x + j * FFT_THREADS;
       slot[j] = -1;
       if (i < nel) {
-        int b;
-        const int e = fdivq(i, nb, inv_nb, b);
         const int64_t row = (int64_t)u * p.ia + (int64_t)e * p.ib;
         slot[j] = b * LP + e;
         if (p.kprune && row >= p.kn1 && row < p.ka2) {
@@ -376,6 +381,12 @@ k_fft_cols(FftCols p) {
                        map_range(v0 + b, p.vn1, p.va1, p.va2)];
         }
       }
+      b += sb_;
+      e += se;
+      if (b >= nb) {
+        b -= nb;
+        ++e;
+      }
     }
 #pragma unroll
     for (int j = 0; j < NPER; ++j)
@@ -384,9 +395,16 @@ k_fft_cols(FftCols p) {
   __syncthreads();
   lds_passes<T, MAXEL, SIGN>(buf, LP, nb, p.npass, p.pass, tws);
   const C *twf = (const C *)p.tw_full;
-  for (int i = threadIdx.x; i < nel; i += blockDim.x) {
-    int b;
-    const int k = fdivq(i, nb, inv_nb, b);
+  int kn = e_first, bn = b_first;
+  for (int i = threadIdx.x; i < nel; i += FFT_THREADS) {
+    // this element (k, b), then the lane's next one
+    const int k = kn, b = bn;
+    bn += sb_;
+    kn += se;
+    if (bn >= nb) {
+      bn -= nb;
+      ++kn;
+    }
     C x = buf[b * LP + k];
     if (p.twiddle && u > 0 && k > 0) {
       int m;
