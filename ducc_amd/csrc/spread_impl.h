@@ -683,28 +683,76 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
   const C2 *gridc = (const C2 *)a.grid;
   const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
             n2 = NDIM > 2 ? a.g.nover[2] : 1;
+  // region staging: cell c = (cu * S + cv) * SP + cw (the inner index runs
+  // over the padded side; the pad cells hold copies of in-range cells,
+  // never used in arithmetic).  A lane's first cell (c = thread) is split
+  // once; its next cells (c + blockDim.x) step by constant increments, so
+  // no cell needs a division, and a batch of STG loads is in flight before
+  // its LDS writes.
+  constexpr int STG = 8;
+  int cu0 = 0, cv0 = 0, cw0 = (int)threadIdx.x, du = 0, dv = 0,
+      dw = (int)blockDim.x;
+  if (NDIM > 1) {
+    cw0 = (int)threadIdx.x % SP;
+    dw = (int)blockDim.x % SP;
+    const int r0 = (int)threadIdx.x / SP, q = (int)blockDim.x / SP;
+    if (NDIM == 2) {
+      cu0 = r0;
+      du = q;
+    } else {
+      cv0 = r0 % S;
+      cu0 = r0 / S;
+      dv = q % S;
+      du = q / S;
+    }
+  }
+  // the grid index of a region coordinate: one compare inside the period
+  auto wrapg = [](int x, int n) {
+    return (unsigned)x < (unsigned)n ? x : wrapi(x, n);
+  };
   for (uint32_t sidx = blockIdx.x; sidx < total; sidx += gridDim.x) {
     const Sub sb = a.subs[sidx];
     int b0[3];
     tile_origin(a.g, sb.tile, b0);
     if (LDS) {
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        // inner index over the padded side; the pad cells hold copies of
-        // in-range cells (wrapped), never used in arithmetic
-        size_t gi;
-        if constexpr (NDIM == 1) {
-          gi = (size_t)wrapi(b0[0] + c, n0);
-        } else if constexpr (NDIM == 2) {
-          const int cu = c / SP, cv = c - cu * SP;
-          gi = (size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1);
-        } else {
-          const int cu = c / (S * SP), r = c - cu * S * SP, cv = r / SP,
-                    cw = r - cv * SP;
-          gi = ((size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1)) *
-                   n2 +
-               wrapi(b0[2] + cw, n2);
+      int cu = cu0, cv = cv0, cw = cw0;
+      for (int cb = 0; cb < ncell; cb += STG * (int)blockDim.x) {
+        C2 v[STG];
+#pragma unroll
+        for (int k = 0; k < STG; ++k) {
+          const bool in = cb + k * (int)blockDim.x + (int)threadIdx.x < ncell;
+          size_t gi;
+          if constexpr (NDIM == 1) {
+            gi = (size_t)wrapg(b0[0] + cw, n0);
+          } else if constexpr (NDIM == 2) {
+            gi = (size_t)wrapg(b0[0] + cu, n0) * n1 + wrapg(b0[1] + cw, n1);
+          } else {
+            gi = ((size_t)wrapg(b0[0] + cu, n0) * n1 + wrapg(b0[1] + cv, n1)) *
+                     n2 +
+                 wrapg(b0[2] + cw, n2);
+          }
+          v[k] = gridc[in ? gi : 0];
+          // the lane's next cell
+          cw += dw;
+          if (NDIM > 1 && cw >= SP) {
+            cw -= SP;
+            if (NDIM == 2) ++cu;
+            else ++cv;
+          }
+          if (NDIM == 3) {
+            cv += dv;
+            if (cv >= S) {
+              cv -= S;
+              ++cu;
+            }
+          }
+          cu += du;
         }
-        buf[c] = gridc[gi];
+#pragma unroll
+        for (int k = 0; k < STG; ++k) {
+          const int c = cb + k * (int)blockDim.x + (int)threadIdx.x;
+          if (c < ncell) buf[c] = v[k];
+        }
       }
       __syncthreads();
     }
