@@ -64,6 +64,69 @@ __device__ __forceinline__ void tile_origin(const Geo &g, uint32_t tile,
   for (int d = 0; d < 3; ++d) b0[d] = ((int)t[d] << g.log2tile) - g.nsafe;
 }
 
+// A lane's walk over the cells c, c + step, c + 2 step, ... of a region
+// with sides (S, S, inner) (the last NDIM of them): c = (u * S + v) *
+// inner + w.  The first cell is split once; each step adds constant
+// increments with carries, so no cell needs an integer division.
+template <int NDIM>
+struct RegionWalk {
+  int u = 0, v = 0, w = 0, du = 0, dv = 0, dw = 0, S = 1, inner = 1;
+  __device__ RegionWalk(int c, int step, int S_, int inner_)
+      : S(S_), inner(inner_) {
+    w = c;
+    dw = step;
+    if (NDIM > 1) {
+      w = c % inner;
+      dw = step % inner;
+      const int r = c / inner, q = step / inner;
+      if (NDIM == 2) {
+        u = r;
+        du = q;
+      } else {
+        v = r % S;
+        u = r / S;
+        dv = q % S;
+        du = q / S;
+      }
+    }
+  }
+  __device__ __forceinline__ void next() {
+    w += dw;
+    if (NDIM > 1 && w >= inner) {
+      w -= inner;
+      if (NDIM == 2) ++u;
+      else ++v;
+    }
+    if (NDIM == 3) {
+      v += dv;
+      if (v >= S) {
+        v -= S;
+        ++u;
+      }
+    }
+    u += du;
+  }
+};
+
+// grid index of a region coordinate: one compare inside the period
+__device__ __forceinline__ int wrapg(int x, int n) {
+  return (unsigned)x < (unsigned)n ? x : wrapi(x, n);
+}
+
+// grid element of the region cell at walk position rw (origin b0)
+template <int NDIM>
+__device__ __forceinline__ size_t region_grid_index(const RegionWalk<NDIM> &rw,
+                                                    const int *b0, int n0,
+                                                    int n1, int n2) {
+  if constexpr (NDIM == 1) return (size_t)wrapg(b0[0] + rw.w, n0);
+  else if constexpr (NDIM == 2)
+    return (size_t)wrapg(b0[0] + rw.u, n0) * n1 + wrapg(b0[1] + rw.w, n1);
+  else
+    return ((size_t)wrapg(b0[0] + rw.u, n0) * n1 + wrapg(b0[1] + rw.v, n1)) *
+               n2 +
+           wrapg(b0[2] + rw.w, n2);
+}
+
 template <typename T, int NDIM, int W>
 __device__ __forceinline__ void point_setup(const SpreadArgs &a, uint32_t p,
                                             uint32_t &j, int *i0, T (*w)[W]) {
@@ -207,6 +270,8 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   T *grid = (T *)a.grid;
   const T *cf = (const T *)a.coef;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the flush's walk over the region (sides S), split once per lane
+  const RegionWalk<NDIM> flush_walk0(tid, (int)blockDim.x, S, S);
   for (uint32_t sidx0 = blockIdx.x; sidx0 < total; sidx0 += gridDim.x) {
     // stored regions: a tile's first subproblem brings all of the tile's
     // subproblems to this workgroup, the others are skipped
@@ -580,7 +645,11 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
             atomicAdd(&grid[2 * gi], (T)__builtin_nanf(""));
           }
         }
+        RegionWalk<NDIM> rw = flush_walk0;
         for (int c = tid; c < ncell && !(FIX && fbad); c += blockDim.x) {
+          // this cell's grid element, then the lane's next cell
+          const size_t gi = region_grid_index(rw, b0, n0, n1, n2);
+          rw.next();
           T re, im;
           if constexpr (FIX) {
             const unsigned long long t = sfx[c];
@@ -594,19 +663,6 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
             im = (T)sim[c];
           }
           if (re == T(0) && im == T(0)) continue;
-          size_t gi;
-          if constexpr (NDIM == 1) {
-            gi = (size_t)wrapi(b0[0] + c, n0);
-          } else if constexpr (NDIM == 2) {
-            const int cu = c / S, cv = c - cu * S;
-            gi = (size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1);
-          } else {
-            const int cu = c / (S * S), r = c - cu * S * S, cv = r / S,
-                      cw = r - cv * S;
-            gi = ((size_t)wrapi(b0[0] + cu, n0) * n1 + wrapi(b0[1] + cv, n1)) *
-                     n2 +
-                 wrapi(b0[2] + cw, n2);
-          }
           atomicAdd(&grid[2 * gi], re);
           atomicAdd(&grid[2 * gi + 1], im);
         }
@@ -683,70 +739,25 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
   const C2 *gridc = (const C2 *)a.grid;
   const int n0 = a.g.nover[0], n1 = NDIM > 1 ? a.g.nover[1] : 1,
             n2 = NDIM > 2 ? a.g.nover[2] : 1;
-  // region staging: cell c = (cu * S + cv) * SP + cw (the inner index runs
-  // over the padded side; the pad cells hold copies of in-range cells,
-  // never used in arithmetic).  A lane's first cell (c = thread) is split
-  // once; its next cells (c + blockDim.x) step by constant increments, so
-  // no cell needs a division, and a batch of STG loads is in flight before
-  // its LDS writes.
+  // region staging: the inner index runs over the padded side (the pad
+  // cells hold copies of in-range cells, never used in arithmetic); a
+  // batch of STG loads of a lane is in flight before its LDS writes
   constexpr int STG = 8;
-  int cu0 = 0, cv0 = 0, cw0 = (int)threadIdx.x, du = 0, dv = 0,
-      dw = (int)blockDim.x;
-  if (NDIM > 1) {
-    cw0 = (int)threadIdx.x % SP;
-    dw = (int)blockDim.x % SP;
-    const int r0 = (int)threadIdx.x / SP, q = (int)blockDim.x / SP;
-    if (NDIM == 2) {
-      cu0 = r0;
-      du = q;
-    } else {
-      cv0 = r0 % S;
-      cu0 = r0 / S;
-      dv = q % S;
-      du = q / S;
-    }
-  }
-  // the grid index of a region coordinate: one compare inside the period
-  auto wrapg = [](int x, int n) {
-    return (unsigned)x < (unsigned)n ? x : wrapi(x, n);
-  };
+  const RegionWalk<NDIM> walk0((int)threadIdx.x, (int)blockDim.x, S, SP);
   for (uint32_t sidx = blockIdx.x; sidx < total; sidx += gridDim.x) {
     const Sub sb = a.subs[sidx];
     int b0[3];
     tile_origin(a.g, sb.tile, b0);
     if (LDS) {
-      int cu = cu0, cv = cv0, cw = cw0;
+      RegionWalk<NDIM> rw = walk0;
       for (int cb = 0; cb < ncell; cb += STG * (int)blockDim.x) {
         C2 v[STG];
 #pragma unroll
         for (int k = 0; k < STG; ++k) {
           const bool in = cb + k * (int)blockDim.x + (int)threadIdx.x < ncell;
-          size_t gi;
-          if constexpr (NDIM == 1) {
-            gi = (size_t)wrapg(b0[0] + cw, n0);
-          } else if constexpr (NDIM == 2) {
-            gi = (size_t)wrapg(b0[0] + cu, n0) * n1 + wrapg(b0[1] + cw, n1);
-          } else {
-            gi = ((size_t)wrapg(b0[0] + cu, n0) * n1 + wrapg(b0[1] + cv, n1)) *
-                     n2 +
-                 wrapg(b0[2] + cw, n2);
-          }
+          const size_t gi = region_grid_index(rw, b0, n0, n1, n2);
           v[k] = gridc[in ? gi : 0];
-          // the lane's next cell
-          cw += dw;
-          if (NDIM > 1 && cw >= SP) {
-            cw -= SP;
-            if (NDIM == 2) ++cu;
-            else ++cv;
-          }
-          if (NDIM == 3) {
-            cv += dv;
-            if (cv >= S) {
-              cv -= S;
-              ++cu;
-            }
-          }
-          cu += du;
+          rw.next();
         }
 #pragma unroll
         for (int k = 0; k < STG; ++k) {
