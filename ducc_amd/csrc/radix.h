@@ -34,24 +34,35 @@ __device__ __forceinline__ unsigned long long digit_peers(uint32_t d,
   return valid ? peers : 0ull;
 }
 
-// exclusive prefix of one value per thread over a 256-thread block
-// (Hillis-Steele in LDS); total receives the block sum
+// exclusive prefix of one value per thread over a 256-thread block; total
+// receives the block sum.  Inclusive scan inside each wave by lane shifts,
+// then the four wave totals through LDS: two barriers (a Hillis-Steele
+// scan in LDS takes seventeen).
+static_assert(ST % 64 == 0 && ST / 64 <= 16 && RT == ST,
+              "block scan: whole waves, the ranking's block size");
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v,
                                                          uint32_t *sh,
                                                          uint32_t &total) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int o = 1; o < ST; o <<= 1) {
-    uint32_t x = t >= o ? sh[t - o] : 0u;
-    __syncthreads();
-    sh[t] += x;
-    __syncthreads();
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, (unsigned)o);
+    if (lane >= o) x += y;
   }
-  uint32_t incl = sh[t];
-  total = sh[ST - 1];
+  if (lane == 63) sh[w] = x;
   __syncthreads();
-  return incl - v;
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < ST / 64; ++i) {
+    const uint32_t s = sh[i];
+    before += i < w ? s : 0u;
+    tot += s;
+  }
+  total = tot;
+  // sh is reused by the next call
+  __syncthreads();
+  return before + x - v;
 }
 
 // LDS of the ranking step
