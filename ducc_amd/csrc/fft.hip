@@ -245,6 +245,124 @@ __device__ __forceinline__ void lds_passes(typename V2<T>::t *buf, int LP,
   }
 }
 
+// One radix-R Stockham pass over a single line (one line per workgroup)
+// whose input (IN_G) or output (OUT_G) is the line in global memory instead
+// of LDS: the first pass reads its butterflies' elements jj + r * nbf
+// directly (consecutive lanes read consecutive elements), the last one
+// writes its results obase + r * ns directly; the pruned ranges of the
+// launch read as zero / are not stored.  A lane past the last butterfly
+// loads the last butterfly's elements (unconditional loads, all in flight
+// at once) and writes nothing.  No barrier is needed between reading
+// global input and writing LDS, nor after writing global output.
+template <typename T, int R, int KM, int SIGN, bool IN_G, bool OUT_G>
+__device__ __forceinline__ void row_pass(typename V2<T>::t *buf,
+                                         const FftPass &ps,
+                                         const typename V2<T>::t *tw,
+                                         typename V2<T>::t *data,
+                                         int64_t lbase, const FftLaunch &p) {
+  using C = typename V2<T>::t;
+  const int nbf = ps.nbf, Ns = ps.ns, twstep = ps.twstep;
+  C v[KM][R];
+  int obase[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    const int q = threadIdx.x + k * blockDim.x;
+    const int jj = IN_G ? min(q, nbf - 1) : q;
+    if (IN_G || q < nbf) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int e = jj + r * nbf;
+        if constexpr (IN_G) {
+          C x = data[lbase + (int64_t)e * p.st_ax];
+          if (p.kprune && e >= p.kn1 && e < p.ka2) {
+            x.x = T(0);
+            x.y = T(0);
+          }
+          v[k][r] = x;
+        } else {
+          v[k][r] = buf[e];
+        }
+      }
+      int kk;
+      const int jq = Ns == 1 ? (kk = 0, jj) : fdivq(jj, Ns, ps.inv_ns, kk);
+      if (Ns > 1) {
+        const int t1 = kk * twstep;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          C w = tw[r * t1];
+          if (SIGN > 0) w.y = -w.y;
+          v[k][r] = cmul(v[k][r], w);
+        }
+      }
+      obase[k] = jq * Ns * R + kk;
+    }
+  }
+  if (!IN_G) __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    const int q = threadIdx.x + k * blockDim.x;
+    if (q < nbf) {
+      dft<T, R>(v[k], SIGN);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int e = obase[k] + r * ps.ns;
+        if constexpr (OUT_G) {
+          if (!(p.sprune && e >= p.sn1 && e < p.sa2))
+            data[lbase + (int64_t)e * p.st_ax] = v[k][r];
+        } else {
+          buf[e] = v[k][r];
+        }
+      }
+    }
+  }
+  if (!OUT_G) __syncthreads();
+}
+
+template <typename T, int MAXEL, int SIGN, bool IN_G, bool OUT_G>
+__device__ __forceinline__ void row_pass_radix(typename V2<T>::t *buf,
+                                               const FftPass &ps,
+                                               const typename V2<T>::t *tw,
+                                               typename V2<T>::t *data,
+                                               int64_t lbase,
+                                               const FftLaunch &p) {
+#define DMX_ROWPASS(R) \
+  row_pass<T, R, FFT_KM_EL(MAXEL, R), SIGN, IN_G, OUT_G>(buf, ps, tw, data, \
+                                                        lbase, p)
+  switch (ps.radix) {
+    case 8: DMX_ROWPASS(8); break;
+    case 4: DMX_ROWPASS(4); break;
+    case 2: DMX_ROWPASS(2); break;
+    case 3: DMX_ROWPASS(3); break;
+    case 5: DMX_ROWPASS(5); break;
+    case 7: DMX_ROWPASS(7); break;
+    case 11: DMX_ROWPASS(11); break;
+    default: break;
+  }
+#undef DMX_ROWPASS
+}
+
+// all passes of a single line: the first reads the line from global
+// memory, the last writes it back, the others run in LDS
+template <typename T, int MAXEL, int SIGN>
+__device__ __forceinline__ void row_passes(typename V2<T>::t *buf,
+                                           const FftLaunch &p,
+                                           const typename V2<T>::t *tw,
+                                           typename V2<T>::t *data,
+                                           int64_t lbase) {
+  const int np = p.npass;
+  if (np == 1) {
+    row_pass_radix<T, MAXEL, SIGN, true, true>(buf, p.pass[0], tw, data,
+                                               lbase, p);
+    return;
+  }
+  row_pass_radix<T, MAXEL, SIGN, true, false>(buf, p.pass[0], tw, data, lbase,
+                                              p);
+  if (np > 2)
+    lds_passes<T, MAXEL, SIGN>(buf, p.L, 1, np - 2, p.pass + 1, tw);
+  row_pass_radix<T, MAXEL, SIGN, false, true>(buf, p.pass[np - 1], tw, data,
+                                              lbase, p);
+}
+
 __device__ __forceinline__ int64_t map_range(int v, int n1, int a1, int a2) {
   return v < n1 ? (int64_t)(a1 + v) : (int64_t)(a2 + (v - n1));
 }
@@ -273,19 +391,12 @@ k_fft_lds(FftLaunch p) {
   // division
   const int64_t lbase =
       ubase + map_range(v0, p.vn1, p.va1, p.va2) * p.st_v;
-  if (nlines == 1) {
-#pragma unroll 4
-    for (int k = threadIdx.x; k < L; k += blockDim.x) {
-      C x;
-      if (p.kprune && k >= p.kn1 && k < p.ka2) {
-        x.x = T(0);
-        x.y = T(0);
-      } else {
-        x = data[lbase + (int64_t)k * p.st_ax];
-      }
-      buf[k] = x;
-    }
-  } else {
+  if (nlines == 1 && p.npass >= 1) {
+    // one line: the first and last passes read and write global memory
+    row_passes<T, MAXEL, SIGN>(buf, p, tw, data, lbase);
+    return;
+  }
+  {
 #pragma unroll 4
     for (int e = threadIdx.x; e < nel; e += blockDim.x) {
       int b, k;
