@@ -273,8 +273,11 @@ __device__ __forceinline__ void row_pass(typename V2<T>::t *buf,
       for (int r = 0; r < R; ++r) {
         const int e = jj + r * nbf;
         if constexpr (IN_G) {
-          C x = data[lbase + (int64_t)e * p.st_ax];
-          if (p.kprune && e >= p.kn1 && e < p.ka2) {
+          // an element known to be zero is not fetched: its load repeats
+          // the butterfly's first element and the value is replaced
+          const bool zero = p.kprune && e >= p.kn1 && e < p.ka2;
+          C x = data[lbase + (int64_t)(zero ? jj : e) * p.st_ax];
+          if (zero) {
             x.x = T(0);
             x.y = T(0);
           }
