@@ -185,6 +185,16 @@ def fibonacci_lattice(npoints):
     return loc
 
 
+def random_alm(lmax, rng):
+    """random a_lm of lmax = mmax in the default layout (m-major, m = 0 first),
+    real and imaginary parts uniform in [-1, 1), the m = 0 entries real
+    (python/test/test_sht.py:31-40)"""
+    n = (lmax + 1) * (lmax + 2) // 2
+    res = rng.uniform(-1., 1., n) + 1j * rng.uniform(-1., 1., n)
+    res[:lmax + 1].imag = 0.
+    return res
+
+
 def bench_sht(args):
     """ducc0.sht.synthesis_general + adjoint_synthesis_general on the
     reference demo's sky (python/demos/sht_general.py:29-41: lmax = mmax =
@@ -195,12 +205,11 @@ def bench_sht(args):
     baseline (median of 3 after a warm-up)."""
     import torch
     from ducc_amd import _lib, sht
-    from oracle import sht as S
     dev = torch.device("cuda:0")
     lmax, npt, eps, spin = 1187, 2_000_000, 1e-7, args.spin
     nc = 1 if spin == 0 else 2
     rng = np.random.default_rng(42)
-    a_np = np.stack([S.random_alm(lmax, lmax, rng) for _ in range(nc)])
+    a_np = np.stack([random_alm(lmax, rng) for _ in range(nc)])
     ofs = 0
     for s_ in range(spin):  # l < spin: zero (test_sht.py:31-40)
         a_np[:, ofs:ofs + spin - s_] = 0
@@ -282,11 +291,10 @@ def bench_sht_pseudo(args):
     map, is the CPU baseline (one call after a warm-up: about 2 s)."""
     import torch
     from ducc_amd import _lib, sht
-    from oracle import sht as S
     dev = torch.device("cuda:0")
     lmax, npt, eps, spin, maxiter = 1187, 2_000_000, 1e-7, 2, 20
     rng = np.random.default_rng(42)
-    a_np = np.stack([S.random_alm(lmax, lmax, rng) for _ in range(2)])
+    a_np = np.stack([random_alm(lmax, rng) for _ in range(2)])
     ofs = 0
     for s_ in range(spin):  # l < spin: zero (sht_general.py:8-17)
         a_np[:, ofs:ofs + spin - s_] = 0

@@ -741,8 +741,10 @@ __global__ __launch_bounds__(256) void k_interp(SpreadArgs a, void *out,
             n2 = NDIM > 2 ? a.g.nover[2] : 1;
   // region staging: the inner index runs over the padded side (the pad
   // cells hold copies of in-range cells, never used in arithmetic); a
-  // batch of STG loads of a lane is in flight before its LDS writes
-  constexpr int STG = 8;
+  // batch of STG loads of a lane is in flight before its LDS writes (two
+  // for FP64: eight double2 registers push the wide FP64 kernels past 2
+  // waves per SIMD)
+  constexpr int STG = sizeof(T) == 8 ? 2 : 8;
   const RegionWalk<NDIM> walk0((int)threadIdx.x, (int)blockDim.x, S, SP);
   for (uint32_t sidx = blockIdx.x; sidx < total; sidx += gridDim.x) {
     const Sub sb = a.subs[sidx];
