@@ -3,9 +3,11 @@
 The network (6 -> 64 -> 64 -> 64 -> 3, ReLU) maps a request's features
 (ndim, mean log2 grid size, log10 M, -log10 eps, fp64 flag, type-1 flag) to
 offsets of (oversampling factor, support W, log2 tile) from ducc0's
-analytic choice (and default tile); the library adds them to the analytic
-setting and maps the sum onto the kernel table, so an output near zero
-reproduces the analytic choice.  Its forward pass, gradient and
+analytic choice and from a fixed tile anchor (the tiles of the training set:
+2^9 1-D, 2^5 / 2^4 for 2-D FP32 / FP64, 2^3 3-D); the library adds them to
+the analytic setting and maps the sum onto the kernel table, so an output
+near zero reproduces the analytic kernel and oversampling (and the anchor
+tile, which differs from the library's 2-D FP64 default of 2^5).  Its forward pass, gradient and
 Adam training run as HIP kernels in libducc_amd.so (mlp.hip).  Trained
 weights and normalisation live in ducc_amd/data/chooser_weights.json; without
 that file `chooser="learned"` falls back to ducc0's analytic choice.

@@ -12,7 +12,7 @@
    towards the smallest support; every candidate row reaches eps by the
    table, and the accuracy of the learned choice is re-checked at run time.
 2. Training: 6 features -> offsets of (sigma, W, log2 tile) from the
-   analytic setting (default tile), normalised, full-batch Adam with the
+   analytic setting (tile: tile_anchor), normalised, full-batch Adam with the
    library's HIP kernels (dmx_mlp_train), checkpoint chosen on a held-out
    20 %.
 3. Evaluation: fresh requests timed with the analytic choice (ducc0's cost
@@ -119,7 +119,9 @@ def candidates(req, f64, db):
     return [(k, t) for k in out for t in tiles]
 
 
-def default_log2tile(nd, f64):
+def tile_anchor(nd, f64):
+    """the log2 tile the third head is an offset from (chooser.hip
+    tile_anchor: the same table, fixed, apart from the library's default)"""
     return 9 if nd == 1 else ((4 if f64 else 5) if nd == 2 else 3)
 
 
@@ -129,14 +131,14 @@ def pick_label(times, an, db, nd, f64, margin=1.35, tie=1.03):
     analytic setting stays the label unless a candidate beats it by more
     than `margin` (smaller gains are within timing noise and what the six
     features can explain: learning them made unseen requests slower); among candidates within `tie` of the fastest the
-    smallest support, then the smallest oversampling, then the default tile
-    wins (timing noise otherwise flips labels between near-equal
+    smallest support, then the smallest oversampling, then the tile nearest
+    tile_anchor wins (timing noise otherwise flips labels between near-equal
     settings)."""
     best = min(times.values())
     if best * margin > an[0]:
         return an[1], an[2]
     near = [kt for kt, t in times.items() if t <= best * tie]
-    dt = default_log2tile(nd, f64)
+    dt = tile_anchor(nd, f64)
     k, t = min(near, key=lambda kt: (db["W"][kt[0]], db["ofactor"][kt[0]],
                                      abs(kt[1] - dt)))
     return k, t
@@ -187,7 +189,7 @@ def label_requests(n, seed, db, log, start=0):
                                       typ == "nu2u"))
             T.append([db["ofactor"][k], db["W"][k], t])
             A.append([db["ofactor"][an[1]], db["W"][an[1]],
-                      default_log2tile(nd, f64)])
+                      tile_anchor(nd, f64)])
             meta.append(dict(req=req, type=typ, analytic=list(an),
                              label=[int(k), int(t)],
                              times=[[int(kk), int(tt), v]
@@ -223,7 +225,7 @@ def fit(X, T, A, log, seed=0, wd=1e-4, chunk=250):
     """Adam on 80 % of the samples in chunks of `chunk` steps; the
     checkpoint with the lowest loss on the held-out 20 % is kept (the
     network has far more weights than there are labels).  Targets are the
-    offsets T - A of the labels from the analytic setting A (default tile):
+    offsets T - A of the labels from the analytic setting A (tile: tile_anchor):
     the library adds the prediction to the analytic setting."""
     T = T - A
     rng = np.random.default_rng(seed)

@@ -126,6 +126,16 @@ bool map_choice(int ndim, const size_t *nuni, double eps, double smin,
   return false;
 }
 
+// The log2 tile the network's third head is an offset from: the tiles of
+// its training set (chooser_train.py tile_anchor), kept apart from the
+// library's default tile so that a change of the default does not move the
+// network's choices.
+static int tile_anchor(int ndim, bool fp64) {
+  if (ndim == 1) return 9;
+  if (ndim == 2) return fp64 ? 4 : 5;
+  return 3;
+}
+
 static std::string sig_key(int ndim, const size_t *nuni, size_t M,
                            double eps, double smin, double smax, bool fp64,
                            bool gridding) {
@@ -170,7 +180,7 @@ bool learned_choose(int ndim, const size_t *nuni, size_t M, double eps,
     } else {
       pred[0] += kernel_db()[kan].ofactor;
       pred[1] += kernel_db()[kan].W;
-      pred[2] += default_log2tile(ndim, fp64);
+      pred[2] += tile_anchor(ndim, fp64);
     }
   }
   LearnedChoice lc;

@@ -97,7 +97,7 @@ def test_network_choice_and_fallback(gpu):
         nufft.nu2u(points=p, coord=c, forward=True, epsilon=eps, out=out,
                    chooser="learned", info=info)
         assert info["chooser_used"] and info["W"] == an["W"] + 1
-        assert info["log2tile"] == 4 - 1   # default 2-D FP64 tile: 2^4
+        assert info["log2tile"] == 4 - 1   # 2-D FP64 tile anchor: 2^4
         assert info["eps_tab"] <= eps
         assert O.l2error(out, ref) <= 2 * eps
         info2 = {}
