@@ -163,12 +163,15 @@ Params choose_params(int ndim, const size_t *nuni, size_t M, bool gridding,
                      const dmx_options &opt);
 
 // the parameters for kernel-table row kidx (of any dimensionality) at
-// oversampling factor ofactor, with the tile logic of choose_params
+// oversampling factor ofactor, with the tile logic of choose_params; npts:
+// the points one spreading / interpolation call processes (0: unknown)
 Params kernel_params(int ndim, const size_t *nuni, const double *period,
                      bool fft_order, bool fp64, double eps, int kidx,
-                     double ofactor, const dmx_options &opt);
-// choose_params' common tail: kernel row, limits, tile
-void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt);
+                     double ofactor, const dmx_options &opt, double npts);
+// choose_params' common tail: kernel row, limits, tile (l2t <= 0: the
+// default for the dimensionality, precision and npts points per call)
+void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt,
+                   double npts);
 
 class Plan {
  public:

@@ -359,11 +359,12 @@ Params choose_params(int ndim, const size_t *nuni, size_t M, bool gridding,
                        p.nover);
     DMX_ASSERT(kidx >= 0, "no appropriate kernel found");
   }
-  finish_params(p, kidx, l2t, opt);
+  finish_params(p, kidx, l2t, opt, (double)M);
   return p;
 }
 
-void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt) {
+void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt,
+                   double npts) {
   const int ndim = p.ndim;
   const KRow &k = kernel_db()[kidx];
   p.kidx = kidx;
@@ -382,7 +383,18 @@ void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt) {
   // tile: requested, else the default shrunk until the LDS region fits
   if (opt.log2tile > 0) l2t = opt.log2tile;
   int def = default_log2tile(ndim, p.fp64);
-  if (l2t <= 0) l2t = def;
+  if (l2t <= 0) {
+    l2t = def;
+    // sparse 2-D fixed-point spreading: below one point per sixteen
+    // oversampled cells a 64 x 64 tile mostly stays within one subproblem
+    // (MAXSUB points) and its region's halo is a smaller share of the cells
+    // flushed and staged (denser sets split such tiles and flush their 70^2
+    // regions several times over: clustered points at one per ten cells on
+    // average already do; FP64 regions of 64^2 leave one workgroup per CU)
+    if (ndim == 2 && p.spread_fix && npts > 0 &&
+        16 * npts < (double)p.nover[0] * (double)p.nover[1])
+      l2t = 6;
+  }
   DMX_ASSERT(l2t >= 2 && l2t <= 10, "log2tile out of range");
   p.use_lds = true;
   while (lds_bytes(ndim, 1 << l2t, p.W, p.fp64) > LDS_LIMIT && l2t > 2) --l2t;
@@ -403,7 +415,7 @@ void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt) {
 
 Params kernel_params(int ndim, const size_t *nuni, const double *period,
                      bool fft_order, bool fp64, double eps, int kidx,
-                     double ofactor, const dmx_options &opt) {
+                     double ofactor, const dmx_options &opt, double npts) {
   DMX_ASSERT(ndim >= 1 && ndim <= 3, "transform must be 1D/2D/3D");
   DMX_ASSERT(kidx >= 0 && kidx < (int)kernel_db().size(),
              "no appropriate kernel found");
@@ -418,7 +430,7 @@ Params kernel_params(int ndim, const size_t *nuni, const double *period,
   p.fft_order = fft_order;
   p.eps = eps;
   p.fp64 = fp64;
-  finish_params(p, kidx, 0, opt);
+  finish_params(p, kidx, 0, opt, npts);
   p.ofactor = ofactor;
   return p;
 }

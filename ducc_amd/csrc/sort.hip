@@ -273,13 +273,22 @@ __global__ __launch_bounds__(ST) void k_tile_subs_one(
   for (size_t c = 0; c < ntiles; c += (size_t)ST * TS_ITEMS) {
     const size_t base = c + (size_t)threadIdx.x * TS_ITEMS;
     uint32_t st[TS_ITEMS], cn[TS_ITEMS], ns[TS_ITEMS], sum = 0;
+    // all loads of a lane in flight at once: unconditional, from clamped
+    // indices (a tile past the end reads the last tile and is discarded)
+    uint32_t ev[TS_ITEMS], sv[TS_ITEMS];
+#pragma unroll
+    for (int k = 0; k < TS_ITEMS; ++k) {
+      const size_t t = min(base + k, ntiles - 1);
+      ev[k] = tend[t];
+      sv[k] = tstart[t];
+    }
 #pragma unroll
     for (int k = 0; k < TS_ITEMS; ++k) {
       const size_t t = base + k;
       st[k] = cn[k] = ns[k] = 0;
       if (t < ntiles) {
-        const uint32_t e = tend[t];
-        st[k] = e ? ~tstart[t] : 0u;
+        const uint32_t e = ev[k];
+        st[k] = e ? ~sv[k] : 0u;
         cn[k] = e - st[k];
         ns[k] = (cn[k] + MAXSUB - 1) / MAXSUB;
         tstart[t] = st[k];
@@ -443,6 +452,7 @@ void launch_tile_subs(size_t ntiles, uint32_t *counts, uint32_t *tstart,
                       const uint32_t *tend, uint32_t *nsub,
                       uint32_t *sub_start, Sub *subs, uint32_t *nsub_total,
                       void *scanwork, hipStream_t s) {
+  DMX_ASSERT(ntiles > 0, "internal: no tiles");
   if (ntiles <= (size_t)ST * TS_ITEMS) {
     hipLaunchKernelGGL(k_tile_subs_one, dim3(1), dim3(ST), 0, s, tstart, tend,
                        ntiles, counts, nsub, sub_start, subs, nsub_total);

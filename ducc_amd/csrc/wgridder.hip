@@ -495,7 +495,7 @@ void WgJob::choose(double eps, double smin, double smax, bool fp64_values,
   }
   DMX_ASSERT(!rows.empty(), "no appropriate kernel found");
   const auto &db = kernel_db();
-  double best = 1e300;
+  double best = 1e300, bnp = 1;
   int bi = -1;
   for (int r : rows) {
     const KRow &k = db[r];
@@ -509,10 +509,13 @@ void WgJob::choose(double eps, double smin, double smax, bool fp64_values,
     if (cost < best) {
       best = cost;
       bi = r;
+      bnp = np;
     }
   }
   const KRow &k = db[bi];
-  p = kernel_params(2, nuni, per, false, fp64, eps, bi, k.ofactor, opt);
+  // each visibility is spread onto W of the bnp planes
+  p = kernel_params(2, nuni, per, false, fp64, eps, bi, k.ofactor, opt,
+                    (double)nvis * k.W / std::max(bnp, 1.0));
   pl.W = k.W;
   dw = mx > 0 ? 0.5 / k.ofactor / mx : 1.0;
   // wgridder.h:683-688

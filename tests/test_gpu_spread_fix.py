@@ -109,3 +109,46 @@ def test_fixed_point_non_finite_values(gpu):
                    fp64_accumulation=True)
         assert not np.all(np.isfinite(ref))
         assert np.all(np.isnan(out))
+
+
+@pytest.mark.parametrize("M,dist,tile", [(30_000, "uniform", 6),
+                                         (30_000, "cluster", 6),
+                                         (400_000, "uniform", 5)])
+def test_sparse_default_tile(gpu, M, dist, tile):
+    """2-D fixed point below one point per sixteen oversampled cells takes
+    64 x 64 tiles by default (denser sets 32 x 32): type 1 and type 2 within
+    eps of the FP64 direct sum and close to explicit 32 x 32 tiles"""
+    from ducc_amd import nufft
+    shape, eps = (512, 512), 1e-5
+    print("sparse tile", M, dist, flush=True)
+    rng = np.random.default_rng(M + len(dist))
+    c = _points(rng, M, 2, dist)
+    v = (rng.normal(size=M) + 1j * rng.normal(size=M)).astype(np.complex64)
+    g = (rng.normal(size=shape) + 1j * rng.normal(size=shape)) \
+        .astype(np.complex64)
+    i1, i2 = {}, {}
+    o1 = np.empty(shape, np.complex64)
+    nufft.nu2u(points=v, coord=c, forward=True, epsilon=eps, out=o1,
+               info=i1, check_offsets=True)
+    o2 = np.empty(M, np.complex64)
+    nufft.u2nu(grid=g, coord=c, forward=False, epsilon=eps, out=o2, info=i2,
+               check_offsets=True)
+    assert i1["log2tile"] == tile and i2["log2tile"] == tile
+    assert 16 * M < np.prod(i1["nover"]) if tile == 6 else True
+    r1 = np.empty(shape, np.complex64)
+    nufft.nu2u(points=v, coord=c, forward=True, epsilon=eps, out=r1,
+               log2tile=5)
+    r2 = np.empty(M, np.complex64)
+    nufft.u2nu(grid=g, coord=c, forward=False, epsilon=eps, out=r2,
+               log2tile=5)
+    sm = rng.choice(shape[0] * shape[1], 400, replace=False)
+    sp = rng.choice(M, 400, replace=False)
+    d1 = nufft.nudft_direct(1, c, v.astype(np.complex128), shape, True, sm)
+    d2 = nufft.nudft_direct(2, c, g.astype(np.complex128), shape, False, sp)
+    e1 = O.l2error(o1.ravel()[sm].astype(np.complex128), d1)
+    e2 = O.l2error(o2[sp].astype(np.complex128), d2)
+    t1 = O.l2error(o1.astype(np.complex128), r1.astype(np.complex128))
+    t2 = O.l2error(o2.astype(np.complex128), r2.astype(np.complex128))
+    print(f"err {e1:.2e} {e2:.2e}, vs 32x32 tiles {t1:.2e} {t2:.2e}")
+    assert e1 <= eps and e2 <= eps
+    assert t1 <= 0.2 * eps and t2 <= 1e-6
