@@ -263,50 +263,61 @@ __global__ void k_fill_subs(const uint32_t *counts, const uint32_t *tstart,
 // tiles (one launch instead of four; longer tables take the launches above,
 // which measured faster from about 10^4 tiles on): per 4096-tile chunk, start / count / subproblem
 // count of each tile, a block prefix of the subproblem counts (with the
-// carry of the chunks before) and the subproblems themselves
+// carry of the chunks before) and the subproblems themselves.  The scan
+// wants TS_ITEMS consecutive tiles per lane, global memory wants
+// consecutive tiles per wave: the chunk's tables pass through LDS (one pad
+// word per TS_ITEMS entries keeps both access patterns free of bank
+// conflicts), so every global load and store of the tables is coalesced.
 constexpr int TS_ITEMS = 16;
+constexpr int TS_CH = ST * TS_ITEMS, TS_CHP = TS_CH + TS_CH / TS_ITEMS;
+__device__ __forceinline__ int ts_slot(int q) { return q + q / TS_ITEMS; }
 __global__ __launch_bounds__(ST) void k_tile_subs_one(
     uint32_t *tstart, const uint32_t *tend, size_t ntiles, uint32_t *counts,
     uint32_t *nsub, uint32_t *sstart, Sub *subs, uint32_t *nsub_total) {
   __shared__ uint32_t sh[ST];
+  __shared__ uint32_t la[TS_CHP], lb[TS_CHP];
+  const int tid = threadIdx.x;
   uint32_t carry = 0;
-  for (size_t c = 0; c < ntiles; c += (size_t)ST * TS_ITEMS) {
-    const size_t base = c + (size_t)threadIdx.x * TS_ITEMS;
-    uint32_t st[TS_ITEMS], cn[TS_ITEMS], ns[TS_ITEMS], sum = 0;
-    // all loads of a lane in flight at once: unconditional, from clamped
-    // indices (a tile past the end reads the last tile and is discarded)
+  for (size_t c = 0; c < ntiles; c += TS_CH) {
+    const int nc = (int)min((size_t)TS_CH, ntiles - c);
+    // coalesced loads, all in flight (clamped indices past the chunk's end
+    // read its last tile; those entries are never used)
     uint32_t ev[TS_ITEMS], sv[TS_ITEMS];
 #pragma unroll
     for (int k = 0; k < TS_ITEMS; ++k) {
-      const size_t t = min(base + k, ntiles - 1);
+      const size_t t = c + (size_t)min(k * ST + tid, nc - 1);
       ev[k] = tend[t];
       sv[k] = tstart[t];
     }
+    __syncthreads();  // the previous chunk's LDS readers are done
 #pragma unroll
     for (int k = 0; k < TS_ITEMS; ++k) {
-      const size_t t = base + k;
+      la[ts_slot(k * ST + tid)] = ev[k];
+      lb[ts_slot(k * ST + tid)] = sv[k];
+    }
+    __syncthreads();
+    const int q0 = tid * TS_ITEMS;
+    uint32_t st[TS_ITEMS], cn[TS_ITEMS], ns[TS_ITEMS], sum = 0;
+#pragma unroll
+    for (int k = 0; k < TS_ITEMS; ++k) {
       st[k] = cn[k] = ns[k] = 0;
-      if (t < ntiles) {
-        const uint32_t e = ev[k];
-        st[k] = e ? ~sv[k] : 0u;
+      if (q0 + k < nc) {
+        const uint32_t e = la[ts_slot(q0 + k)];
+        st[k] = e ? ~lb[ts_slot(q0 + k)] : 0u;
         cn[k] = e - st[k];
         ns[k] = (cn[k] + MAXSUB - 1) / MAXSUB;
-        tstart[t] = st[k];
-        counts[t] = cn[k];
-        nsub[t] = ns[k];
         sum += ns[k];
       }
     }
     uint32_t tot;
-    uint32_t ex = block_exclusive_scan(sum, sh, tot) + carry;
+    const uint32_t ex0 = block_exclusive_scan(sum, sh, tot) + carry;
+    uint32_t ex = ex0;
 #pragma unroll
     for (int k = 0; k < TS_ITEMS; ++k) {
-      const size_t t = base + k;
-      if (t >= ntiles) continue;
-      sstart[t] = ex;
+      if (q0 + k >= nc) continue;
       for (uint32_t q = 0; q < ns[k]; ++q) {
         Sub sb;
-        sb.tile = (uint32_t)t;
+        sb.tile = (uint32_t)(c + q0 + k);
         sb.start = st[k] + q * MAXSUB;
         sb.end = min(st[k] + cn[k], sb.start + (uint32_t)MAXSUB);
         sb.rank = q;
@@ -314,9 +325,43 @@ __global__ __launch_bounds__(ST) void k_tile_subs_one(
       }
       ex += ns[k];
     }
+    // starts and counts, then subproblem counts and first subproblems,
+    // back through LDS to coalesced stores
+    __syncthreads();  // all lanes have read la / lb
+#pragma unroll
+    for (int k = 0; k < TS_ITEMS; ++k) {
+      la[ts_slot(q0 + k)] = st[k];
+      lb[ts_slot(q0 + k)] = cn[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TS_ITEMS; ++k) {
+      const int q = k * ST + tid;
+      if (q < nc) {
+        tstart[c + q] = la[ts_slot(q)];
+        counts[c + q] = lb[ts_slot(q)];
+      }
+    }
+    __syncthreads();
+    ex = ex0;
+#pragma unroll
+    for (int k = 0; k < TS_ITEMS; ++k) {
+      la[ts_slot(q0 + k)] = ns[k];
+      lb[ts_slot(q0 + k)] = ex;
+      ex += ns[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TS_ITEMS; ++k) {
+      const int q = k * ST + tid;
+      if (q < nc) {
+        nsub[c + q] = la[ts_slot(q)];
+        sstart[c + q] = lb[ts_slot(q)];
+      }
+    }
     carry += tot;
   }
-  if (threadIdx.x == 0) *nsub_total = carry;
+  if (tid == 0) *nsub_total = carry;
 }
 
 template <typename T>
