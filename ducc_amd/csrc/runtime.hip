@@ -385,13 +385,13 @@ void finish_params(Params &p, int kidx, int l2t, const dmx_options &opt,
   int def = default_log2tile(ndim, p.fp64);
   if (l2t <= 0) {
     l2t = def;
-    // sparse 2-D fixed-point spreading: below one point per sixteen
-    // oversampled cells a 64 x 64 tile mostly stays within one subproblem
-    // (MAXSUB points) and its region's halo is a smaller share of the cells
-    // flushed and staged (denser sets split such tiles and flush their 70^2
-    // regions several times over: clustered points at one per ten cells on
-    // average already do; FP64 regions of 64^2 leave one workgroup per CU)
-    if (ndim == 2 && p.spread_fix && npts > 0 &&
+    // sparse 2-D FP32: below one point per sixteen oversampled cells a
+    // 64 x 64 tile mostly stays within one subproblem (MAXSUB points) and
+    // its region's halo is a smaller share of the cells flushed and staged
+    // (denser sets split such tiles and flush their 70^2 regions several
+    // times over: clustered points at one per ten cells on average already
+    // do; FP64 regions of 64^2 leave one workgroup per CU)
+    if (ndim == 2 && !p.fp64 && npts > 0 &&
         16 * npts < (double)p.nover[0] * (double)p.nover[1])
       l2t = 6;
   }

@@ -111,16 +111,18 @@ def test_fixed_point_non_finite_values(gpu):
         assert np.all(np.isnan(out))
 
 
-@pytest.mark.parametrize("M,dist,tile", [(30_000, "uniform", 6),
-                                         (30_000, "cluster", 6),
-                                         (400_000, "uniform", 5)])
-def test_sparse_default_tile(gpu, M, dist, tile):
-    """2-D fixed point below one point per sixteen oversampled cells takes
-    64 x 64 tiles by default (denser sets 32 x 32): type 1 and type 2 within
-    eps of the FP64 direct sum and close to explicit 32 x 32 tiles"""
+@pytest.mark.parametrize("M,dist,eps,tile", [(30_000, "uniform", 1e-5, 6),
+                                              (30_000, "cluster", 1e-5, 6),
+                                              (400_000, "uniform", 1e-5, 5),
+                                              (30_000, "cluster", 1e-6, 6)])
+def test_sparse_default_tile(gpu, M, dist, eps, tile):
+    """2-D FP32 below one point per sixteen oversampled cells takes 64 x 64
+    tiles by default (denser sets 32 x 32), with fixed-point and with FP64
+    accumulation: type 1 and type 2 within eps of the FP64 direct sum and
+    close to explicit 32 x 32 tiles"""
     from ducc_amd import nufft
-    shape, eps = (512, 512), 1e-5
-    print("sparse tile", M, dist, flush=True)
+    shape = (512, 512)
+    print("sparse tile", M, dist, eps, flush=True)
     rng = np.random.default_rng(M + len(dist))
     c = _points(rng, M, 2, dist)
     v = (rng.normal(size=M) + 1j * rng.normal(size=M)).astype(np.complex64)
