@@ -929,14 +929,18 @@ struct ShtJob {
     args.nacc = (int64_t)nacc;
   }
   size_t rows() const { return (size_t)(2 * geo.N - 2); }
-  // the doubled-sphere grid's c2c over its rows (theta), scaled 1/(2N-2)
+  // the doubled-sphere grid's c2c over its rows (theta), in place and
+  // unscaled: its factor 1/(2N-2) is applied to the map or the a_lm at the
+  // end (theta_scale), every step in between being linear, so that the
+  // 90 MB grid is not rewritten for it
   std::shared_ptr<void> theta_fft(void *G, bool forward, hipStream_t s) {
     const int64_t st[2] = {geo.nphi, 1};
     const int64_t shape[2] = {(int64_t)rows(), geo.nphi};
     const int axes[1] = {0};
-    return c2c_exec(G, DMX_C128, st, G, st, 2, shape, axes, 1, forward,
-                    1.0 / (double)rows(), s);
+    return c2c_exec(G, DMX_C128, st, G, st, 2, shape, axes, 1, forward, 1.0,
+                    s);
   }
+  double theta_scale() const { return 1.0 / (double)rows(); }
   // the points sorted for the NUFFT of this grid
   std::unique_ptr<Plan> plan(const void *loc, hipStream_t s) {
     auto pl = std::make_unique<Plan>(p, s);
@@ -979,7 +983,7 @@ struct ShtOp {
     const int64_t ust[2] = {job.geo.nphi, 1};
     pl->u2nu(false, G.p, true, ust, z.p, true);
     launch_real_part(z.as<double2>(), job.M, map, job.vf64, ms1,
-                     job.spin > 0 ? ms0 : 0, s);
+                     job.spin > 0 ? ms0 : 0, job.theta_scale(), s);
   }
   void adjoint(const void *map, int64_t ms0, int64_t ms1, void *alm,
                int64_t acomp) {
@@ -996,10 +1000,12 @@ struct ShtOp {
     if (leg2alm_needs_zero(job.args)) launch_zero(acc.p, acc.bytes, s);
     if (job.spin > 0) {
       launch_leg2alm_spin(job.args, G.as<double2>(), acc.as<double2>(), s);
-      launch_store_alm_spin(acc.as<double2>(), job.args, alm, job.vf64, s);
+      launch_store_alm_spin(acc.as<double2>(), job.args, alm, job.vf64,
+                            job.theta_scale(), s);
     } else {
       launch_leg2alm(job.args, G.as<double2>(), acc.as<double2>(), s);
-      launch_store_alm(acc.as<double2>(), job.args, alm, job.vf64, s);
+      launch_store_alm(acc.as<double2>(), job.args, alm, job.vf64,
+                       job.theta_scale(), s);
     }
   }
 };

@@ -56,23 +56,25 @@ void launch_alm2leg(const ShtArgs &g, double2 *G, hipStream_t s);
 void launch_leg2alm(const ShtArgs &g, const double2 *Gt, double2 *acc,
                     hipStream_t s);
 bool leg2alm_needs_zero(const ShtArgs &g);
-// complex values -> the map: component 0 the real parts, and for cstride
-// != 0 component 1 (at map + cstride) the imaginary parts
+// complex values times fct -> the map: component 0 the real parts, and for
+// cstride != 0 component 1 (at map + cstride) the imaginary parts
 void launch_real_part(const double2 *z, size_t M, void *map, bool map_f64,
-                      int64_t mstride, int64_t cstride, hipStream_t s);
+                      int64_t mstride, int64_t cstride, double fct,
+                      hipStream_t s);
 // the map -> complex values: real part component 0, imaginary part
 // component 1 when cstride != 0, else zero
 void launch_to_complex(const void *map, bool map_f64, int64_t mstride,
                        int64_t cstride, size_t M, double2 *z, hipStream_t s);
+// acc times fct -> the a_lm output
 void launch_store_alm(const double2 *acc, const ShtArgs &g, void *alm,
-                      bool alm_f64, hipStream_t s);
+                      bool alm_f64, double fct, hipStream_t s);
 // spin > 0 (g.spin): the same steps with the pairs of spin-weighted
 // functions; G / Gt hold the complex field Q + i U, acc 2 * nacc entries
 void launch_alm2leg_spin(const ShtArgs &g, double2 *G, hipStream_t s);
 void launch_leg2alm_spin(const ShtArgs &g, const double2 *Gt, double2 *acc,
                          hipStream_t s);
 void launch_store_alm_spin(const double2 *acc, const ShtArgs &g, void *alm,
-                           bool alm_f64, hipStream_t s);
+                           bool alm_f64, double fct, hipStream_t s);
 
 // vector steps of the LSMR solver of pseudo_analysis_general: y = a x + b y
 // over n reals; *out += the sum of squares of n reals; *out += the a_lm

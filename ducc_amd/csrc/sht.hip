@@ -15,8 +15,9 @@
 //     (-1)^m leg_m(theta_j), and column -m holds conj(leg_m) (a real map).
 //     The rows are samples of a trigonometric polynomial of degree lmax
 //     in theta, so
-//  2. a c2c FFT over the 2N-2 rows (scaled by 1/(2N-2)) gives its exact
-//     Fourier coefficients C[k, m] (k in FFT order), and
+//  2. a c2c FFT over the 2N-2 rows (scaled by 1/(2N-2): the factor is
+//     applied to the map, or in the adjoint to the a_lm, at the end) gives
+//     its exact Fourier coefficients C[k, m] (k in FFT order), and
 //  3. f at the points is the type-2 NUFFT of C (u2nu, sign +, periodic in
 //     both coordinates with period 2 pi, FFT order), whose real part is
 //     the map.  This is the same decomposition as SphereInterpol (theta
@@ -385,16 +386,16 @@ __global__ __launch_bounds__(SHT_THREADS) void k_leg2alm(ShtArgs g,
   }
 }
 
-// complex results -> real map: the real parts, and with cstride != 0 the
-// imaginary parts as the second component; strided
+// complex results times fct -> real map: the real parts, and with cstride
+// != 0 the imaginary parts as the second component; strided
 template <typename TO>
 __global__ void k_real_part(const double2 *z, size_t M, TO *map,
-                            int64_t mstride, int64_t cstride) {
+                            int64_t mstride, int64_t cstride, double fct) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
   const double2 v = z[i];
-  map[(int64_t)i * mstride] = (TO)v.x;
-  if (cstride) map[cstride + (int64_t)i * mstride] = (TO)v.y;
+  map[(int64_t)i * mstride] = (TO)(v.x * fct);
+  if (cstride) map[cstride + (int64_t)i * mstride] = (TO)(v.y * fct);
 }
 
 // real map -> complex values: the second component (cstride != 0) as the
@@ -409,14 +410,17 @@ __global__ void k_to_complex(const TI *map, int64_t mstride, int64_t cstride,
   z[i] = make_double2(re, im);
 }
 
-// accumulated a_lm -> output array (m = 0: the real part, as the real
-// field's a_l0 are real)
+// accumulated a_lm times fct -> output array (m = 0: the real part, as the
+// real field's a_l0 are real)
 template <typename TO>
-__global__ void k_store_alm(const double2 *acc, ShtArgs g, TO *alm) {
+__global__ void k_store_alm(const double2 *acc, ShtArgs g, TO *alm,
+                            double fct) {
   const int m = blockIdx.y;
   const int l = m + blockIdx.x * blockDim.x + threadIdx.x;
   if (l > g.lmax) return;
   double2 v = acc[g.accofs[m] + (l - m)];
+  v.x *= fct;
+  v.y *= fct;
   if (m == 0) v.y = 0.0;
   const int64_t ix = g.mstart[m] + (int64_t)l * g.lstride;
   alm[2 * ix] = (TO)v.x;
@@ -783,14 +787,17 @@ __global__ __launch_bounds__(SHT_THREADS) void k_leg2alm_spin(
 // / 2 (zero for l < max(m, s); m = 0 real; deriv1: E weighted by
 // sqrt(l (l + 1)))
 template <typename TO>
-__global__ void k_store_alm_spin(const double2 *acc, ShtArgs g, TO *alm) {
+__global__ void k_store_alm_spin(const double2 *acc, ShtArgs g, TO *alm,
+                                 double fct) {
   const int m = blockIdx.y, s = g.spin;
   const int l = m + blockIdx.x * blockDim.x + threadIdx.x;
   if (l > g.lmax) return;
   double2 e = make_double2(0.0, 0.0), b = e;
   if (l >= max(m, s)) {
     const size_t o = g.accofs[m] + (l - m);
-    const double2 hp = acc[o], hm = acc[g.nacc + o];
+    const double2 ha = acc[o], hb = acc[g.nacc + o];
+    const double2 hp = make_double2(ha.x * fct, ha.y * fct),
+                  hm = make_double2(hb.x * fct, hb.y * fct);
     const double sg = (s & 1) ? -1.0 : 1.0;
     const double cx = sg * hm.x, cy = -sg * hm.y;  // (-1)^s conj(H-)
     e = make_double2(-0.5 * (hp.x + cx), -0.5 * (hp.y + cy));
@@ -1008,26 +1015,27 @@ void launch_leg2alm_spin(const ShtArgs &g, const double2 *Gt, double2 *acc,
 }
 
 void launch_store_alm_spin(const double2 *acc, const ShtArgs &g, void *alm,
-                           bool alm_f64, hipStream_t s) {
+                           bool alm_f64, double fct, hipStream_t s) {
   const dim3 grid(nblk((size_t)g.lmax + 1, 256), (unsigned)(g.mmax + 1));
   if (alm_f64)
     hipLaunchKernelGGL(k_store_alm_spin<double>, grid, dim3(256), 0, s, acc,
-                       g, (double *)alm);
+                       g, (double *)alm, fct);
   else
     hipLaunchKernelGGL(k_store_alm_spin<float>, grid, dim3(256), 0, s, acc, g,
-                       (float *)alm);
+                       (float *)alm, fct);
   DMX_HIP(hipGetLastError());
 }
 
 void launch_real_part(const double2 *z, size_t M, void *map, bool map_f64,
-                      int64_t mstride, int64_t cstride, hipStream_t s) {
+                      int64_t mstride, int64_t cstride, double fct,
+                      hipStream_t s) {
   if (!M) return;
   if (map_f64)
     hipLaunchKernelGGL(k_real_part<double>, dim3(nblk(M, 256)), dim3(256), 0,
-                       s, z, M, (double *)map, mstride, cstride);
+                       s, z, M, (double *)map, mstride, cstride, fct);
   else
     hipLaunchKernelGGL(k_real_part<float>, dim3(nblk(M, 256)), dim3(256), 0,
-                       s, z, M, (float *)map, mstride, cstride);
+                       s, z, M, (float *)map, mstride, cstride, fct);
   DMX_HIP(hipGetLastError());
 }
 
@@ -1044,14 +1052,14 @@ void launch_to_complex(const void *map, bool map_f64, int64_t mstride,
 }
 
 void launch_store_alm(const double2 *acc, const ShtArgs &g, void *alm,
-                      bool alm_f64, hipStream_t s) {
+                      bool alm_f64, double fct, hipStream_t s) {
   const dim3 grid(nblk((size_t)g.lmax + 1, 256), (unsigned)(g.mmax + 1));
   if (alm_f64)
     hipLaunchKernelGGL(k_store_alm<double>, grid, dim3(256), 0, s, acc, g,
-                       (double *)alm);
+                       (double *)alm, fct);
   else
     hipLaunchKernelGGL(k_store_alm<float>, grid, dim3(256), 0, s, acc, g,
-                       (float *)alm);
+                       (float *)alm, fct);
   DMX_HIP(hipGetLastError());
 }
 
